@@ -1,0 +1,101 @@
+// Oracle: C entry points used by the Python checker (ctypes).
+// TEST INFRASTRUCTURE ONLY (see ref_math.h).
+#include <cstdint>
+#include <cstring>
+
+#include "ref_world.h"
+
+using namespace nslref;
+
+extern "C" {
+
+void* nslo_create(void) { return new World(); }
+void nslo_destroy(void* w) { delete static_cast<World*>(w); }
+
+void nslo_set_options(void* wp, int enable_sleep, int enable_continuous, int enable_warm_starting, int enable_speculative) {
+  World* w = static_cast<World*>(wp);
+  w->enable_sleep = enable_sleep != 0;
+  w->enable_continuous = enable_continuous != 0;
+  w->enable_warm_starting = enable_warm_starting != 0;
+  w->enable_speculative = enable_speculative != 0;
+}
+
+void nslo_set_gravity(void* wp, float x, float y, float z) { static_cast<World*>(wp)->gravity = Vec3{x, y, z}; }
+
+int nslo_create_body(void* wp, int type, const double* pos, const float* rot, const float* v, const float* av,
+                     int enable_sleep, float linear_damping, float angular_damping, float gravity_scale) {
+  BodyDef d;
+  d.type = (BodyType)type;
+  d.position = Pos{pos[0], pos[1], pos[2]};
+  d.rotation = Quat{{rot[0], rot[1], rot[2]}, rot[3]};
+  d.linear_velocity = Vec3{v[0], v[1], v[2]};
+  d.angular_velocity = Vec3{av[0], av[1], av[2]};
+  d.enable_sleep = enable_sleep != 0;
+  d.linear_damping = linear_damping;
+  d.angular_damping = angular_damping;
+  d.gravity_scale = gravity_scale;
+  return static_cast<World*>(wp)->create_body(d);
+}
+
+static ShapeDef shape_def(float density, float friction, float restitution) {
+  ShapeDef s;
+  s.density = density;
+  s.friction = friction;
+  s.restitution = restitution;
+  return s;
+}
+
+int nslo_create_box(void* wp, int body, float hx, float hy, float hz, float density, float friction, float restitution) {
+  return static_cast<World*>(wp)->create_hull_shape(body, shape_def(density, friction, restitution), make_box_hull(hx, hy, hz));
+}
+int nslo_create_sphere(void* wp, int body, const float* c, float r, float density, float friction, float restitution) {
+  return static_cast<World*>(wp)->create_sphere_shape(body, shape_def(density, friction, restitution), Sphere{{c[0], c[1], c[2]}, r});
+}
+int nslo_create_capsule(void* wp, int body, const float* c1, const float* c2, float r, float density, float friction,
+                        float restitution) {
+  return static_cast<World*>(wp)->create_capsule_shape(body, shape_def(density, friction, restitution),
+                                                       Capsule{{c1[0], c1[1], c1[2]}, {c2[0], c2[1], c2[2]}, r});
+}
+
+void nslo_step(void* wp, float dt, int substeps) { static_cast<World*>(wp)->step(dt, substeps); }
+uint64_t nslo_state_hash(void* wp) { return static_cast<World*>(wp)->state_hash(); }
+int nslo_body_capacity(void* wp) { return (int)static_cast<World*>(wp)->bodies.size(); }
+int nslo_contact_count(void* wp) { return static_cast<World*>(wp)->contact_count(); }
+
+// Per body id: origin position (f64 x3), rotation (x,y,z,s), linear and angular velocity.
+void nslo_get_bodies(void* wp, double* pos, float* rot, float* v, float* av) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->bodies.size(); ++i) {
+    const BodySim& s = w->body_sim((int32_t)i);
+    BodyState st = w->body_state((int32_t)i);
+    pos[3 * i + 0] = s.transform.p.x;
+    pos[3 * i + 1] = s.transform.p.y;
+    pos[3 * i + 2] = s.transform.p.z;
+    rot[4 * i + 0] = s.transform.q.v.x;
+    rot[4 * i + 1] = s.transform.q.v.y;
+    rot[4 * i + 2] = s.transform.q.v.z;
+    rot[4 * i + 3] = s.transform.q.s;
+    v[3 * i + 0] = st.v.x; v[3 * i + 1] = st.v.y; v[3 * i + 2] = st.v.z;
+    av[3 * i + 0] = st.w.x; av[3 * i + 1] = st.w.y; av[3 * i + 2] = st.w.z;
+  }
+}
+
+// [pairs_created, destroyed, begin_touch, end_touch, recycled, sat_calls, sat_hits, fast_bodies, overflow]
+void nslo_get_stats(void* wp, int64_t* out) {
+  const StepStats& s = static_cast<World*>(wp)->stats;
+  out[0] = s.pairs_created; out[1] = s.contacts_destroyed; out[2] = s.begin_touch; out[3] = s.end_touch;
+  out[4] = s.recycled; out[5] = s.sat_calls; out[6] = s.sat_hits; out[7] = s.fast_bodies; out[8] = s.overflow_constraints;
+}
+
+// Graph coloring snapshot: per contact id, its color (-1 when not in the graph) and local index.
+int nslo_contact_capacity(void* wp) { return (int)static_cast<World*>(wp)->contacts.size(); }
+void nslo_get_contact_colors(void* wp, int32_t* color, int32_t* local, int32_t* alive) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->contacts.size(); ++i) {
+    color[i] = w->contacts[i].color_index;
+    local[i] = w->contacts[i].local_index;
+    alive[i] = w->contacts[i].contact_id != kNullIndex;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,397 @@
+// Oracle: CPU restatement of the reference's rigid-body step (box, sphere and
+// capsule shapes; static, kinematic and dynamic bodies; contacts only).
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU baseline.
+// Nothing under natsel_amd/ includes or links this code.
+//
+// Layouts and semantics follow the reference:
+//   BodyState/BodySim          src/simulation/world_runtime.rs:209-299
+//   Contact                    src/simulation/world_runtime.rs:517-555
+//   Manifold/ManifoldPoint     src/types/collision_types.rs:8-47
+//   SATCache/FeaturePair       src/types/collision_types.rs:97-121
+//   ConstraintGraph            src/simulation/constraint_graph.rs
+//   World::step                src/simulation/solver_integration.rs:807-1015
+// Islands, sleeping, CCD, joints, sensors, events and meshes are not restated: the
+// pinned scenes run with them inert (see the oracle's header in tests/).
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <unordered_set>
+#include <vector>
+
+#include "ref_math.h"
+
+namespace nslref {
+
+// src/types/shape_def.rs:70-85
+enum ShapeType : int32_t { kCapsule = 0, kCompound = 1, kHeight = 2, kHull = 3, kMesh = 4, kSphere = 5 };
+// src/types/body_def.rs (BodyType)
+enum BodyType : int32_t { kStatic = 0, kKinematic = 1, kDynamic = 2 };
+
+// src/simulation/world_runtime.rs:81-118
+namespace BF {
+constexpr uint32_t LOCK_LINEAR_X = 0x1, LOCK_LINEAR_Y = 0x2, LOCK_LINEAR_Z = 0x4;
+constexpr uint32_t LOCK_ANGULAR_X = 0x8, LOCK_ANGULAR_Y = 0x10, LOCK_ANGULAR_Z = 0x20;
+constexpr uint32_t IS_FAST = 0x40, IS_BULLET = 0x80, IS_SPEED_CAPPED = 0x100, HAD_TOI = 0x200;
+constexpr uint32_t ALLOW_FAST_ROTATION = 0x400, ENLARGE_BULLET_BOUNDS = 0x800;
+constexpr uint32_t DYNAMIC_FLAG = 0x1000, ENABLE_SLEEP = 0x2000, ENABLE_CONTACT_RECYCLING = 0x4000;
+constexpr uint32_t DIRTY_MASS = 0x8000, ENABLE_GYROSCOPIC_TORQUE = 0x10000;
+constexpr uint32_t FIXED_ROTATION = LOCK_ANGULAR_X | LOCK_ANGULAR_Y | LOCK_ANGULAR_Z;
+constexpr uint32_t TRANSIENT = IS_SPEED_CAPPED | HAD_TOI;
+}  // namespace BF
+
+// src/simulation/body_api.rs:22-40
+namespace CF {
+constexpr uint32_t TOUCHING = 0x1, HIT_EVENT = 0x2, ENABLE_CONTACT_EVENTS = 0x4, STATIC_FLAG = 0x8, RECYCLE = 0x10;
+constexpr uint32_t SIM_TOUCHING = 0x10000, SIM_DISJOINT = 0x20000, SIM_STARTED_TOUCHING = 0x40000;
+constexpr uint32_t SIM_STOPPED_TOUCHING = 0x80000, SIM_ENABLE_HIT_EVENT = 0x100000;
+constexpr uint32_t SIM_ENABLE_PRE_SOLVE = 0x200000, SIM_MESH_CONTACT = 0x400000;
+constexpr uint32_t RELATIVE_TRANSFORM_VALID = 0x800000, ENABLE_SPECULATIVE_POINTS = 0x1000000;
+}  // namespace CF
+
+struct BodyState {  // 64 bytes
+  Vec3 v; float pad1;
+  Vec3 w; float pad2;
+  Vec3 dp; uint32_t flags;
+  Quat dq;
+};
+static_assert(sizeof(BodyState) == 64, "BodyState layout");
+
+struct BodySim {
+  WXf transform;
+  Pos center;
+  uint32_t flags;
+  float min_extent;
+  Vec3 max_extent;
+  Quat rotation0;
+  Pos center0;
+  Vec3 local_center;
+  Vec3 force, torque;
+  float inv_mass;
+  Mat3 inertia, inv_inertia_local, inv_inertia_world;
+  float linear_damping, angular_damping, gravity_scale;
+  int32_t body_id;
+};
+
+struct Body {
+  int32_t set_index = kNullIndex, local_index = kNullIndex;
+  int32_t head_contact_key = kNullIndex, contact_count = 0;
+  int32_t head_shape_id = kNullIndex, shape_count = 0;
+  float sleep_threshold = 0, sleep_time = 0, sleep_velocity = 0, safety_factor = 0, mass = 0;
+  int32_t id = kNullIndex;
+  uint32_t flags = 0;
+  BodyType type = kStatic;
+};
+
+// Box hull with the reference's half-edge topology (src/collision/hull.rs:736-852).
+struct HalfEdge { uint8_t next, twin, origin, face; };
+struct Hull {
+  int vertex_count = 8, edge_count = 24, face_count = 6;
+  Vec3 points[8];
+  uint8_t vertex_edge[8];
+  HalfEdge edges[24];
+  uint8_t face_edge[6];
+  Plane planes[6];
+  float edge_cosines[12];
+  // structure-of-arrays copies, stride 8 (hull.rs:171-203)
+  float vx[8], vy[8], vz[8];
+  float nx[8], ny[8], nz[8];
+  AABB aabb;
+  float surface_area, volume, inner_radius;
+  Vec3 center;
+  Mat3 central_inertia;
+};
+Hull make_transformed_box_hull(float hx, float hy, float hz, Xf t);
+inline Hull make_box_hull(float hx, float hy, float hz) { return make_transformed_box_hull(hx, hy, hz, Xf{kZero3, kQuatIdentity}); }
+
+struct Sphere { Vec3 center; float radius; };
+struct Capsule { Vec3 c1, c2; float radius; };
+
+struct Filter { uint64_t category = 1, mask = ~0ull; int32_t group = 0; };
+
+struct Shape {
+  int32_t id = kNullIndex, body_id = kNullIndex;
+  ShapeType type = kHull;
+  int32_t hull = -1;  // index into World::hulls
+  Sphere sphere{};
+  Capsule capsule{};
+  float density = 1000.f, friction = 0.6f, restitution = 0.f, rolling_resistance = 0.f;
+  Vec3 tangent_velocity = kZero3;
+  Filter filter;
+  bool enable_speculative = true, enable_hit_events = false;
+  AABB aabb{};
+  float aabb_margin = 0;
+  int32_t next_shape_id = kNullIndex, prev_shape_id = kNullIndex;
+  // broad-phase proxy (tree type = body type)
+  int32_t proxy_type = -1;
+  bool proxy_moved = false;
+};
+
+struct FeaturePair { uint8_t owner1, index1, owner2, index2; };
+
+// src/types/collision_types.rs:5-47
+struct ManifoldPoint {
+  Vec3 anchor_a, anchor_b;
+  float separation, normal_impulse, total_normal_impulse, normal_velocity, base_separation;
+  uint32_t feature_id;
+  int32_t triangle_index;
+  bool persisted;
+};
+struct Manifold {
+  ManifoldPoint points[4];
+  Vec3 normal;
+  float twist_impulse;
+  Vec3 friction_impulse;
+  Vec3 rolling_impulse;
+  int32_t point_count;
+};
+
+enum SepFeature : uint8_t {
+  kInvalidAxis = 0, kBacksideAxis, kFaceAxisA, kFaceAxisB, kEdgePairAxis, kClosestPointsAxis,
+  kManualFaceAxisA, kManualFaceAxisB, kManualEdgePairAxis
+};
+struct SATCache { float separation; uint8_t type, index_a, index_b, hit; };
+
+// src/types/queries.rs SimplexCache
+struct SimplexCache { uint16_t count; uint8_t index_a[4], index_b[4]; };
+
+struct LocalPoint { Vec3 point; float separation; FeaturePair pair; int32_t triangle_index; };
+struct LocalManifold { Vec3 normal; LocalPoint pts[4]; int32_t point_count; };
+
+struct ContactEdge { int32_t body_id, prev_key, next_key; };
+
+struct Contact {
+  int32_t set_index = kNullIndex, color_index = kNullIndex, local_index = kNullIndex;
+  ContactEdge edges[2]{};
+  int32_t shape_id_a = kNullIndex, shape_id_b = kNullIndex, child_index = 0;
+  int32_t contact_id = kNullIndex;
+  int32_t encoded_a = kNullIndex, encoded_b = kNullIndex;
+  uint32_t flags = 0;
+  int32_t manifold_count = 0;
+  Manifold manifold{};
+  Quat cached_rotation_a{}, cached_rotation_b{};
+  Xf cached_relative_pose{};
+  float friction = 0;
+  SATCache sat_cache{};
+  SimplexCache simplex_cache{};
+  float restitution = 0, rolling_resistance = 0;
+  Vec3 tangent_velocity{};
+};
+
+// src/containers/id_pool.rs: LIFO free list
+struct IDPool {
+  std::vector<int32_t> free_list;
+  int32_t next_index = 0;
+  int32_t alloc() {
+    if (!free_list.empty()) { int32_t id = free_list.back(); free_list.pop_back(); return id; }
+    return next_index++;
+  }
+  void free(int32_t id) {
+    if (id == next_index) { next_index -= 1; return; }
+    free_list.push_back(id);
+  }
+};
+
+struct BitSet {
+  std::vector<uint64_t> words;
+  bool get(uint32_t i) const { uint32_t w = i >> 6; return w < words.size() && ((words[w] >> (i & 63)) & 1); }
+  void set_grow(uint32_t i) { uint32_t w = i >> 6; if (w >= words.size()) words.resize(w + 1, 0); words[w] |= 1ull << (i & 63); }
+  void clear(uint32_t i) { uint32_t w = i >> 6; if (w < words.size()) words[w] &= ~(1ull << (i & 63)); }
+};
+
+struct ContactSpec { int32_t contact_id; int32_t manifold_start; uint16_t manifold_count; };
+struct GraphColor {
+  BitSet body_set;
+  std::vector<int32_t> convex_contacts;
+  std::vector<ContactSpec> contacts;  // overflow (and mesh) contacts
+};
+
+struct Softness { float bias_rate, mass_scale, impulse_scale; };
+Softness make_soft(float hertz, float zeta, float h);
+
+// src/simulation/contact_solver.rs:19-115
+struct SymMat3 { float cxx, cxy, cxz, cyy, cyz, czz; };
+inline SymMat3 sym_from(Mat3 m) { return {m.cx.x, m.cx.y, m.cx.z, m.cy.y, m.cy.z, m.cz.z}; }
+inline Vec3 sym_mv(SymMat3 s, Vec3 a) {
+  return {s.cxx * a.x + (s.cxy * a.y + s.cxz * a.z), s.cxy * a.x + (s.cyy * a.y + s.cyz * a.z),
+          s.cxz * a.x + (s.cyz * a.y + s.czz * a.z)};
+}
+inline SymMat3 add_sym(SymMat3 a, SymMat3 b) {
+  return {a.cxx + b.cxx, a.cxy + b.cxy, a.cxz + b.cxz, a.cyy + b.cyy, a.cyz + b.cyz, a.czz + b.czz};
+}
+SymMat3 invert_sym(SymMat3 m);
+// contact_solver.rs:291-300
+inline Vec3 inv_rotate_vector_wide(Quat q, Vec3 a) {
+  Vec3 t0 = cross(q.v, a);
+  Vec3 t = {t0.x + t0.x, t0.y + t0.y, t0.z + t0.z};
+  Vec3 u = cross(q.v, t);
+  return {(a.x - q.s * t.x) + u.x, (a.y - q.s * t.y) + u.y, (a.z - q.s * t.z) + u.z};
+}
+
+struct ConstraintPoint {
+  Vec3 r_a, r_b;
+  float base_separation, relative_velocity, normal_impulse, total_normal_impulse, restitution_impulse, normal_mass, lever_arm;
+};
+struct ManifoldConstraint {
+  ConstraintPoint points[4];
+  int32_t point_count;
+  Vec3 normal, tangent1, tangent2, center_a, center_b;
+  float twist_mass, twist_impulse;
+  Mat2 tangent_mass;
+  Vec2 friction_impulse;
+  Vec3 rolling_impulse;
+  float tangent_velocity1, tangent_velocity2;
+};
+struct ContactConstraint {
+  int32_t contact_id, index_a, index_b;
+  float inv_mass_a, inv_mass_b;
+  SymMat3 inv_ia, inv_ib, rolling_mass;
+  Softness softness;
+  float friction, restitution, rolling_resistance;
+  bool is_static;
+  int32_t manifold_start, manifold_count;
+};
+// src/simulation/overflow_contact_solver.rs:17-33
+struct OverflowConstraint {
+  int32_t contact_id, index_a, index_b;
+  float inv_mass_a, inv_mass_b;
+  Mat3 inv_ia, inv_ib, rolling_mass;
+  Softness softness;
+  float friction, restitution, rolling_resistance;
+  int32_t manifold_start, manifold_count;
+};
+
+struct StepContext {
+  float dt = 0, inv_dt = 0, h = 0, inv_h = 0;
+  int sub_step_count = 1;
+  float max_linear_velocity = 0;
+  Softness contact_softness{}, static_softness{};
+  bool enable_warm_starting = true;
+};
+
+struct BodyDef {
+  BodyType type = kStatic;
+  Pos position{0, 0, 0};
+  Quat rotation = kQuatIdentity;
+  Vec3 linear_velocity = kZero3, angular_velocity = kZero3;
+  float linear_damping = 0, angular_damping = 0, gravity_scale = 1.0f;
+  float sleep_threshold = 0.05f, safety_factor = 0.5f;
+  uint32_t lock_flags = 0;
+  bool enable_sleep = true, is_bullet = false, allow_fast_rotation = false, enable_contact_recycling = true;
+};
+
+struct ShapeDef {
+  float density = 1000.f, friction = 0.6f, restitution = 0.f, rolling_resistance = 0.f;
+  Filter filter;
+  bool enable_speculative = true, invoke_contact_creation = true, enable_hit_events = false;
+};
+
+struct StepStats {
+  int64_t pairs_created = 0, contacts_destroyed = 0, begin_touch = 0, end_touch = 0;
+  int64_t recycled = 0, sat_calls = 0, sat_hits = 0, fast_bodies = 0, overflow_constraints = 0;
+};
+
+class World {
+ public:
+  World();
+  int32_t create_body(const BodyDef& def);
+  int32_t create_hull_shape(int32_t body, const ShapeDef& def, const Hull& hull);
+  int32_t create_sphere_shape(int32_t body, const ShapeDef& def, Sphere s);
+  int32_t create_capsule_shape(int32_t body, const ShapeDef& def, Capsule c);
+  void step(float dt, int sub_steps);
+  uint64_t state_hash() const;
+
+  const BodySim& body_sim(int32_t body_id) const;
+  BodyState body_state(int32_t body_id) const;
+  int32_t body_count() const { return body_pool.next_index - (int32_t)body_pool.free_list.size(); }
+  int32_t contact_count() const { return contact_pool.next_index - (int32_t)contact_pool.free_list.size(); }
+
+  // public world settings (world_runtime.rs:1243-1304)
+  Vec3 gravity{0.0f, -10.0f, 0.0f};
+  float restitution_threshold = 1.0f;
+  int restitution_iterations = 2;
+  bool enable_restitution_propagation = false;
+  float max_linear_speed = 400.0f;
+  float isotropic_inertia_tolerance = 0.2f;
+  float contact_speed = 3.0f, contact_hertz = 30.0f, contact_damping_ratio = 10.0f;
+  float contact_recycle_distance_ = contact_recycle_distance();
+  bool enable_sleep = true, enable_warm_starting = true, enable_continuous = true, enable_speculative = true;
+
+  StepStats stats;
+
+  // --- state (public so the checker can inspect intermediate results) ---
+  std::vector<Body> bodies;
+  IDPool body_pool;
+  std::vector<BodySim> static_sims, awake_sims;
+  std::vector<BodyState> awake_states;
+  std::vector<Shape> shapes;
+  IDPool shape_pool;
+  std::vector<AABB> fat_aabbs;
+  std::vector<Hull> hulls;
+  std::vector<Contact> contacts;
+  IDPool contact_pool;
+  std::vector<int32_t> awake_contact_indices;  // non-touching contacts of the awake set
+  std::vector<int32_t> disabled_contact_indices;
+  GraphColor colors[kGraphColorCount];
+  std::unordered_set<uint64_t> pair_set;
+  std::vector<ContactConstraint> color_constraints[kGraphColorCount];
+  std::vector<ManifoldConstraint> manifold_pool;
+  std::vector<OverflowConstraint> overflow_constraints;
+  std::vector<ManifoldConstraint> overflow_manifolds;
+  bool solve_any_restitution = false;
+  std::vector<uint64_t> last_pair_keys;  // sorted keys created by the last pair update
+
+ private:
+  void update_body_mass_data(int32_t body_id);
+  void sync_body_flags(int32_t body_id);
+  int32_t create_shape_internal(int32_t body_id, const ShapeDef& def, Shape shape);
+  AABB compute_shape_aabb(const Shape& s, Xf xf) const;
+  AABB compute_fat_shape_aabb(const Shape& s, WXf xf, float extra) const;
+  float compute_shape_margin(const Shape& s) const;
+  BodySim& sim_of(const Body& b);
+  const BodySim& sim_of(const Body& b) const;
+  int32_t encode_body_sim_index(const Body& b) const;
+
+  void update_broad_phase_pairs();
+  bool should_create_pair(int32_t a, int32_t b) const;
+  void create_contact(int32_t shape_a, int32_t shape_b, int32_t child);
+  void destroy_contact(int32_t contact_id);
+  void add_contact_to_graph(int32_t contact_id);
+  void remove_contact_from_graph(int32_t body_a, int32_t body_b, int32_t color, int32_t local, bool mesh);
+  void add_non_touching_contact(int32_t contact_id);
+  void remove_non_touching_contact(int32_t local_index);
+
+  void collide();
+  void collide_one_contact(int32_t contact_id, std::vector<uint8_t>& state_bits);
+  bool update_contact(int32_t contact_id, Vec3 lc_a, WXf xf_a, Vec3 lc_b, WXf xf_b);
+  bool compute_convex_manifold(Contact& c, const Shape& sa, WXf xf_a, const Shape& sb, WXf xf_b);
+
+  void prepare_contacts(const StepContext& ctx);
+  void prepare_overflow_contacts(Softness soft, Softness static_soft);
+  void prepare_contact_constraint(int32_t contact_id, ContactConstraint& out, int32_t manifold_start, Softness soft, Softness static_soft, float warm_start_scale);
+  void solve_stages(const StepContext& ctx);
+  void integrate_velocities(float h);
+  void integrate_positions(float h, float inv_dt);
+  void warm_start_color(int ci);
+  void push_color(int ci, const StepContext& ctx);
+  void relax_color(int ci, const StepContext& ctx);
+  void restitution_color(int ci, const StepContext& ctx);
+  void store_color(int ci);
+  void warm_start_overflow();
+  void push_overflow(const StepContext& ctx);
+  void relax_overflow(const StepContext& ctx);
+  void restitution_overflow(const StepContext& ctx);
+  void store_overflow();
+  void finalize_bodies(const StepContext& ctx);
+};
+
+// narrow phase (src/collision/convex_manifold.rs, convex_manifold_shapes.rs)
+void collide_hulls(LocalManifold& m, const Hull& a, const Hull& b, Xf b_to_a, SATCache& cache, int64_t* sat_calls, int64_t* sat_hits);
+void collide_spheres(LocalManifold& m, Sphere a, Sphere b, Xf b_to_a);
+void collide_capsule_and_sphere(LocalManifold& m, Capsule a, Sphere b, Xf b_to_a);
+void collide_capsules(LocalManifold& m, Capsule a, Capsule b, Xf b_to_a);
+void collide_hull_and_sphere(LocalManifold& m, const Hull& a, Sphere b, Xf b_to_a, SimplexCache& cache);
+void collide_hull_and_capsule(LocalManifold& m, const Hull& a, Capsule b, Xf b_to_a, SimplexCache& cache);
+uint32_t make_feature_id(FeaturePair p);
+
+}  // namespace nslref
