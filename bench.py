@@ -1,0 +1,171 @@
+"""Benchmark: body-substeps/s of the rigid-body step on the 10,000-body random pile.
+
+Workload (one GPU): 10,000 dynamic bodies (1/3 boxes, 1/3 spheres, 1/3 capsules,
+sizes U[0.25, 0.75] m, density 1000, friction U[0.4, 0.8]) dropped from seeded
+lattice cells of a 20 x 60 x 20 m region into a static 5-wall container; dt = 1/60,
+4 substeps, f32, continuous collision and sleeping off. A "step" is one World::step
+(pair update, narrow phase, constraint preparation, 4 soft-step substeps, impulse
+storage, finalize). W warmup steps run untimed, then K steps are timed with a device
+synchronise on both sides, as the reference harness times steps 1..N.
+
+value = dynamic bodies x substeps x K / seconds (inputs resident in HBM: the scene is
+uploaded before the timed region). The roofline object reports the dominant kernel
+class: algorithmic bytes (the byte model below) / its summed launch time from HIP
+events. cpu_baseline is the CPU oracle (a restatement of the reference step) timed on a
+bounded sample of the same workload on this host, one thread.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per item (distinct fields the reference formula reads plus
+# writes, f32 = 4 B, each body-state gather counted per constraint touch).
+# Per colored constraint with one manifold: base + per manifold + per point.
+BYTE_MODEL = {
+    "integrate_velocities": {"per_body": 124},
+    "integrate_positions": {"per_body": 112},
+    "warm_start": {"per_constraint": 64 + 72 + 56 + 48, "per_point": 28},
+    "push": {"per_constraint": 68 + 12 + 112 + 48, "per_point": 40},
+    "relax": {"per_constraint": 68 + 108 + 112 + 48, "per_point": 52},
+    "prepare": {"per_constraint": 268 + 452, "per_point": 0},
+    "store": {"per_constraint": 76, "per_point": 16},
+    "finalize": {"per_body": 400},
+    # contact record read and written (~300 B each way), both bodies' poses (2 x 64 B),
+    # both fat AABBs (2 x 24 B)
+    "collide": {"per_contact": 600 + 128 + 48},
+}
+
+
+def algorithmic_bytes(cls: str, ks: dict, substeps: int) -> float:
+    m = BYTE_MODEL.get(cls)
+    if m is None:
+        return 0.0
+    if "per_body" in m:
+        per_launch_bodies = ks["bodies"] / max(ks["steps"], 1)
+        return m["per_body"] * per_launch_bodies * ks["launches"][cls]
+    if "per_contact" in m:
+        return m["per_contact"] * ks["collided"]
+    reps = substeps if cls in ("warm_start", "push", "relax") else 1
+    return reps * (m["per_constraint"] * ks["constraints"] + m["per_point"] * ks["points"])
+
+
+def cpu_baseline(scene, seconds: float, max_steps: int) -> dict:
+    from oracle import pyoracle
+
+    w = pyoracle.load_scene(scene)
+    n = scene.dynamic_count
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < max_steps and time.perf_counter() - t0 < seconds:
+        w.step(scene.dt, scene.substeps)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {
+        "value": n * scene.substeps * steps / dt,
+        "unit": "body-substeps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"CPU oracle (C++ restatement of the reference step, g++ -O2, 1 thread), steps 1..{steps} of the same pile",
+        "steps": steps,
+        "seconds": dt,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--bodies", type=int, default=10_000)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from natsel_amd import scenes
+    from natsel_amd.world import load_scene
+
+    scene = scenes.pile(args.bodies)
+    world = load_scene(scene, device=0)
+    for _ in range(args.warmup):
+        world.step(scene.dt, scene.substeps)
+    world.reset_kernel_stats()
+    world.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    marks = []
+    for _ in range(args.steps):
+        world.step(scene.dt, scene.substeps)  # returns after the step's device work
+        marks.append(time.perf_counter())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    world.set_kernel_timing(False)
+    ks = world.kernel_stats()
+    n = scene.dynamic_count
+    value = n * scene.substeps * args.steps / elapsed
+
+    ms = ks["ms"]
+    dominant = max(ms, key=lambda k: ms[k])
+    dom_bytes = algorithmic_bytes(dominant, ks, scene.substeps)
+    dom_s = ms[dominant] / 1e3
+    achieved = dom_bytes / dom_s / 1e9 if dom_s > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "kernel": dominant,
+        "launches": ks["launches"][dominant],
+        "avg_launch_us": 1e3 * ms[dominant] / max(ks["launches"][dominant], 1),
+    }
+    out = {
+        "metric": "body-substeps/s (10k pile)",
+        "value": value,
+        "unit": "body-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded pile generator)",
+        "config": {"workload": f"random pile, {n} dynamic bodies (boxes/spheres/capsules), 5-wall container, dt 1/60, 4 substeps, "
+                               "continuous collision and sleep off", "bodies": n, "substeps": scene.substeps,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "kernel_ms": {k: round(v, 3) for k, v in ms.items()},
+        "contacts_collided_per_step": ks["collided"] / max(ks["steps"], 1),
+        "constraints_per_step": ks["constraints"] / max(ks["steps"], 1),
+        "points_per_step": ks["points"] / max(ks["steps"], 1),
+        "fast_bodies": ks["fast_bodies"],
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(scene, args.cpu_seconds, args.steps)
+        k = min(cb["steps"], len(marks))
+        # GPU rate over the same leading steps as the CPU sample, for a like-for-like ratio
+        cb["gpu_value_same_steps"] = n * scene.substeps * k / (marks[k - 1] - t0)
+        cb["gpu_over_cpu_same_steps"] = cb["gpu_value_same_steps"] / cb["value"]
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

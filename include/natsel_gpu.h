@@ -1,0 +1,132 @@
+/* natsel_gpu.h - C ABI of the MI355X (gfx950) rigid-body stepper.
+ *
+ * The reference engine has no FFI: its seam is the public World API. Each entry point
+ * below stands in for one World method so the GPU stepper is a drop-in for the CPU
+ * stepper (citations are to the reference sources):
+ *
+ *   nsl_world_create / destroy     World::new / Drop            src/simulation/world_runtime.rs:1162, :1799
+ *   nsl_world_set_options          public World fields          src/simulation/world_runtime.rs:1243-1304
+ *   nsl_create_body                World::create_body           src/simulation/world_runtime.rs:1506
+ *   nsl_create_box                 create_hull_shape(make_box_hull)  src/simulation/shape_creation.rs:559,
+ *                                                               src/collision/hull.rs:854
+ *   nsl_create_sphere              World::create_sphere_shape   src/simulation/shape_creation.rs:504
+ *   nsl_create_capsule             World::create_capsule_shape  src/simulation/shape_creation.rs:523
+ *   nsl_world_step                 World::step(dt, substeps)    src/simulation/solver_integration.rs:807
+ *   nsl_world_get_bodies           body_transform / body_linear_velocity / body_angular_velocity
+ *                                                               src/simulation/body_api.rs:186-262
+ *   nsl_world_state_hash           World::world_state_hash      src/simulation/body_queries.rs:432
+ *   nsl_world_get_contact_colors   constraint-graph colour of each contact (graph_tests.rs:152-172)
+ *   nsl_world_get_profile          World::profile               src/types/world_def.rs:158-184
+ *   nsl_world_*kernel_*            per-kernel device timing (no counterpart: the reference times phases
+ *                                  on the host, solver_stages.rs:551-586)
+ *
+ * Conventions: every call returns 0 on success or a negative status; on failure
+ * nsl_last_error() returns a message. Pointers are plain host memory owned by the
+ * caller and are copied during the call. One world per host thread; calls on a world
+ * are ordered on the world's own HIP stream and return after their device work (and
+ * any copy back) has finished. Rotations are quaternions (x, y, z, s).
+ */
+#ifndef NATSEL_GPU_H
+#define NATSEL_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSL_OK 0
+#define NSL_ERR_INVALID (-1)   /* bad argument or handle */
+#define NSL_ERR_DEVICE (-2)    /* HIP runtime error */
+#define NSL_ERR_INTERNAL (-3)  /* other failure */
+
+#define NSL_BODY_STATIC 0
+#define NSL_BODY_KINEMATIC 1
+#define NSL_BODY_DYNAMIC 2
+
+typedef struct nsl_world nsl_world;
+
+typedef struct nsl_options {
+  float gravity[3];                  /* default (0, -10, 0) */
+  float contact_hertz;               /* 30 */
+  float contact_damping_ratio;       /* 10 */
+  float contact_speed;               /* 3 m/s */
+  float max_linear_speed;            /* 400 m/s */
+  float restitution_threshold;       /* 1 m/s */
+  int32_t restitution_iterations;    /* 2 */
+  int32_t enable_sleep;              /* 1 */
+  int32_t enable_continuous;         /* 1: fast bodies are counted (continuous collision is not run) */
+  int32_t enable_warm_starting;      /* 1 */
+  int32_t enable_speculative;        /* 1 */
+} nsl_options;
+
+typedef struct nsl_body_def {
+  int32_t type;                      /* NSL_BODY_* */
+  double position[3];
+  float rotation[4];
+  float linear_velocity[3];
+  float angular_velocity[3];
+  float linear_damping, angular_damping, gravity_scale;
+  int32_t enable_sleep;
+} nsl_body_def;
+
+typedef struct nsl_material {
+  float density, friction, restitution, rolling_resistance;
+} nsl_material;
+
+typedef struct nsl_profile {
+  float pairs_ms, collide_ms, state_pass_ms, solve_ms, finalize_ms, step_ms;
+  int32_t contacts, touching, colors_used, overflow;
+  int32_t fast_bodies, sat_calls, sat_hits, recycled;
+} nsl_profile;
+
+/* Kernel classes of nsl_kernel_stats.ms / launches. */
+enum {
+  NSL_K_COLLIDE = 0, NSL_K_PREPARE, NSL_K_INTEGRATE_VELOCITIES, NSL_K_WARM_START, NSL_K_PUSH,
+  NSL_K_INTEGRATE_POSITIONS, NSL_K_RELAX, NSL_K_RESTITUTION, NSL_K_STORE, NSL_K_OVERFLOW, NSL_K_FINALIZE,
+  NSL_K_CLASS_COUNT
+};
+
+/* Cumulative device time per kernel class (HIP events, only while timing is on) and
+ * work counters since the last reset. */
+typedef struct nsl_kernel_stats {
+  double ms[NSL_K_CLASS_COUNT];
+  int64_t launches[NSL_K_CLASS_COUNT];
+  int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
+} nsl_kernel_stats;
+
+const char* nsl_last_error(void);
+void nsl_default_options(nsl_options* out);
+
+int nsl_world_create(int device, const nsl_options* options, nsl_world** out);
+int nsl_world_destroy(nsl_world* world);
+int nsl_world_set_options(nsl_world* world, const nsl_options* options);
+
+int nsl_create_body(nsl_world* world, const nsl_body_def* def, int32_t* out_body);
+int nsl_create_box(nsl_world* world, int32_t body, const float half_extents[3], const nsl_material* m, int32_t* out_shape);
+int nsl_create_sphere(nsl_world* world, int32_t body, const float center[3], float radius, const nsl_material* m,
+                      int32_t* out_shape);
+int nsl_create_capsule(nsl_world* world, int32_t body, const float c1[3], const float c2[3], float radius,
+                       const nsl_material* m, int32_t* out_shape);
+
+int nsl_world_step(nsl_world* world, float dt, int32_t sub_steps);
+int nsl_world_step_n(nsl_world* world, float dt, int32_t sub_steps, int32_t steps);
+
+int nsl_world_body_capacity(nsl_world* world, int32_t* out);
+/* Arrays sized by nsl_world_body_capacity: pos[3n] (f64), rot[4n], v[3n], w[3n]. */
+int nsl_world_get_bodies(nsl_world* world, double* pos, float* rot, float* v, float* w);
+int nsl_world_state_hash(nsl_world* world, uint64_t* out);
+int nsl_world_contact_capacity(nsl_world* world, int32_t* out);
+/* Arrays sized by nsl_world_contact_capacity. */
+int nsl_world_get_contact_colors(nsl_world* world, int32_t* color, int32_t* local, int32_t* alive);
+int nsl_world_get_profile(nsl_world* world, nsl_profile* out);
+/* Per-kernel timing adds one event pair per launch; the step's results do not change. */
+int nsl_world_set_kernel_timing(nsl_world* world, int32_t on);
+int nsl_world_reset_kernel_stats(nsl_world* world);
+int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NATSEL_GPU_H */

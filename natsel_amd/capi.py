@@ -1,0 +1,132 @@
+"""ctypes binding of include/natsel_gpu.h (libnatsel_gpu.so, built in-tree)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "lib", "libnatsel_gpu.so")
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("gravity", ctypes.c_float * 3),
+        ("contact_hertz", ctypes.c_float),
+        ("contact_damping_ratio", ctypes.c_float),
+        ("contact_speed", ctypes.c_float),
+        ("max_linear_speed", ctypes.c_float),
+        ("restitution_threshold", ctypes.c_float),
+        ("restitution_iterations", ctypes.c_int32),
+        ("enable_sleep", ctypes.c_int32),
+        ("enable_continuous", ctypes.c_int32),
+        ("enable_warm_starting", ctypes.c_int32),
+        ("enable_speculative", ctypes.c_int32),
+    ]
+
+
+class BodyDef(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("position", ctypes.c_double * 3),
+        ("rotation", ctypes.c_float * 4),
+        ("linear_velocity", ctypes.c_float * 3),
+        ("angular_velocity", ctypes.c_float * 3),
+        ("linear_damping", ctypes.c_float),
+        ("angular_damping", ctypes.c_float),
+        ("gravity_scale", ctypes.c_float),
+        ("enable_sleep", ctypes.c_int32),
+    ]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [
+        ("density", ctypes.c_float),
+        ("friction", ctypes.c_float),
+        ("restitution", ctypes.c_float),
+        ("rolling_resistance", ctypes.c_float),
+    ]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("pairs_ms", "collide_ms", "state_pass_ms", "solve_ms", "finalize_ms", "step_ms")] + [
+        (n, ctypes.c_int32)
+        for n in ("contacts", "touching", "colors_used", "overflow", "fast_bodies", "sat_calls", "sat_hits", "recycled")
+    ]
+
+
+KERNEL_CLASSES = ["collide", "prepare", "integrate_velocities", "warm_start", "push", "integrate_positions", "relax",
+                  "restitution", "store", "overflow", "finalize"]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * len(KERNEL_CLASSES)), ("launches", ctypes.c_int64 * len(KERNEL_CLASSES))] + [
+        (n, ctypes.c_int64)
+        for n in ("steps", "bodies", "collided", "constraints", "points", "overflow_points", "recycled", "sat_calls",
+                  "sat_hits", "fast_bodies")
+    ]
+
+
+# Every entry point declared in include/natsel_gpu.h, with its argument types.
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_F = ctypes.c_float
+SIGNATURES = {
+    "nsl_last_error": (ctypes.c_char_p, []),
+    "nsl_default_options": (None, [ctypes.POINTER(Options)]),
+    "nsl_world_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Options), ctypes.POINTER(_P)]),
+    "nsl_world_destroy": (ctypes.c_int, [_P]),
+    "nsl_world_set_options": (ctypes.c_int, [_P, ctypes.POINTER(Options)]),
+    "nsl_create_body": (ctypes.c_int, [_P, ctypes.POINTER(BodyDef), ctypes.POINTER(_I)]),
+    "nsl_create_box": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(Material), ctypes.POINTER(_I)]),
+    "nsl_create_sphere": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), _F, ctypes.POINTER(Material), ctypes.POINTER(_I)]),
+    "nsl_create_capsule": (
+        ctypes.c_int,
+        [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), _F, ctypes.POINTER(Material), ctypes.POINTER(_I)],
+    ),
+    "nsl_world_step": (ctypes.c_int, [_P, _F, _I]),
+    "nsl_world_step_n": (ctypes.c_int, [_P, _F, _I, _I]),
+    "nsl_world_body_capacity": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "nsl_world_get_bodies": (
+        ctypes.c_int,
+        [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)],
+    ),
+    "nsl_world_state_hash": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "nsl_world_contact_capacity": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "nsl_world_get_contact_colors": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "nsl_world_get_profile": (ctypes.c_int, [_P, ctypes.POINTER(Profile)]),
+    "nsl_world_set_kernel_timing": (ctypes.c_int, [_P, _I]),
+    "nsl_world_reset_kernel_stats": (ctypes.c_int, [_P]),
+    "nsl_world_get_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStats)]),
+}
+
+
+def load_library():
+    """Load the in-tree HIP library. Raises if it has not been built: there is no
+    other implementation to fall back to."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (make -C natsel_amd/csrc)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+class NatselError(RuntimeError):
+    pass
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = _LIB.nsl_last_error().decode() if _LIB is not None else "?"
+        raise NatselError(f"natsel_gpu status {status}: {msg}")

@@ -1,0 +1,237 @@
+// extern "C" boundary of the GPU stepper (declared in include/natsel_gpu.h).
+#include <exception>
+#include <new>
+#include <string>
+
+#include "../../include/natsel_gpu.h"
+#include "nsl_world.h"
+
+struct nsl_world {
+  nsl::GpuWorld* impl;
+};
+
+namespace {
+thread_local std::string g_error;
+
+int fail(int code, const char* what) {
+  g_error = what;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return NSL_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(NSL_ERR_DEVICE, "out of memory");
+  } catch (const std::exception& e) {
+    return fail(NSL_ERR_DEVICE, e.what());
+  } catch (...) {
+    return fail(NSL_ERR_INTERNAL, "unknown failure");
+  }
+}
+
+void apply(nsl::WorldOptions& o, const nsl_options& in) {
+  o.gravity = nsl::Vec3{in.gravity[0], in.gravity[1], in.gravity[2]};
+  o.contact_hertz = in.contact_hertz;
+  o.contact_damping_ratio = in.contact_damping_ratio;
+  o.contact_speed = in.contact_speed;
+  o.max_linear_speed = in.max_linear_speed;
+  o.restitution_threshold = in.restitution_threshold;
+  o.restitution_iterations = in.restitution_iterations;
+  o.enable_sleep = in.enable_sleep != 0;
+  o.enable_continuous = in.enable_continuous != 0;
+  o.enable_warm_starting = in.enable_warm_starting != 0;
+  o.enable_speculative = in.enable_speculative != 0;
+}
+
+nsl::Material material(const nsl_material* m) {
+  nsl::Material out;
+  if (m) {
+    out.density = m->density;
+    out.friction = m->friction;
+    out.restitution = m->restitution;
+    out.rolling_resistance = m->rolling_resistance;
+  }
+  return out;
+}
+
+bool valid_body(nsl_world* w, int32_t body) { return w && body >= 0 && body < w->impl->body_capacity(); }
+}  // namespace
+
+extern "C" {
+
+const char* nsl_last_error(void) { return g_error.c_str(); }
+
+void nsl_default_options(nsl_options* out) {
+  if (!out) return;
+  nsl::WorldOptions o;
+  out->gravity[0] = o.gravity.x;
+  out->gravity[1] = o.gravity.y;
+  out->gravity[2] = o.gravity.z;
+  out->contact_hertz = o.contact_hertz;
+  out->contact_damping_ratio = o.contact_damping_ratio;
+  out->contact_speed = o.contact_speed;
+  out->max_linear_speed = o.max_linear_speed;
+  out->restitution_threshold = o.restitution_threshold;
+  out->restitution_iterations = o.restitution_iterations;
+  out->enable_sleep = o.enable_sleep;
+  out->enable_continuous = o.enable_continuous;
+  out->enable_warm_starting = o.enable_warm_starting;
+  out->enable_speculative = o.enable_speculative;
+}
+
+int nsl_world_create(int device, const nsl_options* options, nsl_world** out) {
+  if (!out) return fail(NSL_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  return guarded([&] {
+    nsl_world* w = new nsl_world{new nsl::GpuWorld(device)};
+    if (options) apply(w->impl->options, *options);
+    *out = w;
+  });
+}
+
+int nsl_world_destroy(nsl_world* world) {
+  if (!world) return fail(NSL_ERR_INVALID, "null world");
+  return guarded([&] {
+    delete world->impl;
+    delete world;
+  });
+}
+
+int nsl_world_set_options(nsl_world* world, const nsl_options* options) {
+  if (!world || !options) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { apply(world->impl->options, *options); });
+}
+
+int nsl_create_body(nsl_world* world, const nsl_body_def* def, int32_t* out_body) {
+  if (!world || !def || !out_body) return fail(NSL_ERR_INVALID, "null argument");
+  if (def->type < NSL_BODY_STATIC || def->type > NSL_BODY_DYNAMIC) return fail(NSL_ERR_INVALID, "bad body type");
+  return guarded([&] {
+    nsl::BodyDesc d;
+    d.type = def->type;
+    d.position = nsl::Pos{def->position[0], def->position[1], def->position[2]};
+    d.rotation = nsl::Quat{{def->rotation[0], def->rotation[1], def->rotation[2]}, def->rotation[3]};
+    d.linear_velocity = nsl::Vec3{def->linear_velocity[0], def->linear_velocity[1], def->linear_velocity[2]};
+    d.angular_velocity = nsl::Vec3{def->angular_velocity[0], def->angular_velocity[1], def->angular_velocity[2]};
+    d.linear_damping = def->linear_damping;
+    d.angular_damping = def->angular_damping;
+    d.gravity_scale = def->gravity_scale;
+    d.enable_sleep = def->enable_sleep != 0;
+    *out_body = world->impl->create_body(d);
+  });
+}
+
+int nsl_create_box(nsl_world* world, int32_t body, const float h[3], const nsl_material* m, int32_t* out_shape) {
+  if (!valid_body(world, body) || !h || !out_shape) return fail(NSL_ERR_INVALID, "bad body or null argument");
+  return guarded([&] { *out_shape = world->impl->create_box(body, nsl::Vec3{h[0], h[1], h[2]}, material(m)); });
+}
+
+int nsl_create_sphere(nsl_world* world, int32_t body, const float c[3], float radius, const nsl_material* m,
+                      int32_t* out_shape) {
+  if (!valid_body(world, body) || !c || !out_shape) return fail(NSL_ERR_INVALID, "bad body or null argument");
+  return guarded([&] { *out_shape = world->impl->create_sphere(body, nsl::Vec3{c[0], c[1], c[2]}, radius, material(m)); });
+}
+
+int nsl_create_capsule(nsl_world* world, int32_t body, const float c1[3], const float c2[3], float radius,
+                       const nsl_material* m, int32_t* out_shape) {
+  if (!valid_body(world, body) || !c1 || !c2 || !out_shape) return fail(NSL_ERR_INVALID, "bad body or null argument");
+  return guarded([&] {
+    *out_shape = world->impl->create_capsule(body, nsl::Vec3{c1[0], c1[1], c1[2]}, nsl::Vec3{c2[0], c2[1], c2[2]}, radius,
+                                             material(m));
+  });
+}
+
+int nsl_world_step(nsl_world* world, float dt, int32_t sub_steps) {
+  if (!world || !(dt >= 0.0f)) return fail(NSL_ERR_INVALID, "null world or negative time step");
+  return guarded([&] { world->impl->step(dt, sub_steps); });
+}
+
+int nsl_world_step_n(nsl_world* world, float dt, int32_t sub_steps, int32_t steps) {
+  if (!world || !(dt >= 0.0f) || steps < 0) return fail(NSL_ERR_INVALID, "bad argument");
+  return guarded([&] {
+    for (int32_t i = 0; i < steps; ++i) world->impl->step(dt, sub_steps);
+  });
+}
+
+int nsl_world_body_capacity(nsl_world* world, int32_t* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  *out = world->impl->body_capacity();
+  return NSL_OK;
+}
+
+int nsl_world_get_bodies(nsl_world* world, double* pos, float* rot, float* v, float* w) {
+  if (!world || !pos || !rot || !v || !w) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { world->impl->get_bodies(pos, rot, v, w); });
+}
+
+int nsl_world_state_hash(nsl_world* world, uint64_t* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { *out = world->impl->state_hash(); });
+}
+
+int nsl_world_contact_capacity(nsl_world* world, int32_t* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  *out = world->impl->contact_capacity();
+  return NSL_OK;
+}
+
+int nsl_world_get_contact_colors(nsl_world* world, int32_t* color, int32_t* local, int32_t* alive) {
+  if (!world || !color || !local || !alive) return fail(NSL_ERR_INVALID, "null argument");
+  world->impl->get_contact_colors(color, local, alive);
+  return NSL_OK;
+}
+
+int nsl_world_get_profile(nsl_world* world, nsl_profile* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  const nsl::StepProfile& p = world->impl->profile();
+  out->pairs_ms = p.pairs;
+  out->collide_ms = p.collide;
+  out->state_pass_ms = p.state_pass;
+  out->solve_ms = p.solve;
+  out->finalize_ms = p.finalize;
+  out->step_ms = p.step;
+  out->contacts = p.contacts;
+  out->touching = p.touching;
+  out->colors_used = p.colors_used;
+  out->overflow = p.overflow;
+  out->fast_bodies = p.fast_bodies;
+  out->sat_calls = p.sat_calls;
+  out->sat_hits = p.sat_hits;
+  out->recycled = p.recycled;
+  return NSL_OK;
+}
+
+int nsl_world_set_kernel_timing(nsl_world* world, int32_t on) {
+  if (!world) return fail(NSL_ERR_INVALID, "null world");
+  return guarded([&] { world->impl->set_kernel_timing(on != 0); });
+}
+
+int nsl_world_reset_kernel_stats(nsl_world* world) {
+  if (!world) return fail(NSL_ERR_INVALID, "null world");
+  return guarded([&] { world->impl->reset_kernel_stats(); });
+}
+
+int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  static_assert((int)NSL_K_CLASS_COUNT == (int)nsl::kKernelClassCount, "kernel class count");
+  const auto& k = world->impl->kernel_stats();
+  for (int i = 0; i < NSL_K_CLASS_COUNT; ++i) {
+    out->ms[i] = k.ms[i];
+    out->launches[i] = k.launches[i];
+  }
+  out->steps = k.steps;
+  out->bodies = k.bodies;
+  out->collided = k.collided;
+  out->constraints = k.constraints;
+  out->points = k.points;
+  out->overflow_points = k.overflow_points;
+  out->recycled = k.recycled;
+  out->sat_calls = k.sat_calls;
+  out->sat_hits = k.sat_hits;
+  out->fast_bodies = k.fast_bodies;
+  return NSL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+// Host side of the GPU stepper: the world/body/shape/step API of the reference's
+// World (src/simulation/world_runtime.rs:942), its integer bookkeeping (broad-phase
+// pair list, contact lifecycle, graph colouring) and the launch schedule of one step.
+// Float work of the step runs on the device (nsl_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "nsl_device.h"
+
+namespace nsl {
+
+struct BodyDesc {
+  int32_t type = kBodyStatic;
+  Pos position{0, 0, 0};
+  Quat rotation{{0, 0, 0}, 1};
+  Vec3 linear_velocity{0, 0, 0}, angular_velocity{0, 0, 0};
+  float linear_damping = 0, angular_damping = 0, gravity_scale = 1.0f;
+  float sleep_threshold = 0.05f, safety_factor = 0.5f;
+  uint32_t lock_flags = 0;
+  bool enable_sleep = true, is_bullet = false, allow_fast_rotation = false, enable_contact_recycling = true;
+};
+
+struct Material {
+  float density = 1000.0f, friction = 0.6f, restitution = 0.0f, rolling_resistance = 0.0f;
+  bool enable_speculative = true, enable_hit_events = false;
+  uint64_t category_bits = 1, mask_bits = ~0ull;
+  int32_t group_index = 0;
+};
+
+struct WorldOptions {
+  Vec3 gravity{0.0f, -10.0f, 0.0f};
+  float restitution_threshold = 1.0f;
+  int32_t restitution_iterations = 2;
+  bool enable_restitution_propagation = false;
+  float max_linear_speed = 400.0f;
+  float contact_speed = 3.0f, contact_hertz = 30.0f, contact_damping_ratio = 10.0f;
+  float isotropic_inertia_tolerance = 0.2f;
+  bool enable_sleep = true, enable_warm_starting = true, enable_continuous = true, enable_speculative = true;
+};
+
+// Wall-clock phases of the last step, in milliseconds (types/world_def.rs:158-184).
+struct StepProfile {
+  float pairs, collide, state_pass, solve, finalize, step;
+  int32_t contacts, touching, colors_used, overflow;
+  int32_t fast_bodies, sat_calls, sat_hits, recycled;
+};
+
+class GpuWorld {
+ public:
+  explicit GpuWorld(int device);
+  ~GpuWorld();
+  GpuWorld(const GpuWorld&) = delete;
+  GpuWorld& operator=(const GpuWorld&) = delete;
+
+  WorldOptions options;
+
+  int32_t create_body(const BodyDesc& desc);
+  int32_t create_box(int32_t body, Vec3 half_extents, const Material& m);
+  int32_t create_sphere(int32_t body, Vec3 center, float radius, const Material& m);
+  int32_t create_capsule(int32_t body, Vec3 c1, Vec3 c2, float radius, const Material& m);
+
+  void step(float dt, int32_t sub_steps);
+
+  int32_t body_capacity() const { return (int32_t)bodies_.size(); }
+  int32_t contact_count() const { return contact_pool_next_ - (int32_t)contact_free_.size(); }
+  // Per body id: origin position, rotation (x, y, z, s), linear and angular velocity.
+  void get_bodies(double* pos, float* rot, float* v, float* w);
+  uint64_t state_hash();
+  // Per contact id: colour (-1 when not in the graph), local index and liveness.
+  int32_t contact_capacity() const { return (int32_t)hcontacts_.size(); }
+  void get_contact_colors(int32_t* color, int32_t* local, int32_t* alive) const;
+  const StepProfile& profile() const { return profile_; }
+
+  // Cumulative GPU time per kernel class (HIP events on the world's stream, only while
+  // timing is on) and device work counters, since the last reset.
+  struct KernelStats {
+    double ms[kKernelClassCount];
+    int64_t launches[kKernelClassCount];
+    int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
+  };
+  void set_kernel_timing(bool on);
+  const KernelStats& kernel_stats() const { return kstats_; }
+  void reset_kernel_stats();
+
+ private:
+  struct HBody {
+    int32_t type = kBodyStatic, set = 0, local = -1, head_shape = -1;
+    uint32_t flags = 0;
+  };
+  struct HShape {
+    int32_t id = -1, body = -1, kind = kShapeHull, hull = -1;
+    uint64_t category = 1, mask = ~0ull;
+    int32_t group = 0;
+    float density = 0, restitution = 0, rolling_resistance = 0, radius = 0;
+    bool speculative = true;
+    bool proxy_moved = false;
+    Aabb fat{};
+  };
+  struct HContact {
+    int32_t shape_a = -1, shape_b = -1, color = -1, local = -1, set = -1;
+    int32_t alive = 0;
+    float restitution = 0;
+  };
+  struct ColorSet {
+    std::vector<uint64_t> bits;
+    std::vector<int32_t> convex;
+    std::vector<int32_t> overflow;
+    bool get(int32_t b) const { uint32_t w = (uint32_t)b >> 6; return w < bits.size() && ((bits[w] >> (b & 63)) & 1); }
+    void set(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w >= bits.size()) bits.resize(w + 1, 0); bits[w] |= 1ull << (b & 63); }
+    void clear(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w < bits.size()) bits[w] &= ~(1ull << (b & 63)); }
+  };
+
+  int32_t add_shape(int32_t body, Shape shape, const Material& m, float radius);
+  void update_mass(int32_t body);
+  Aabb shape_local_aabb(const Shape& s, Quat q) const;
+  int32_t encode(const HBody& b) const { return b.set == 2 ? b.local : -(b.local + 2); }
+
+  void upload_bodies();
+  void pull_bodies();
+  template <class T> void ensure(T*& ptr, size_t& cap, size_t n, bool keep);
+
+  void update_pairs();
+  void create_contact(int32_t a, int32_t b);
+  void destroy_contact(int32_t id);
+  void add_to_graph(int32_t id);
+  void remove_from_graph(int32_t body_a, int32_t body_b, int32_t color, int32_t local);
+  void state_pass();
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+  KernelTimer timer_{};
+  bool timing_ = false;
+  KernelStats kstats_{};
+  int32_t counters_host_[8] = {};
+  int32_t counters_base_[8] = {};
+
+  std::vector<HBody> bodies_;
+  std::vector<BodySim> awake_host_, static_host_;
+  std::vector<BodyState> states_host_;
+  std::vector<Shape> shapes_;
+  std::vector<HShape> hshapes_;
+  std::vector<BoxHull> hulls_;
+  std::vector<Mat3> hull_inertia_;
+  std::vector<float> hull_volume_;
+  std::vector<HContact> hcontacts_;
+  std::vector<int32_t> contact_free_;
+  int32_t contact_pool_next_ = 0;
+  std::vector<int32_t> non_touching_;
+  ColorSet colors_[kColorCount];
+  std::unordered_set<uint64_t> pair_set_;
+  std::vector<int32_t> init_ids_;
+  std::vector<Contact> init_records_;
+  bool bodies_dirty_ = true;   // host copies are newer than the device arrays
+  bool device_newer_ = false;  // device arrays are newer than the host copies
+  bool shapes_dirty_ = true;
+
+  DeviceWorld d_{};
+  size_t cap_sims_ = 0, cap_states_ = 0, cap_static_ = 0, cap_shapes_ = 0, cap_fat_ = 0, cap_moved_ = 0, cap_hulls_ = 0;
+  size_t cap_contacts_ = 0, cap_change_ = 0, cap_constraints_ = 0, cap_overflow_ = 0, cap_collide_ = 0, cap_solve_ = 0;
+  size_t cap_overflow_ids_ = 0, cap_init_ids_ = 0, cap_init_rec_ = 0;
+  int32_t* d_init_ids_ = nullptr;
+  Contact* d_init_records_ = nullptr;
+  std::vector<uint8_t> change_host_;
+  std::vector<uint8_t> moved_host_;
+  std::vector<Aabb> fat_host_;
+  StepProfile profile_{};
+};
+
+}  // namespace nsl

@@ -107,3 +107,87 @@ def many_pyramids(base_count: int = 10, rows: int = 14, cols: int = 14) -> Scene
                     sc.add_shape(ShapeSpec(b, BOX, (float(extent),) * 3, density=100.0))
         z = f32(z + delta_z)
     return sc
+
+
+class XorShift32:
+    """Seeded xorshift32 (Marsaglia) for the synthetic scenes."""
+
+    def __init__(self, seed: int):
+        self.x = (seed & 0xFFFFFFFF) or 0x9E3779B9
+
+    def next_u32(self) -> int:
+        x = self.x
+        x ^= (x << 13) & 0xFFFFFFFF
+        x ^= x >> 17
+        x ^= (x << 5) & 0xFFFFFFFF
+        self.x = x & 0xFFFFFFFF
+        return self.x
+
+    def uniform(self, lo: float, hi: float) -> np.float32:
+        u = f32((self.next_u32() >> 8) * (1.0 / 16777216.0))
+        return f32(lo) + (f32(hi) - f32(lo)) * u
+
+    def below(self, n: int) -> int:
+        return self.next_u32() % n
+
+
+def _container(sc: Scene, half_x: float, half_z: float, height: float) -> None:
+    """Static floor (top at y = 0) and four walls around a half_x x half_z footprint."""
+    g = sc.add_body(BodySpec(STATIC, (0.0, -0.5, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (half_x + 1.0, 0.5, half_z + 1.0)))
+    hy = height / 2.0 + 1.0
+    for sx in (-1.0, 1.0):
+        w = sc.add_body(BodySpec(STATIC, (sx * (half_x + 0.5), hy - 1.0, 0.0)))
+        sc.add_shape(ShapeSpec(w, BOX, (0.5, hy, half_z + 1.0)))
+    for sz in (-1.0, 1.0):
+        w = sc.add_body(BodySpec(STATIC, (0.0, hy - 1.0, sz * (half_z + 0.5))))
+        sc.add_shape(ShapeSpec(w, BOX, (half_x + 1.0, hy, 0.5)))
+
+
+def pile(count: int = 10_000, seed: int = 0x5EED, size_x: int = 20, size_y: int = 60, size_z: int = 20,
+         base_y: float = 1.0) -> Scene:
+    """Random pile: `count` dynamic bodies, a third each boxes, spheres and capsules.
+
+    Sizes s ~ U[0.25, 0.75] m are full extents: box edge lengths (per axis), sphere
+    diameter, capsule total length (radius s/4, segment s/2 along local y). Density
+    1000, friction ~ U[0.4, 0.8]. Bodies take distinct cells of a 1 m lattice over a
+    size_x x size_y x size_z region above the floor (seeded shuffle), jittered by at
+    most (1 - s)/2 so no two start overlapping. The container is a static floor and
+    four walls. Continuous collision and sleeping are off.
+    """
+    sc = Scene(name=f"pile_{count}", enable_sleep=False, enable_continuous=False)
+    _container(sc, size_x / 2.0, size_z / 2.0, float(size_y) + base_y + 2.0)
+    rng = XorShift32(seed)
+    cells = size_x * size_y * size_z
+    if count > cells:
+        raise ValueError("more bodies than lattice cells")
+    order = list(range(cells))
+    for i in range(cells - 1, 0, -1):  # Fisher-Yates
+        j = rng.below(i + 1)
+        order[i], order[j] = order[j], order[i]
+    n_box = count - 2 * (count // 3)
+    for k in range(count):
+        cell = order[k]
+        cx, rest = cell % size_x, cell // size_x
+        cy, cz = rest % size_y, rest // size_y
+        kind = BOX if k < n_box else (SPHERE if k < n_box + count // 3 else CAPSULE)
+        friction = float(rng.uniform(0.4, 0.8))
+        if kind == BOX:
+            s = [rng.uniform(0.25, 0.75) for _ in range(3)]
+            smax = max(s)
+        else:
+            s = rng.uniform(0.25, 0.75)
+            smax = s
+        slack = (f32(1.0) - f32(smax)) * f32(0.5)
+        jitter = [float(rng.uniform(-1.0, 1.0) * slack) for _ in range(3)]
+        pos = (-size_x / 2.0 + cx + 0.5 + jitter[0], base_y + cy + 0.5 + jitter[1], -size_z / 2.0 + cz + 0.5 + jitter[2])
+        pos = tuple(float(f32(p)) for p in pos)
+        b = sc.add_body(BodySpec(DYNAMIC, pos, enable_sleep=False))
+        if kind == BOX:
+            sc.add_shape(ShapeSpec(b, BOX, tuple(float(f32(v) * f32(0.5)) for v in s), density=1000.0, friction=friction))
+        elif kind == SPHERE:
+            sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, float(f32(s) * f32(0.5))), density=1000.0, friction=friction))
+        else:
+            q = float(f32(s) * f32(0.25))
+            sc.add_shape(ShapeSpec(b, CAPSULE, (0.0, -q, 0.0, 0.0, q, 0.0, q), density=1000.0, friction=friction))
+    return sc

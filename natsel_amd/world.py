@@ -1,0 +1,229 @@
+"""Python mirror of the reference's World API over the HIP library.
+
+Names follow the reference (src/simulation/world_runtime.rs, shape_creation.rs,
+body_api.rs, solver_integration.rs) so code written against the CPU stepper reads the
+same: World.create_body(BodyDef), create_hull_shape(body, ShapeDef, make_box_hull(...)),
+create_sphere_shape, create_capsule_shape, step(time_step, sub_step_count),
+world_state_hash(). Every call goes through include/natsel_gpu.h; nothing here
+computes physics.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .scenes import BOX, CAPSULE, DYNAMIC, KINEMATIC, SPHERE, STATIC, Scene  # noqa: F401
+
+
+@dataclass
+class BodyDef:  # src/types/body_def.rs:80-101 defaults
+    type: int = STATIC
+    position: tuple = (0.0, 0.0, 0.0)
+    rotation: tuple = (0.0, 0.0, 0.0, 1.0)
+    linear_velocity: tuple = (0.0, 0.0, 0.0)
+    angular_velocity: tuple = (0.0, 0.0, 0.0)
+    linear_damping: float = 0.0
+    angular_damping: float = 0.0
+    gravity_scale: float = 1.0
+    enable_sleep: bool = True
+
+
+@dataclass
+class ShapeDef:  # src/types/shape_def.rs:114-135 defaults
+    density: float = 1000.0
+    friction: float = 0.6
+    restitution: float = 0.0
+    rolling_resistance: float = 0.0
+
+
+@dataclass
+class BoxHull:
+    """make_box_hull(hx, hy, hz) (src/collision/hull.rs:854)."""
+    hx: float
+    hy: float
+    hz: float
+
+
+def make_box_hull(hx: float, hy: float, hz: float) -> BoxHull:
+    return BoxHull(hx, hy, hz)
+
+
+@dataclass
+class Sphere:
+    center: tuple
+    radius: float
+
+
+@dataclass
+class Capsule:
+    center1: tuple
+    center2: tuple
+    radius: float
+
+
+@dataclass
+class WorldOptions:  # world_runtime.rs:1243-1304
+    gravity: tuple = (0.0, -10.0, 0.0)
+    contact_hertz: float = 30.0
+    contact_damping_ratio: float = 10.0
+    contact_speed: float = 3.0
+    max_linear_speed: float = 400.0
+    restitution_threshold: float = 1.0
+    restitution_iterations: int = 2
+    enable_sleep: bool = True
+    enable_continuous: bool = True
+    enable_warm_starting: bool = True
+    enable_speculative: bool = True
+
+
+def _material(d: ShapeDef) -> capi.Material:
+    return capi.Material(d.density, d.friction, d.restitution, d.rolling_resistance)
+
+
+class World:
+    def __init__(self, device: int = 0, options: WorldOptions | None = None):
+        self._lib = capi.load_library()
+        self._opts = options or WorldOptions()
+        handle = ctypes.c_void_p()
+        capi.check(self._lib.nsl_world_create(device, ctypes.byref(self._c_options()), ctypes.byref(handle)))
+        self._h = handle
+
+    def _c_options(self) -> capi.Options:
+        o = self._opts
+        c = capi.Options()
+        c.gravity[:] = o.gravity
+        c.contact_hertz, c.contact_damping_ratio, c.contact_speed = o.contact_hertz, o.contact_damping_ratio, o.contact_speed
+        c.max_linear_speed, c.restitution_threshold = o.max_linear_speed, o.restitution_threshold
+        c.restitution_iterations = o.restitution_iterations
+        c.enable_sleep, c.enable_continuous = int(o.enable_sleep), int(o.enable_continuous)
+        c.enable_warm_starting, c.enable_speculative = int(o.enable_warm_starting), int(o.enable_speculative)
+        return c
+
+    @property
+    def options(self) -> WorldOptions:
+        return self._opts
+
+    def apply_options(self) -> None:
+        capi.check(self._lib.nsl_world_set_options(self._h, ctypes.byref(self._c_options())))
+
+    def enable_sleeping(self, flag: bool) -> None:
+        self._opts.enable_sleep = bool(flag)
+        self.apply_options()
+
+    def close(self) -> None:
+        if self._h:
+            capi.check(self._lib.nsl_world_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- creation -------------------------------------------------------------------
+    def create_body(self, d: BodyDef) -> int:
+        c = capi.BodyDef()
+        c.type = d.type
+        c.position[:] = d.position
+        c.rotation[:] = d.rotation
+        c.linear_velocity[:] = d.linear_velocity
+        c.angular_velocity[:] = d.angular_velocity
+        c.linear_damping, c.angular_damping, c.gravity_scale = d.linear_damping, d.angular_damping, d.gravity_scale
+        c.enable_sleep = int(d.enable_sleep)
+        out = ctypes.c_int32()
+        capi.check(self._lib.nsl_create_body(self._h, ctypes.byref(c), ctypes.byref(out)))
+        return out.value
+
+    def create_hull_shape(self, body: int, d: ShapeDef, hull: BoxHull) -> int:
+        out = ctypes.c_int32()
+        h = (ctypes.c_float * 3)(hull.hx, hull.hy, hull.hz)
+        capi.check(self._lib.nsl_create_box(self._h, body, h, ctypes.byref(_material(d)), ctypes.byref(out)))
+        return out.value
+
+    def create_sphere_shape(self, body: int, d: ShapeDef, s: Sphere) -> int:
+        out = ctypes.c_int32()
+        c = (ctypes.c_float * 3)(*s.center)
+        capi.check(self._lib.nsl_create_sphere(self._h, body, c, s.radius, ctypes.byref(_material(d)), ctypes.byref(out)))
+        return out.value
+
+    def create_capsule_shape(self, body: int, d: ShapeDef, c: Capsule) -> int:
+        out = ctypes.c_int32()
+        c1 = (ctypes.c_float * 3)(*c.center1)
+        c2 = (ctypes.c_float * 3)(*c.center2)
+        capi.check(self._lib.nsl_create_capsule(self._h, body, c1, c2, c.radius, ctypes.byref(_material(d)), ctypes.byref(out)))
+        return out.value
+
+    # --- stepping and queries ----------------------------------------------------------
+    def step(self, time_step: float, sub_step_count: int) -> None:
+        capi.check(self._lib.nsl_world_step(self._h, time_step, sub_step_count))
+
+    def step_n(self, time_step: float, sub_step_count: int, steps: int) -> None:
+        capi.check(self._lib.nsl_world_step_n(self._h, time_step, sub_step_count, steps))
+
+    def world_state_hash(self) -> int:
+        out = ctypes.c_uint64()
+        capi.check(self._lib.nsl_world_state_hash(self._h, ctypes.byref(out)))
+        return out.value
+
+    def body_arrays(self):
+        """(position f64 [n,3], rotation f32 [n,4], linear velocity [n,3], angular velocity [n,3])."""
+        n = ctypes.c_int32()
+        capi.check(self._lib.nsl_world_body_capacity(self._h, ctypes.byref(n)))
+        n = n.value
+        pos = np.zeros((n, 3), np.float64)
+        rot = np.zeros((n, 4), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        w = np.zeros((n, 3), np.float32)
+        f = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))  # noqa: E731
+        capi.check(self._lib.nsl_world_get_bodies(self._h, f(pos, ctypes.c_double), f(rot, ctypes.c_float), f(v, ctypes.c_float), f(w, ctypes.c_float)))
+        return pos, rot, v, w
+
+    def contact_colors(self):
+        n = ctypes.c_int32()
+        capi.check(self._lib.nsl_world_contact_capacity(self._h, ctypes.byref(n)))
+        color, local, alive = (np.zeros(n.value, np.int32) for _ in range(3))
+        f = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        capi.check(self._lib.nsl_world_get_contact_colors(self._h, f(color), f(local), f(alive)))
+        return color, local, alive
+
+    def set_kernel_timing(self, on: bool) -> None:
+        capi.check(self._lib.nsl_world_set_kernel_timing(self._h, int(on)))
+
+    def reset_kernel_stats(self) -> None:
+        capi.check(self._lib.nsl_world_reset_kernel_stats(self._h))
+
+    def kernel_stats(self) -> dict:
+        k = capi.KernelStats()
+        capi.check(self._lib.nsl_world_get_kernel_stats(self._h, ctypes.byref(k)))
+        out = {n: getattr(k, n) for n, _ in capi.KernelStats._fields_[2:]}
+        out["ms"] = dict(zip(capi.KERNEL_CLASSES, list(k.ms)))
+        out["launches"] = dict(zip(capi.KERNEL_CLASSES, list(k.launches)))
+        return out
+
+    def profile(self) -> dict:
+        p = capi.Profile()
+        capi.check(self._lib.nsl_world_get_profile(self._h, ctypes.byref(p)))
+        return {name: getattr(p, name) for name, _ in capi.Profile._fields_}
+
+
+def load_scene(scene: Scene, device: int = 0) -> World:
+    """Build a World from a seeded scene description (natsel_amd.scenes)."""
+    opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous)
+    w = World(device, opts)
+    ids = []
+    for b in scene.bodies:
+        ids.append(w.create_body(BodyDef(b.body_type, b.position, b.rotation, b.linear_velocity, b.angular_velocity,
+                                         b.linear_damping, b.angular_damping, b.gravity_scale, b.enable_sleep)))
+    for s in scene.shapes:
+        d = ShapeDef(s.density, s.friction, s.restitution)
+        if s.kind == BOX:
+            w.create_hull_shape(ids[s.body], d, make_box_hull(*s.params))
+        elif s.kind == SPHERE:
+            w.create_sphere_shape(ids[s.body], d, Sphere(tuple(s.params[:3]), s.params[3]))
+        else:
+            w.create_capsule_shape(ids[s.body], d, Capsule(tuple(s.params[:3]), tuple(s.params[3:6]), s.params[6]))
+    return w

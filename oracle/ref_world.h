@@ -153,7 +153,7 @@ enum SepFeature : uint8_t {
 struct SATCache { float separation; uint8_t type, index_a, index_b, hit; };
 
 // src/types/queries.rs SimplexCache
-struct SimplexCache { uint16_t count; uint8_t index_a[4], index_b[4]; };
+struct SimplexCache { float metric; uint16_t count; uint8_t index_a[4], index_b[4]; };
 
 struct LocalPoint { Vec3 point; float separation; FeaturePair pair; int32_t triangle_index; };
 struct LocalManifold { Vec3 normal; LocalPoint pts[4]; int32_t point_count; };
