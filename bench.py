@@ -151,6 +151,7 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "kernel_ms": {k: round(v, 3) for k, v in ms.items()},
+        "host_phase_ms": {k: round(v, 3) for k, v in ks["phase_ms"].items()},
         "contacts_collided_per_step": ks["collided"] / max(ks["steps"], 1),
         "constraints_per_step": ks["constraints"] / max(ks["steps"], 1),
         "points_per_step": ks["points"] / max(ks["steps"], 1),

@@ -93,6 +93,7 @@ typedef struct nsl_kernel_stats {
   double ms[NSL_K_CLASS_COUNT];
   int64_t launches[NSL_K_CLASS_COUNT];
   int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
+  double phase_ms[4]; /* host wall time: pair update, collide (incl. wait), state pass, solve+finalize (incl. wait) */
 } nsl_kernel_stats;
 
 const char* nsl_last_error(void);

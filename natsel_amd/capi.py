@@ -67,7 +67,7 @@ class KernelStats(ctypes.Structure):
         (n, ctypes.c_int64)
         for n in ("steps", "bodies", "collided", "constraints", "points", "overflow_points", "recycled", "sat_calls",
                   "sat_hits", "fast_bodies")
-    ]
+    ] + [("phase_ms", ctypes.c_double * 4)]
 
 
 # Every entry point declared in include/natsel_gpu.h, with its argument types.

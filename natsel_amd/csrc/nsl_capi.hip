@@ -231,6 +231,7 @@ int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out) {
   out->sat_calls = k.sat_calls;
   out->sat_hits = k.sat_hits;
   out->fast_bodies = k.fast_bodies;
+  for (int i = 0; i < 4; ++i) out->phase_ms[i] = k.phase_ms[i];
   return NSL_OK;
 }
 

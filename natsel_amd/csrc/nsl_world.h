@@ -82,6 +82,7 @@ class GpuWorld {
     double ms[kKernelClassCount];
     int64_t launches[kKernelClassCount];
     int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
+    double phase_ms[4];  // host wall time: pair update, collide (with its wait), state pass, solve+finalize (with its wait)
   };
   void set_kernel_timing(bool on);
   const KernelStats& kernel_stats() const { return kstats_; }

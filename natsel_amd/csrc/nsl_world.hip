@@ -761,6 +761,10 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     }
   }
   const double t4 = now_ms();
+  kstats_.phase_ms[0] += t1 - t0;
+  kstats_.phase_ms[1] += t2 - t1;
+  kstats_.phase_ms[2] += t3 - t2;
+  kstats_.phase_ms[3] += t4 - t3;
   profile_.pairs = (float)(t1 - t0);
   profile_.collide = (float)(t2 - t1);
   profile_.state_pass = (float)(t3 - t2);

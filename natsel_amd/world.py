@@ -199,7 +199,8 @@ class World:
     def kernel_stats(self) -> dict:
         k = capi.KernelStats()
         capi.check(self._lib.nsl_world_get_kernel_stats(self._h, ctypes.byref(k)))
-        out = {n: getattr(k, n) for n, _ in capi.KernelStats._fields_[2:]}
+        out = {n: getattr(k, n) for n, _ in capi.KernelStats._fields_[2:-1]}
+        out["phase_ms"] = dict(zip(["pairs", "collide", "state_pass", "solve_finalize"], list(k.phase_ms)))
         out["ms"] = dict(zip(capi.KERNEL_CLASSES, list(k.ms)))
         out["launches"] = dict(zip(capi.KERNEL_CLASSES, list(k.launches)))
         return out
