@@ -27,35 +27,39 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# Algorithmic bytes per item (distinct fields the reference formula reads plus
-# writes, f32 = 4 B, each body-state gather counted per constraint touch).
-# Per colored constraint with one manifold: base + per manifold + per point.
+# Algorithmic bytes per item: distinct fields the reference formula reads plus writes,
+# f32 = 4 B, each body-state gather counted per constraint touch. Coloured constraint
+# stages: per constraint (one manifold) + per manifold point.
 BYTE_MODEL = {
-    "integrate_velocities": {"per_body": 124},
-    "integrate_positions": {"per_body": 112},
+    "prepare": {"per_constraint": 268 + 452, "per_point": 0},
+    "integrate_velocities": {"per_body": 100 + 24},
     "warm_start": {"per_constraint": 64 + 72 + 56 + 48, "per_point": 28},
     "push": {"per_constraint": 68 + 12 + 112 + 48, "per_point": 40},
+    "integrate_positions": {"per_body": 56 + 56},
     "relax": {"per_constraint": 68 + 108 + 112 + 48, "per_point": 52},
-    "prepare": {"per_constraint": 268 + 452, "per_point": 0},
     "store": {"per_constraint": 76, "per_point": 16},
     "finalize": {"per_body": 400},
     # contact record read and written (~300 B each way), both bodies' poses (2 x 64 B),
     # both fat AABBs (2 x 24 B)
     "collide": {"per_contact": 600 + 128 + 48},
 }
+PER_SUBSTEP = ("integrate_velocities", "warm_start", "push", "integrate_positions", "relax")
 
 
-def algorithmic_bytes(cls: str, ks: dict, substeps: int) -> float:
-    m = BYTE_MODEL.get(cls)
-    if m is None:
-        return 0.0
+def stage_bytes(stage: str, ks: dict, substeps: int) -> float:
+    """Algorithmic bytes of one stage summed over the measured steps."""
+    m = BYTE_MODEL[stage]
+    reps = substeps if stage in PER_SUBSTEP else 1
     if "per_body" in m:
-        per_launch_bodies = ks["bodies"] / max(ks["steps"], 1)
-        return m["per_body"] * per_launch_bodies * ks["launches"][cls]
+        return reps * m["per_body"] * ks["bodies"]
     if "per_contact" in m:
         return m["per_contact"] * ks["collided"]
-    reps = substeps if cls in ("warm_start", "push", "relax") else 1
     return reps * (m["per_constraint"] * ks["constraints"] + m["per_point"] * ks["points"])
+
+
+def solve_bytes(ks: dict, substeps: int) -> float:
+    """Algorithmic bytes of the step kernel (prepare .. finalize) over the measured steps."""
+    return sum(stage_bytes(k, ks, substeps) for k in BYTE_MODEL if k != "collide")
 
 
 def cpu_baseline(scene, seconds: float, max_steps: int) -> dict:
@@ -88,6 +92,7 @@ def main() -> int:
     ap.add_argument("--bodies", type=int, default=10_000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--blocks", type=int, default=0, help="workgroups of the step kernel (0: sized from the largest phase)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)", file=sys.stderr)
@@ -99,7 +104,7 @@ def main() -> int:
     from natsel_amd.world import load_scene
 
     scene = scenes.pile(args.bodies)
-    world = load_scene(scene, device=0)
+    world = load_scene(scene, device=0, solver_blocks=args.blocks)
     for _ in range(args.warmup):
         world.step(scene.dt, scene.substeps)
     world.reset_kernel_stats()
@@ -118,9 +123,10 @@ def main() -> int:
     value = n * scene.substeps * args.steps / elapsed
 
     ms = ks["ms"]
-    dominant = max(ms, key=lambda k: ms[k])
-    dom_bytes = algorithmic_bytes(dominant, ks, scene.substeps)
-    dom_s = ms[dominant] / 1e3
+    # dominant launch: the step kernel (one per step); its HIP-event time over the timed region
+    launches = max(ks["launches"]["solve"], 1)
+    dom_s = ms["solve"] / 1e3
+    dom_bytes = solve_bytes(ks, scene.substeps)
     achieved = dom_bytes / dom_s / 1e9 if dom_s > 0 else 0.0
     roofline = {
         "bound": "hbm",
@@ -129,9 +135,12 @@ def main() -> int:
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": None,
-        "kernel": dominant,
-        "launches": ks["launches"][dominant],
-        "avg_launch_us": 1e3 * ms[dominant] / max(ks["launches"][dominant], 1),
+        "kernel": "k_solve_step",
+        "launches": ks["launches"]["solve"],
+        "avg_launch_us": 1e3 * ms["solve"] / launches,
+        "algorithmic_bytes_per_launch": dom_bytes / launches,
+        "grid_barriers_per_launch": ks["barriers"] / launches,
+        "workgroups": ks["solver_blocks"],
     }
     out = {
         "metric": "body-substeps/s (10k pile)",
@@ -151,10 +160,13 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "kernel_ms": {k: round(v, 3) for k, v in ms.items()},
-        "host_phase_ms": {k: round(v, 3) for k, v in ks["phase_ms"].items()},
+        "step_kernel_phase_ms": {k: round(v, 3) for k, v in ks["phase_ms"].items()},
+        "step_kernel_phase_count": ks["phase_count"],
+        "host_phase_ms": {k: round(v, 3) for k, v in ks["host_ms"].items()},
         "contacts_collided_per_step": ks["collided"] / max(ks["steps"], 1),
         "constraints_per_step": ks["constraints"] / max(ks["steps"], 1),
         "points_per_step": ks["points"] / max(ks["steps"], 1),
+        "pair_tests_per_step": ks["pair_tests"] / max(ks["steps"], 1),
         "fast_bodies": ks["fast_bodies"],
     }
     if not args.no_cpu:

@@ -58,6 +58,7 @@ typedef struct nsl_options {
   int32_t enable_continuous;         /* 1: fast bodies are counted (continuous collision is not run) */
   int32_t enable_warm_starting;      /* 1 */
   int32_t enable_speculative;        /* 1 */
+  int32_t solver_blocks;             /* workgroups of the step kernel; 0 = sized from the largest phase */
 } nsl_options;
 
 typedef struct nsl_body_def {
@@ -80,20 +81,27 @@ typedef struct nsl_profile {
   int32_t fast_bodies, sat_calls, sat_hits, recycled;
 } nsl_profile;
 
-/* Kernel classes of nsl_kernel_stats.ms / launches. */
+/* Launch classes of nsl_kernel_stats.ms / launches (HIP events around each launch). */
+enum { NSL_K_INIT = 0, NSL_K_COLLIDE, NSL_K_SOLVE, NSL_K_CLASS_COUNT };
+
+/* Phases of the step kernel (NSL_K_SOLVE), timed with the device wall clock at its grid
+ * barriers; each phase's time includes the barrier that ends it. */
 enum {
-  NSL_K_COLLIDE = 0, NSL_K_PREPARE, NSL_K_INTEGRATE_VELOCITIES, NSL_K_WARM_START, NSL_K_PUSH,
-  NSL_K_INTEGRATE_POSITIONS, NSL_K_RELAX, NSL_K_RESTITUTION, NSL_K_STORE, NSL_K_OVERFLOW, NSL_K_FINALIZE,
-  NSL_K_CLASS_COUNT
+  NSL_P_PREPARE = 0, NSL_P_INTEGRATE_VELOCITIES, NSL_P_WARM_START, NSL_P_PUSH, NSL_P_INTEGRATE_POSITIONS,
+  NSL_P_RELAX, NSL_P_RESTITUTION, NSL_P_OVERFLOW, NSL_P_STORE_FINALIZE, NSL_P_CLASS_COUNT
 };
 
-/* Cumulative device time per kernel class (HIP events, only while timing is on) and
- * work counters since the last reset. */
+/* Device time (only while timing is on) and work counters since the last reset. */
 typedef struct nsl_kernel_stats {
   double ms[NSL_K_CLASS_COUNT];
   int64_t launches[NSL_K_CLASS_COUNT];
+  double phase_ms[NSL_P_CLASS_COUNT];
+  int64_t phase_count[NSL_P_CLASS_COUNT];
   int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
-  double phase_ms[4]; /* host wall time: pair update, collide (incl. wait), state pass, solve+finalize (incl. wait) */
+  int64_t barriers;      /* grid barriers executed by the step kernel */
+  int64_t solver_blocks; /* workgroups of the last step kernel launch */
+  int64_t pair_tests;    /* candidate shape pairs tested by the device pair finder */
+  double host_ms[4];     /* host wall time: pair update, collide (incl. wait), state pass, solve+finalize (incl. wait) */
 } nsl_kernel_stats;
 
 const char* nsl_last_error(void);
@@ -121,7 +129,7 @@ int nsl_world_contact_capacity(nsl_world* world, int32_t* out);
 /* Arrays sized by nsl_world_contact_capacity. */
 int nsl_world_get_contact_colors(nsl_world* world, int32_t* color, int32_t* local, int32_t* alive);
 int nsl_world_get_profile(nsl_world* world, nsl_profile* out);
-/* Per-kernel timing adds one event pair per launch; the step's results do not change. */
+/* Timing adds an event pair per launch and a wall-clock read per phase; results do not change. */
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on);
 int nsl_world_reset_kernel_stats(nsl_world* world);
 int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out);

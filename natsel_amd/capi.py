@@ -25,6 +25,7 @@ class Options(ctypes.Structure):
         ("enable_continuous", ctypes.c_int32),
         ("enable_warm_starting", ctypes.c_int32),
         ("enable_speculative", ctypes.c_int32),
+        ("solver_blocks", ctypes.c_int32),
     ]
 
 
@@ -58,16 +59,23 @@ class Profile(ctypes.Structure):
     ]
 
 
-KERNEL_CLASSES = ["collide", "prepare", "integrate_velocities", "warm_start", "push", "integrate_positions", "relax",
-                  "restitution", "store", "overflow", "finalize"]
+KERNEL_CLASSES = ["init", "collide", "solve"]
+PHASE_CLASSES = ["prepare", "integrate_velocities", "warm_start", "push", "integrate_positions", "relax", "restitution",
+                 "overflow", "store_finalize"]
+HOST_PHASES = ["pairs", "collide", "state_pass", "solve_finalize"]
 
 
 class KernelStats(ctypes.Structure):
-    _fields_ = [("ms", ctypes.c_double * len(KERNEL_CLASSES)), ("launches", ctypes.c_int64 * len(KERNEL_CLASSES))] + [
+    _fields_ = [
+        ("ms", ctypes.c_double * len(KERNEL_CLASSES)),
+        ("launches", ctypes.c_int64 * len(KERNEL_CLASSES)),
+        ("phase_ms", ctypes.c_double * len(PHASE_CLASSES)),
+        ("phase_count", ctypes.c_int64 * len(PHASE_CLASSES)),
+    ] + [
         (n, ctypes.c_int64)
         for n in ("steps", "bodies", "collided", "constraints", "points", "overflow_points", "recycled", "sat_calls",
-                  "sat_hits", "fast_bodies")
-    ] + [("phase_ms", ctypes.c_double * 4)]
+                  "sat_hits", "fast_bodies", "barriers", "solver_blocks", "pair_tests")
+    ] + [("host_ms", ctypes.c_double * len(HOST_PHASES))]
 
 
 # Every entry point declared in include/natsel_gpu.h, with its argument types.

@@ -44,6 +44,7 @@ void apply(nsl::WorldOptions& o, const nsl_options& in) {
   o.enable_continuous = in.enable_continuous != 0;
   o.enable_warm_starting = in.enable_warm_starting != 0;
   o.enable_speculative = in.enable_speculative != 0;
+  o.solver_blocks = in.solver_blocks;
 }
 
 nsl::Material material(const nsl_material* m) {
@@ -80,6 +81,7 @@ void nsl_default_options(nsl_options* out) {
   out->enable_continuous = o.enable_continuous;
   out->enable_warm_starting = o.enable_warm_starting;
   out->enable_speculative = o.enable_speculative;
+  out->solver_blocks = o.solver_blocks;
 }
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out) {
@@ -231,7 +233,15 @@ int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out) {
   out->sat_calls = k.sat_calls;
   out->sat_hits = k.sat_hits;
   out->fast_bodies = k.fast_bodies;
-  for (int i = 0; i < 4; ++i) out->phase_ms[i] = k.phase_ms[i];
+  static_assert((int)NSL_P_CLASS_COUNT == (int)nsl::kPhaseClassCount, "phase class count");
+  for (int i = 0; i < NSL_P_CLASS_COUNT; ++i) {
+    out->phase_ms[i] = k.phase_ms[i];
+    out->phase_count[i] = k.phase_count[i];
+  }
+  out->barriers = k.barriers;
+  out->solver_blocks = k.solver_blocks;
+  out->pair_tests = k.pair_tests;
+  for (int i = 0; i < 4; ++i) out->host_ms[i] = k.host_ms[i];
   return NSL_OK;
 }
 

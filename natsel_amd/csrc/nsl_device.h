@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "nsl_pairset.h"
 #include "nsl_types.h"
 
 namespace nsl {
@@ -23,14 +24,19 @@ struct DeviceWorld {
   int32_t* solve_ids;       // coloured contact ids, colour-major (slot order)
   int32_t* overflow_ids;    // overflow contact ids in graph order
   int32_t* counters;        // [0] fast bodies flagged, [1] sat calls, [2] sat cache hits, [3] recycled,
-                            // [4] manifold points prepared (coloured), [5] overflow points
+                            // [4] manifold points prepared (coloured), [5] overflow points,
+                            // [6] proxies moved (fat AABB enlarged) since the last pair update
+  PairSet pairs;            // keys of the shape pairs that have a contact
   int32_t n_awake;
 };
 
-// Kernel classes timed with HIP events when kernel timing is on.
-enum KernelClass : int {
-  kKCollide = 0, kKPrepare, kKIntegrateVelocities, kKWarmStart, kKPush, kKIntegratePositions, kKRelax,
-  kKRestitution, kKStore, kKOverflow, kKFinalize, kKernelClassCount
+// Launches timed with HIP events when kernel timing is on.
+enum KernelClass : int { kKInit = 0, kKCollide, kKSolve, kKernelClassCount };
+
+// Phases inside the step kernel; timed with the device wall clock at each grid barrier.
+enum PhaseClass : int {
+  kPPrepare = 0, kPIntegrateVelocities, kPWarmStart, kPPush, kPIntegratePositions, kPRelax, kPRestitution,
+  kPOverflow, kPStoreFinalize, kPhaseClassCount
 };
 
 // Event pairs bracketing launches on one stream; read back after the step's sync.
@@ -38,7 +44,7 @@ struct KernelTimer {
   hipEvent_t* events = nullptr;   // pool, 2 per mark
   int capacity = 0;               // marks
   int used = 0;
-  int classes[4096];
+  int classes[64];
   void begin(int cls, hipStream_t s) {
     if (used < capacity) {
       classes[used] = cls;
@@ -53,18 +59,42 @@ struct KernelTimer {
   }
 };
 
+// Grid barrier state of the step kernel (zeroed at creation; count returns to 0 after
+// every completed barrier).
+struct GridBarrier {
+  uint32_t count;
+  uint32_t gen;
+  uint32_t timed_out;
+  uint32_t pad;
+};
+
 struct ColorRanges {
   int32_t begin[kColorCount];
   int32_t end[kColorCount];
 };
 
+// Everything the step kernel needs beyond the world pointers.
+struct SolvePlan {
+  ColorRanges cr;                 // coloured slots per colour (colour-major)
+  int32_t n_constraints;          // coloured slots in total
+  int32_t n_overflow;             // overflow constraints, solved in list order by one lane
+  int32_t any_restitution;
+  int32_t restitution_iterations;
+  GridBarrier* barrier;
+  uint64_t* phase_log;            // null: no phase timing; else [2 * kMaxPhaseLog]: (class, wall clock)
+};
+constexpr int kMaxPhaseLog = 1024;
+
 // kernels (nsl_kernels.hip)
 // `timer` may be null (no timing).
 void launch_init_contacts(const DeviceWorld& d, const int32_t* ids, const Contact* records, int n, hipStream_t s);
 void launch_collide(const DeviceWorld& d, int n, hipStream_t s, KernelTimer* timer);
-void launch_prepare(const DeviceWorld& d, int n, int n_overflow, const StepParams& p, hipStream_t s, KernelTimer* timer);
-void launch_solve(const DeviceWorld& d, const ColorRanges& cr, int n_overflow, bool any_restitution, int restitution_iterations,
-                  const StepParams& p, hipStream_t s, KernelTimer* timer);
-void launch_finalize(const DeviceWorld& d, const StepParams& p, hipStream_t s, KernelTimer* timer);
+void launch_pairset_insert(PairSet set, const uint64_t* keys, int n, hipStream_t s);
+// One cooperative launch: prepare, the substep schedule, restitution, store and finalize.
+void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
+                       KernelTimer* timer);
+// Workgroups of the step kernel that fit on the device at once.
+int solve_step_max_blocks(int device);
+constexpr int kSolveBlock = 256;
 
 }  // namespace nsl

@@ -8,8 +8,12 @@
 // per constraint within a colour reproduces the CPU's Gauss-Seidel order exactly.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+#include <string>
+
 #include "nsl_round_collide.h"
 #include "nsl_device.h"
+#include "nsl_pairset.h"
 
 namespace nsl {
 
@@ -35,7 +39,14 @@ __device__ __forceinline__ void store_state(const DeviceWorld& d, int32_t i, con
 __global__ void k_init_contacts(DeviceWorld d, const int32_t* ids, const Contact* records, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  d.contacts[ids[t]] = records[t];
+  const Contact& c = records[t];
+  d.contacts[ids[t]] = c;
+  pairset_insert(d.pairs, pair_key(c.shape_a, c.shape_b));
+}
+
+__global__ void k_pairset_insert(PairSet set, const uint64_t* keys, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) pairset_insert(set, keys[t]);
 }
 
 // ------------------------------------------------------------------------------
@@ -145,6 +156,7 @@ __global__ void k_collide(DeviceWorld d, int n) {
     c.flags &= ~cflag::kSimTouching;
     d.contacts[id].flags = c.flags;
     d.contact_change[id] = kChangeDisjoint;
+    pairset_erase(d.pairs, pair_key(c.shape_a, c.shape_b));  // the host destroys this contact
     return;
   }
   BodyView ba = body_view(d, c.enc_a), bb = body_view(d, c.enc_b);
@@ -224,9 +236,8 @@ __global__ void k_collide(DeviceWorld d, int n) {
 // ------------------------------------------------------------------------------
 // constraint preparation (contact_solver.rs:395-662)
 
-__global__ void k_prepare(DeviceWorld d, int n, StepParams p) {
-  int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n) return;
+// Returns the number of manifold points prepared.
+__device__ __forceinline__ int prepare_item(const DeviceWorld& d, int slot, const StepParams& p) {
   const int32_t id = d.solve_ids[slot];
   const Contact& contact = d.contacts[id];
   const float inv_tau = 1.0f / kSpeculativeDistance;
@@ -320,13 +331,11 @@ __global__ void k_prepare(DeviceWorld d, int n, StepParams p) {
     }
   }
   d.constraints[slot] = cc;
-  atomicAdd(&d.counters[4], cc.point_count);
+  return cc.point_count;
 }
 
 // overflow_contact_solver.rs:92-257
-__global__ void k_prepare_overflow(DeviceWorld d, int n, StepParams p) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+__device__ __forceinline__ int prepare_overflow_item(const DeviceWorld& d, int k, const StepParams& p) {
   const int32_t id = d.overflow_ids[k];
   Contact& contact = d.contacts[id];
   const float warm = p.warm_start_scale;
@@ -415,15 +424,13 @@ __global__ void k_prepare_overflow(DeviceWorld d, int n, StepParams p) {
   c.twist_impulse = warm * contact.twist_impulse;
   c.rolling_impulse = scale(warm, contact.rolling_impulse);
   d.overflow[k] = c;
-  atomicAdd(&d.counters[5], c.point_count);
+  return c.point_count;
 }
 
 // ------------------------------------------------------------------------------
 // integration (solver_integration.rs:263-426)
 
-__global__ void k_integrate_velocities(DeviceWorld d, StepParams p) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n_awake) return;
+__device__ __forceinline__ void integrate_velocities_item(const DeviceWorld& d, int i, const StepParams& p) {
   const float h = p.h;
   const BodySim& sim = d.sims[i];
   BodyState state = d.states[i];
@@ -461,9 +468,7 @@ __global__ void k_integrate_velocities(DeviceWorld d, StepParams p) {
   d.states[i].w = w;
 }
 
-__global__ void k_integrate_positions(DeviceWorld d, StepParams p) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n_awake) return;
+__device__ __forceinline__ void integrate_positions_item(const DeviceWorld& d, int i, const StepParams& p) {
   BodyState st = d.states[i];
   const float max_lin = p.max_linear_speed;
   const float max_ang = kMaxRotation * p.inv_dt;
@@ -490,12 +495,11 @@ __global__ void k_integrate_positions(DeviceWorld d, StepParams p) {
 }
 
 // ------------------------------------------------------------------------------
-// coloured stages: one lane per constraint slot of one colour
+// coloured stages: one lane per constraint slot of one colour; the colour's slots touch
+// distinct dynamic bodies, so they run in any order
 
 // contact_solver.rs:697-763
-__global__ void k_warm_start(DeviceWorld d, int begin, int end) {
-  int slot = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= end) return;
+__device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot) {
   const Constraint& c = d.constraints[slot];
   Vec3 linear = zero3(), ang_a = zero3(), ang_b = zero3();
   if (c.manifold_count > 0) {
@@ -532,9 +536,7 @@ __global__ void k_warm_start(DeviceWorld d, int begin, int end) {
 }
 
 // contact_solver.rs:796-894
-__global__ void k_push(DeviceWorld d, int begin, int end, StepParams p) {
-  int slot = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= end) return;
+__device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const StepParams& p) {
   Constraint& c = d.constraints[slot];
   BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
   const Softness sf = c.is_static ? p.static_softness : p.contact_softness;
@@ -576,9 +578,7 @@ __global__ void k_push(DeviceWorld d, int begin, int end, StepParams p) {
 }
 
 // contact_solver.rs:927-1097
-__global__ void k_relax(DeviceWorld d, int begin, int end, StepParams p) {
-  int slot = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= end) return;
+__device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const StepParams& p) {
   Constraint& c = d.constraints[slot];
   BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
   const float inv_h = p.inv_h, eps = kFltEps;
@@ -664,9 +664,7 @@ __global__ void k_relax(DeviceWorld d, int begin, int end, StepParams p) {
 }
 
 // contact_solver.rs:1135-1229
-__global__ void k_restitution(DeviceWorld d, int begin, int end, StepParams p) {
-  int slot = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= end) return;
+__device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot, const StepParams& p) {
   Constraint& c = d.constraints[slot];
   const bool propagate = p.enable_restitution_propagation != 0;
   if (!propagate && c.restitution == 0.0f) return;
@@ -709,9 +707,7 @@ __global__ void k_restitution(DeviceWorld d, int begin, int end, StepParams p) {
 }
 
 // contact_solver.rs:1258-1299
-__global__ void k_store(DeviceWorld d, int n) {
-  int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n) return;
+__device__ __forceinline__ void store_item(const DeviceWorld& d, int slot) {
   const Constraint& c = d.constraints[slot];
   Contact& contact = d.contacts[c.contact_id];
   if (contact.manifold_count <= 0 || c.manifold_count <= 0) return;
@@ -738,8 +734,8 @@ __device__ __forceinline__ void overflow_store_velocity(const DeviceWorld& d, in
   }
 }
 
-__global__ void k_overflow_warm(DeviceWorld d, int n) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// One lane walks the whole list.
+__device__ void overflow_warm_all(const DeviceWorld& d, int n) {
   for (int k = 0; k < n; ++k) {
     const OverflowConstraint& c = d.overflow[k];
     Vec3 linear = zero3(), ang_a = zero3(), ang_b = zero3();
@@ -775,8 +771,7 @@ __global__ void k_overflow_warm(DeviceWorld d, int n) {
 }
 
 // mode 0: push (biased), 1: relax, 2: restitution
-__global__ void k_overflow_solve(DeviceWorld d, int n, StepParams p, int mode) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__device__ void overflow_solve_all(const DeviceWorld& d, int n, const StepParams& p, int mode) {
   const float inv_h = p.inv_h;
   for (int k = 0; k < n; ++k) {
     OverflowConstraint& c = d.overflow[k];
@@ -918,9 +913,7 @@ __global__ void k_overflow_solve(DeviceWorld d, int n, StepParams p, int mode) {
   }
 }
 
-__global__ void k_overflow_store(DeviceWorld d, int n) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+__device__ __forceinline__ void overflow_store_item(const DeviceWorld& d, int k) {
   const OverflowConstraint& c = d.overflow[k];
   if (c.manifold_count == 0) return;
   Contact& m = d.contacts[c.contact_id];
@@ -949,9 +942,7 @@ __device__ Aabb shape_local_aabb(const DeviceWorld& d, const Shape& s, Quat q) {
   return Aabb{sub(min3(c1, c2), e), add(max3(c1, c2), e)};
 }
 
-__global__ void k_finalize(DeviceWorld d, StepParams p) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n_awake) return;
+__device__ __forceinline__ void finalize_item(const DeviceWorld& d, int i, const StepParams& p) {
   BodyState state = d.states[i];
   BodySim sim = d.sims[i];
   const float time_step = p.dt, inv_time_step = p.inv_dt;
@@ -997,11 +988,178 @@ __global__ void k_finalize(DeviceWorld d, StepParams p) {
       if (!aabb_contains(d.fat_aabbs[sid], aabb)) {
         d.fat_aabbs[sid] = aabb_inflate(aabb, s.aabb_margin);
         d.shape_moved[sid] = 1;
+        atomicAdd(&d.counters[6], 1);
       }
     }
   }
   d.states[i] = state;
   d.sims[i] = sim;
+}
+
+// ------------------------------------------------------------------------------
+// the step kernel: one cooperative launch per step
+
+// Polls of a barrier wait before the kernel gives up (a few seconds; far above any phase).
+constexpr uint32_t kBarrierSpinLimit = 1u << 22;
+
+// Grid-wide barrier of a cooperative launch (every workgroup is resident). The last
+// workgroup to arrive resets the count and advances the generation; agent-scope
+// release/acquire make every store issued before the barrier visible to every load after
+// it, across the XCDs' L2 caches. A wait that exceeds the spin limit raises `timed_out`
+// and every workgroup leaves the kernel at its next barrier; returns false then.
+__device__ bool grid_sync(GridBarrier* b) {
+  __shared__ int s_ok;
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached L2 before lane 0 releases them
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const uint32_t g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint32_t arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      uint32_t spins = 0;
+      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        ++spins;
+        if (spins > kBarrierSpinLimit ||
+            ((spins & 1023u) == 0 && __hip_atomic_load(&b->timed_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+          __hip_atomic_store(&b->timed_out, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+// Sum of one value over the workgroup, added to *dst by one lane.
+__device__ void block_add(int32_t* dst, int v) {
+  __shared__ int s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd(&s_sum, v);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_sum != 0) atomicAdd(dst, s_sum);
+}
+
+struct PhaseClock {
+  uint64_t* log;
+  int n;
+  __device__ void mark(int cls) {
+    if (log != nullptr && blockIdx.x == 0 && threadIdx.x == 0 && n < kMaxPhaseLog) {
+      log[2 * n] = (uint64_t)cls;
+      log[2 * n + 1] = wall_clock64();
+    }
+    ++n;
+  }
+};
+
+// Stage schedule of one step (solver_stages.rs:543-589, solver_integration.rs:807-1015):
+// prepare; per substep: integrate velocities, warm start, push, integrate positions,
+// relax, each stage running the overflow list first and then colours 0..22 in order;
+// restitution iterations; store impulses; finalize. A grid barrier separates dependent
+// phases; the overflow list is walked by one lane.
+__global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, SolvePlan plan, StepParams p) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  const int n_ovf = plan.n_overflow;
+  const bool lane0 = tid == 0;
+  GridBarrier* bar = plan.barrier;
+  PhaseClock clock{plan.phase_log, 0};
+  clock.mark(-1);
+
+  // colours with work, in order
+  int active[kOverflowColor];
+  int n_active = 0;
+  for (int ci = 0; ci < kOverflowColor; ++ci)
+    if (plan.cr.end[ci] > plan.cr.begin[ci]) active[n_active++] = ci;
+
+  {
+    int points = 0, ovf_points = 0;
+    for (int slot = tid; slot < plan.n_constraints; slot += nthreads) points += prepare_item(d, slot, p);
+    for (int k = tid; k < n_ovf; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);
+    block_add(&d.counters[4], points);
+    block_add(&d.counters[5], ovf_points);
+  }
+  if (!grid_sync(bar)) return;
+  clock.mark(kPPrepare);
+
+  for (int sub = 0; sub < p.sub_steps; ++sub) {
+    for (int i = tid; i < d.n_awake; i += nthreads) integrate_velocities_item(d, i, p);
+    if (!grid_sync(bar)) return;
+    clock.mark(kPIntegrateVelocities);
+
+    if (n_ovf > 0) {
+      if (lane0) overflow_warm_all(d, n_ovf);
+      if (!grid_sync(bar)) return;
+      clock.mark(kPOverflow);
+    }
+    for (int a = 0; a < n_active; ++a) {
+      const int ci = active[a];
+      for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) warm_start_item(d, slot);
+      if (!grid_sync(bar)) return;
+      clock.mark(kPWarmStart);
+    }
+
+    if (n_ovf > 0) {
+      if (lane0) overflow_solve_all(d, n_ovf, p, 0);
+      if (!grid_sync(bar)) return;
+      clock.mark(kPOverflow);
+    }
+    for (int a = 0; a < n_active; ++a) {
+      const int ci = active[a];
+      for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) push_item(d, slot, p);
+      if (!grid_sync(bar)) return;
+      clock.mark(kPPush);
+    }
+
+    for (int i = tid; i < d.n_awake; i += nthreads) integrate_positions_item(d, i, p);
+    if (!grid_sync(bar)) return;
+    clock.mark(kPIntegratePositions);
+
+    if (n_ovf > 0) {
+      if (lane0) overflow_solve_all(d, n_ovf, p, 1);
+      if (!grid_sync(bar)) return;
+      clock.mark(kPOverflow);
+    }
+    for (int a = 0; a < n_active; ++a) {
+      const int ci = active[a];
+      for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) relax_item(d, slot, p);
+      if (!grid_sync(bar)) return;
+      clock.mark(kPRelax);
+    }
+  }
+
+  if (plan.any_restitution) {
+    for (int it = 0; it < plan.restitution_iterations; ++it) {
+      if (n_ovf > 0) {
+        if (lane0) overflow_solve_all(d, n_ovf, p, 2);
+        if (!grid_sync(bar)) return;
+        clock.mark(kPOverflow);
+      }
+      for (int a = 0; a < n_active; ++a) {
+        const int ci = active[a];
+        for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) restitution_item(d, slot, p);
+        if (!grid_sync(bar)) return;
+        clock.mark(kPRestitution);
+      }
+    }
+  }
+
+  // impulses go to the contacts and bodies are finalised: disjoint data, one phase
+  for (int k = tid; k < n_ovf; k += nthreads) overflow_store_item(d, k);
+  for (int slot = tid; slot < plan.n_constraints; slot += nthreads) store_item(d, slot);
+  for (int i = tid; i < d.n_awake; i += nthreads) finalize_item(d, i, p);
+  if (!grid_sync(bar)) return;
+  clock.mark(kPStoreFinalize);
 }
 
 inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
@@ -1020,58 +1178,33 @@ void launch_init_contacts(const DeviceWorld& d, const int32_t* ids, const Contac
   if (n > 0) hipLaunchKernelGGL(k_init_contacts, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, ids, records, n);
 }
 
+void launch_pairset_insert(PairSet set, const uint64_t* keys, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_pairset_insert, dim3(blocks_for(n)), dim3(kBlock), 0, s, set, keys, n);
+}
+
 void launch_collide(const DeviceWorld& d, int n, hipStream_t s, KernelTimer* t) {
   if (n > 0) NSL_TIMED(t, kKCollide, s, hipLaunchKernelGGL(k_collide, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, n));
 }
 
-void launch_prepare(const DeviceWorld& d, int n, int n_overflow, const StepParams& p, hipStream_t s, KernelTimer* t) {
-  if (n > 0) NSL_TIMED(t, kKPrepare, s, hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, n, p));
-  if (n_overflow > 0)
-    NSL_TIMED(t, kKOverflow, s, hipLaunchKernelGGL(k_prepare_overflow, dim3(blocks_for(n_overflow)), dim3(kBlock), 0, s, d, n_overflow, p));
+int solve_step_max_blocks(int device) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_step), kSolveBlock, 0) !=
+      hipSuccess)
+    return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  return per_cu * cus;
 }
 
-// solver_stages.rs:543-589
-void launch_solve(const DeviceWorld& d, const ColorRanges& cr, int n_overflow, bool any_restitution, int restitution_iterations,
-                  const StepParams& p, hipStream_t s, KernelTimer* t) {
-  const int nb = blocks_for(d.n_awake);
-  for (int sub_i = 0; sub_i < p.sub_steps; ++sub_i) {
-    if (nb > 0) NSL_TIMED(t, kKIntegrateVelocities, s, hipLaunchKernelGGL(k_integrate_velocities, dim3(nb), dim3(kBlock), 0, s, d, p));
-    if (n_overflow > 0) NSL_TIMED(t, kKOverflow, s, hipLaunchKernelGGL(k_overflow_warm, dim3(1), dim3(64), 0, s, d, n_overflow));
-    for (int ci = 0; ci < kOverflowColor; ++ci) {
-      int n = cr.end[ci] - cr.begin[ci];
-      if (n > 0)
-        NSL_TIMED(t, kKWarmStart, s, hipLaunchKernelGGL(k_warm_start, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, cr.begin[ci], cr.end[ci]));
-    }
-    if (n_overflow > 0) NSL_TIMED(t, kKOverflow, s, hipLaunchKernelGGL(k_overflow_solve, dim3(1), dim3(64), 0, s, d, n_overflow, p, 0));
-    for (int ci = 0; ci < kOverflowColor; ++ci) {
-      int n = cr.end[ci] - cr.begin[ci];
-      if (n > 0) NSL_TIMED(t, kKPush, s, hipLaunchKernelGGL(k_push, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, cr.begin[ci], cr.end[ci], p));
-    }
-    if (nb > 0) NSL_TIMED(t, kKIntegratePositions, s, hipLaunchKernelGGL(k_integrate_positions, dim3(nb), dim3(kBlock), 0, s, d, p));
-    if (n_overflow > 0) NSL_TIMED(t, kKOverflow, s, hipLaunchKernelGGL(k_overflow_solve, dim3(1), dim3(64), 0, s, d, n_overflow, p, 1));
-    for (int ci = 0; ci < kOverflowColor; ++ci) {
-      int n = cr.end[ci] - cr.begin[ci];
-      if (n > 0) NSL_TIMED(t, kKRelax, s, hipLaunchKernelGGL(k_relax, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, cr.begin[ci], cr.end[ci], p));
-    }
-  }
-  if (any_restitution) {
-    for (int it = 0; it < restitution_iterations; ++it) {
-      if (n_overflow > 0) NSL_TIMED(t, kKOverflow, s, hipLaunchKernelGGL(k_overflow_solve, dim3(1), dim3(64), 0, s, d, n_overflow, p, 2));
-      for (int ci = 0; ci < kOverflowColor; ++ci) {
-        int n = cr.end[ci] - cr.begin[ci];
-        if (n > 0)
-          NSL_TIMED(t, kKRestitution, s, hipLaunchKernelGGL(k_restitution, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, cr.begin[ci], cr.end[ci], p));
-      }
-    }
-  }
-  if (n_overflow > 0) NSL_TIMED(t, kKOverflow, s, hipLaunchKernelGGL(k_overflow_store, dim3(blocks_for(n_overflow)), dim3(kBlock), 0, s, d, n_overflow));
-  int total = cr.end[kOverflowColor - 1];
-  if (total > 0) NSL_TIMED(t, kKStore, s, hipLaunchKernelGGL(k_store, dim3(blocks_for(total)), dim3(kBlock), 0, s, d, total));
-}
-
-void launch_finalize(const DeviceWorld& d, const StepParams& p, hipStream_t s, KernelTimer* t) {
-  int nb = blocks_for(d.n_awake);
-  if (nb > 0) NSL_TIMED(t, kKFinalize, s, hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(kBlock), 0, s, d, p));
+void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
+                       KernelTimer* t) {
+  DeviceWorld dw = d;
+  SolvePlan pl = plan;
+  StepParams pp = p;
+  void* args[] = {&dw, &pl, &pp};
+  hipError_t err = hipSuccess;
+  NSL_TIMED(t, kKSolve, s,
+            err = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_solve_step), dim3(blocks), dim3(kSolveBlock), args, 0, s));
+  if (err != hipSuccess) throw std::runtime_error(std::string("step kernel launch: ") + hipGetErrorString(err));
 }
 
 }  // namespace nsl

@@ -6,10 +6,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
-#include <unordered_set>
 #include <vector>
 
+#include "nsl_broadphase.h"
 #include "nsl_device.h"
 
 namespace nsl {
@@ -41,6 +42,7 @@ struct WorldOptions {
   float contact_speed = 3.0f, contact_hertz = 30.0f, contact_damping_ratio = 10.0f;
   float isotropic_inertia_tolerance = 0.2f;
   bool enable_sleep = true, enable_warm_starting = true, enable_continuous = true, enable_speculative = true;
+  int32_t solver_blocks = 0;  // workgroups of the step kernel; 0 = sized from the largest phase
 };
 
 // Wall-clock phases of the last step, in milliseconds (types/world_def.rs:158-184).
@@ -76,13 +78,17 @@ class GpuWorld {
   void get_contact_colors(int32_t* color, int32_t* local, int32_t* alive) const;
   const StepProfile& profile() const { return profile_; }
 
-  // Cumulative GPU time per kernel class (HIP events on the world's stream, only while
-  // timing is on) and device work counters, since the last reset.
+  // Device time per launch class (HIP events on the world's stream) and per phase of the
+  // step kernel (device wall clock at its grid barriers), only while timing is on; device
+  // work counters always; since the last reset.
   struct KernelStats {
     double ms[kKernelClassCount];
     int64_t launches[kKernelClassCount];
+    double phase_ms[kPhaseClassCount];
+    int64_t phase_count[kPhaseClassCount];
     int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
-    double phase_ms[4];  // host wall time: pair update, collide (with its wait), state pass, solve+finalize (with its wait)
+    int64_t barriers, solver_blocks, pair_tests;
+    double host_ms[4];  // host wall time: pair update, collide (with its wait), state pass, solve+finalize (with its wait)
   };
   void set_kernel_timing(bool on);
   const KernelStats& kernel_stats() const { return kstats_; }
@@ -126,6 +132,7 @@ class GpuWorld {
   template <class T> void ensure(T*& ptr, size_t& cap, size_t n, bool keep);
 
   void update_pairs();
+  void ensure_pair_set();
   void create_contact(int32_t a, int32_t b);
   void destroy_contact(int32_t id);
   void add_to_graph(int32_t id);
@@ -136,6 +143,12 @@ class GpuWorld {
   hipStream_t stream_ = nullptr;
   KernelTimer timer_{};
   bool timing_ = false;
+  GridBarrier* d_barrier_ = nullptr;
+  uint64_t* d_phase_log_ = nullptr;
+  std::vector<uint64_t> phase_log_host_;
+  GridBarrier barrier_host_{};
+  int max_blocks_ = 0;
+  double wall_clock_khz_ = 0.0;
   KernelStats kstats_{};
   int32_t counters_host_[8] = {};
   int32_t counters_base_[8] = {};
@@ -153,7 +166,6 @@ class GpuWorld {
   int32_t contact_pool_next_ = 0;
   std::vector<int32_t> non_touching_;
   ColorSet colors_[kColorCount];
-  std::unordered_set<uint64_t> pair_set_;
   std::vector<int32_t> init_ids_;
   std::vector<Contact> init_records_;
   bool bodies_dirty_ = true;   // host copies are newer than the device arrays
@@ -167,8 +179,19 @@ class GpuWorld {
   int32_t* d_init_ids_ = nullptr;
   Contact* d_init_records_ = nullptr;
   std::vector<uint8_t> change_host_;
-  std::vector<uint8_t> moved_host_;
+  std::vector<uint8_t> moved_host_;  // staging for uploads and pulls
   std::vector<Aabb> fat_host_;
+  // device pair finder and the pair-key set it checks (new contacts insert, disjoint ones erase)
+  std::unique_ptr<PairFinder> finder_;
+  Proxy* d_proxies_ = nullptr;
+  size_t cap_proxies_ = 0;
+  std::vector<Proxy> proxies_host_;
+  std::vector<uint64_t> new_keys_;
+  uint64_t* d_rebuild_keys_ = nullptr;
+  size_t cap_rebuild_keys_ = 0;
+  size_t n_device_shapes_ = 0;  // shapes whose fat AABB and moved flag live on the device
+  int64_t pair_set_tombs_ = 0;
+  bool pairs_pending_ = true;   // proxies marked moved on the host since the last pair update
   StepProfile profile_{};
 };
 

@@ -21,13 +21,6 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// hash_set.rs:25-40: (smaller shape << 42) | (larger shape << 20) | child
-uint64_t pair_key(int32_t s1, int32_t s2) {
-  const uint64_t mask = (1ull << 22) - 1;
-  uint64_t a = mask & (uint64_t)(int64_t)s1, b = mask & (uint64_t)(int64_t)s2;
-  return s1 < s2 ? (a << 42) | (b << 20) : (b << 42) | (a << 20);
-}
-
 // hull.rs:754-852 for the box; geometry_queries.rs:81-87 for its inertia.
 BoxHull make_box(Vec3 half, Mat3* central_inertia, float* volume) {
   const float min_h = 0.2f * kLinearSlop;
@@ -62,11 +55,22 @@ BoxHull make_box(Vec3 half, Mat3* central_inertia, float* volume) {
 GpuWorld::GpuWorld(int device) : device_(device) {
   NSL_CHECK(hipSetDevice(device_));
   NSL_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  timer_.capacity = 4096;
+  timer_.capacity = 64;
   timer_.events = new hipEvent_t[2 * timer_.capacity];
   for (int i = 0; i < 2 * timer_.capacity; ++i) NSL_CHECK(hipEventCreate(&timer_.events[i]));
   NSL_CHECK(hipMalloc(&d_.counters, 8 * sizeof(int32_t)));
   NSL_CHECK(hipMemset(d_.counters, 0, 8 * sizeof(int32_t)));
+  NSL_CHECK(hipMalloc(&d_barrier_, sizeof(GridBarrier)));
+  NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
+  NSL_CHECK(hipMalloc(&d_phase_log_, 2 * kMaxPhaseLog * sizeof(uint64_t)));
+  phase_log_host_.resize(2 * kMaxPhaseLog);
+  max_blocks_ = solve_step_max_blocks(device_);
+  if (max_blocks_ <= 0) throw std::runtime_error("step kernel does not fit on the device");
+  int khz = 0;
+  NSL_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
+  wall_clock_khz_ = (double)khz;
+  finder_ = std::make_unique<PairFinder>(stream_);
+  ensure_pair_set();
 }
 
 GpuWorld::~GpuWorld() {
@@ -74,7 +78,9 @@ GpuWorld::~GpuWorld() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   void* ptrs[] = {d_.sims, d_.states, d_.static_sims, d_.shapes, d_.fat_aabbs, d_.shape_moved, d_.hulls, d_.contacts,
                   d_.contact_change, d_.constraints, d_.overflow, d_.collide_ids, d_.solve_ids, d_.overflow_ids,
-                  d_.counters, d_init_ids_, d_init_records_};
+                  d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
+                  d_rebuild_keys_, d_.pairs.slots};
+  finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int i = 0; i < 2 * timer_.capacity; ++i) (void)hipEventDestroy(timer_.events[i]);
@@ -407,16 +413,29 @@ void GpuWorld::upload_bodies() {
     ensure(d_.shapes, cap_shapes_, std::max<size_t>(nsh, 1), false);
     ensure(d_.fat_aabbs, cap_fat_, std::max<size_t>(nsh, 1), false);
     ensure(d_.shape_moved, cap_moved_, std::max<size_t>(nsh, 1), false);
+    ensure(d_proxies_, cap_proxies_, std::max<size_t>(nsh, 1), false);
     ensure(d_.hulls, cap_hulls_, std::max<size_t>(hulls_.size(), 1), false);
     fat_host_.resize(nsh);
-    for (size_t i = 0; i < nsh; ++i) fat_host_[i] = hshapes_[i].fat;
+    moved_host_.resize(nsh);
+    proxies_host_.resize(nsh);
+    for (size_t i = 0; i < nsh; ++i) {
+      const HShape& hs = hshapes_[i];
+      fat_host_[i] = hs.fat;
+      moved_host_[i] = hs.proxy_moved ? 1 : 0;
+      pairs_pending_ = pairs_pending_ || hs.proxy_moved;
+      proxies_host_[i] = Proxy{hs.body, bodies_[hs.body].type, hs.group, 0, hs.category, hs.mask};
+    }
     if (nsh) {
       NSL_CHECK(hipMemcpyAsync(d_.shapes, shapes_.data(), nsh * sizeof(Shape), hipMemcpyHostToDevice, stream_));
       NSL_CHECK(hipMemcpyAsync(d_.fat_aabbs, fat_host_.data(), nsh * sizeof(Aabb), hipMemcpyHostToDevice, stream_));
-      NSL_CHECK(hipMemsetAsync(d_.shape_moved, 0, nsh, stream_));
+      NSL_CHECK(hipMemcpyAsync(d_.shape_moved, moved_host_.data(), nsh, hipMemcpyHostToDevice, stream_));
+      NSL_CHECK(hipMemcpyAsync(d_proxies_, proxies_host_.data(), nsh * sizeof(Proxy), hipMemcpyHostToDevice, stream_));
     }
     if (!hulls_.empty())
       NSL_CHECK(hipMemcpyAsync(d_.hulls, hulls_.data(), hulls_.size() * sizeof(BoxHull), hipMemcpyHostToDevice, stream_));
+    // the staging vectors are read by the copies above
+    NSL_CHECK(hipStreamSynchronize(stream_));
+    n_device_shapes_ = nsh;
     shapes_dirty_ = false;
   }
   bodies_dirty_ = false;
@@ -430,58 +449,63 @@ void GpuWorld::pull_bodies() {
     NSL_CHECK(hipMemcpyAsync(states_host_.data(), d_.states, na * sizeof(BodyState), hipMemcpyDeviceToHost, stream_));
     NSL_CHECK(hipStreamSynchronize(stream_));
   }
+  if (n_device_shapes_) {  // finalize moves fat AABBs and marks proxies on the device
+    fat_host_.resize(n_device_shapes_);
+    moved_host_.resize(n_device_shapes_);
+    NSL_CHECK(hipMemcpyAsync(fat_host_.data(), d_.fat_aabbs, n_device_shapes_ * sizeof(Aabb), hipMemcpyDeviceToHost, stream_));
+    NSL_CHECK(hipMemcpyAsync(moved_host_.data(), d_.shape_moved, n_device_shapes_, hipMemcpyDeviceToHost, stream_));
+    NSL_CHECK(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < n_device_shapes_; ++i) {
+      hshapes_[i].fat = fat_host_[i];
+      hshapes_[i].proxy_moved = moved_host_[i] != 0;
+    }
+  }
   for (size_t i = 0; i < awake_host_.size(); ++i) bodies_[awake_host_[i].body_id].flags = awake_host_[i].body_flags;
   device_newer_ = false;
 }
 
 // ---------------------------------------------------------------------------------
-// broad phase: contact_creation.rs:901-1042. New pairs are proxy pairs with
-// overlapping fat AABBs, at least one proxy moved since the last update, at least
-// one dynamic, not already paired, passing the shape filter; created in sorted key
-// order. A sweep along x over the fat AABBs finds exactly that set.
+// pair update: contact_creation.rs:901-1042. New pairs come from the device pair finder
+// (nsl_broadphase.h) as sorted keys; contacts are created in that order.
 
 void GpuWorld::update_pairs() {
-  bool any_moved = false;
-  for (const HShape& s : hshapes_) any_moved = any_moved || s.proxy_moved;
-  if (!any_moved) return;
-  struct Item { float lo; int32_t shape; };
-  std::vector<Item> items;
-  items.reserve(hshapes_.size());
-  for (const HShape& s : hshapes_) items.push_back(Item{s.fat.lo.x, s.id});
-  std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.lo < b.lo || (a.lo == b.lo && a.shape < b.shape); });
-  std::vector<uint64_t> keys;
-  std::vector<int32_t> active;
-  for (const Item& it : items) {
-    const HShape& sa = hshapes_[it.shape];
-    size_t w = 0;
-    for (size_t k = 0; k < active.size(); ++k) {
-      const HShape& sb = hshapes_[active[k]];
-      if (sb.fat.hi.x < sa.fat.lo.x) continue;
-      active[w++] = active[k];
-      if (!(sa.proxy_moved || sb.proxy_moved)) continue;
-      const int32_t ta = bodies_[sa.body].type, tb = bodies_[sb.body].type;
-      if (ta != kBodyDynamic && tb != kBodyDynamic) continue;
-      if (sa.body == sb.body) continue;
-      if (!aabb_overlaps(sa.fat, sb.fat)) continue;
-      uint64_t key = pair_key(sa.id, sb.id);
-      if (pair_set_.count(key)) continue;
-      // shape_runtime.rs:285-292
-      if (sa.group == sb.group && sa.group != 0) {
-        if (sa.group < 0) continue;
-      } else if (!((sa.mask & sb.category) != 0 && (sa.category & sb.mask) != 0)) {
-        continue;
-      }
-      keys.push_back(key);
-    }
-    active.resize(w);
-    active.push_back(it.shape);
-  }
+  if (!pairs_pending_ && counters_host_[6] == 0) return;
+  finder_->find(d_, d_proxies_, (int)n_device_shapes_, new_keys_);
   for (HShape& s : hshapes_) s.proxy_moved = false;
-  std::sort(keys.begin(), keys.end());
-  for (size_t i = 0; i < keys.size(); ++i) {
-    if (i > 0 && keys[i] == keys[i - 1]) continue;
-    create_contact((int32_t)((keys[i] >> 42) & ((1u << 22) - 1)), (int32_t)((keys[i] >> 20) & ((1u << 22) - 1)));
+  pairs_pending_ = false;
+  kstats_.pair_tests += finder_->last_tests();
+  for (uint64_t key : new_keys_)
+    create_contact((int32_t)((key >> 42) & ((1u << 22) - 1)), (int32_t)((key >> 20) & ((1u << 22) - 1)));
+}
+
+// Keeps the device pair-key set at a load factor of at most 1/2 (live keys plus
+// tombstones); rebuilds it from the live contacts when that would be exceeded.
+void GpuWorld::ensure_pair_set() {
+  const int64_t live = contact_count();
+  const uint32_t cap = d_.pairs.slots ? d_.pairs.mask + 1 : 0;
+  if (cap && 2 * (live + pair_set_tombs_) <= (int64_t)cap) return;
+  uint32_t want = 1024;
+  while ((int64_t)want < 4 * live) want <<= 1;
+  if (want != cap) {
+    if (d_.pairs.slots) {
+      NSL_CHECK(hipStreamSynchronize(stream_));
+      NSL_CHECK(hipFree(d_.pairs.slots));
+    }
+    NSL_CHECK(hipMalloc(&d_.pairs.slots, (size_t)want * sizeof(uint64_t)));
+    d_.pairs.mask = want - 1;
   }
+  NSL_CHECK(hipMemsetAsync(d_.pairs.slots, 0xFF, (size_t)want * sizeof(uint64_t), stream_));  // all kPairEmpty
+  std::vector<uint64_t> keys;
+  keys.reserve((size_t)live);
+  for (const HContact& c : hcontacts_)
+    if (c.alive) keys.push_back(pair_key(c.shape_a, c.shape_b));
+  if (!keys.empty()) {
+    ensure(d_rebuild_keys_, cap_rebuild_keys_, keys.size(), false);
+    NSL_CHECK(hipMemcpyAsync(d_rebuild_keys_, keys.data(), keys.size() * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+    launch_pairset_insert(d_.pairs, d_rebuild_keys_, (int)keys.size(), stream_);
+    NSL_CHECK(hipStreamSynchronize(stream_));  // `keys` is read by the copy
+  }
+  pair_set_tombs_ = 0;
 }
 
 // contact_creation.rs:219-425; the primary shape order is hull > capsule > sphere.
@@ -509,7 +533,6 @@ void GpuWorld::create_contact(int32_t a, int32_t b) {
   hc.restitution = maxf(sa.restitution, sb.restitution);
   hc.local = (int32_t)non_touching_.size();
   non_touching_.push_back(id);
-  pair_set_.insert(pair_key(a, b));
   Contact c{};
   c.shape_a = a;
   c.shape_b = b;
@@ -542,7 +565,7 @@ static int32_t swap_remove(std::vector<T>& v, size_t index) {
 void GpuWorld::destroy_contact(int32_t id) {
   HContact& c = hcontacts_[id];
   if (!c.alive) return;
-  pair_set_.erase(pair_key(c.shape_a, c.shape_b));
+  pair_set_tombs_ += 1;  // the collide kernel erased the key
   if (c.color == kNullIndex) {
     int32_t local = c.local;
     if (swap_remove(non_touching_, (size_t)local) != kNullIndex) hcontacts_[non_touching_[local]].local = local;
@@ -660,6 +683,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   ensure(d_.contacts, cap_contacts_, std::max<size_t>(ncap, 1), true);
   ensure(d_.contact_change, cap_change_, std::max<size_t>(ncap, 1), false);
   if (!init_ids_.empty()) {
+    ensure_pair_set();
     ensure(d_init_ids_, cap_init_ids_, init_ids_.size(), false);
     ensure(d_init_records_, cap_init_rec_, init_records_.size(), false);
     NSL_CHECK(hipMemcpyAsync(d_init_ids_, init_ids_.data(), init_ids_.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
@@ -721,20 +745,33 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     NSL_CHECK(hipMemcpyAsync(d_.overflow_ids, ovf.data(), ovf.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
   }
   KernelTimer* tm = timing_ ? &timer_ : nullptr;
-  launch_prepare(d_, (int)solve.size(), (int)ovf.size(), p, stream_, tm);
-  launch_solve(d_, cr, (int)ovf.size(), any_restitution, options.restitution_iterations, p, stream_, tm);
-  launch_finalize(d_, p, stream_, tm);
+  SolvePlan plan{};
+  plan.cr = cr;
+  plan.n_constraints = (int32_t)solve.size();
+  plan.n_overflow = (int32_t)ovf.size();
+  plan.any_restitution = any_restitution ? 1 : 0;
+  plan.restitution_iterations = options.restitution_iterations;
+  plan.barrier = d_barrier_;
+  plan.phase_log = timing_ ? d_phase_log_ : nullptr;
+  int widest = (d_.n_awake + 3) / 4;
+  for (int ci = 0; ci < kOverflowColor; ++ci) widest = std::max(widest, cr.end[ci] - cr.begin[ci]);
+  int blocks = options.solver_blocks > 0 ? options.solver_blocks : (widest + kSolveBlock - 1) / kSolveBlock;
+  blocks = std::max(1, std::min(blocks, std::min(max_blocks_, 128)));
+  launch_solve_step(d_, plan, p, blocks, stream_, tm);
   device_newer_ = true;
 
-  // moved proxies and their new fat AABBs feed the next pair update
-  const size_t nsh = hshapes_.size();
-  moved_host_.resize(nsh);
-  fat_host_.resize(nsh);
-  NSL_CHECK(hipMemcpyAsync(moved_host_.data(), d_.shape_moved, nsh, hipMemcpyDeviceToHost, stream_));
-  NSL_CHECK(hipMemcpyAsync(fat_host_.data(), d_.fat_aabbs, nsh * sizeof(Aabb), hipMemcpyDeviceToHost, stream_));
-  NSL_CHECK(hipMemsetAsync(d_.shape_moved, 0, nsh, stream_));
+  // counters[6] (proxies moved by finalize) tells the next pair update whether to run
   NSL_CHECK(hipMemcpyAsync(counters_host_, d_.counters, sizeof(counters_host_), hipMemcpyDeviceToHost, stream_));
+  NSL_CHECK(hipMemcpyAsync(&barrier_host_, d_barrier_, sizeof(GridBarrier), hipMemcpyDeviceToHost, stream_));
+  if (timing_)
+    NSL_CHECK(hipMemcpyAsync(phase_log_host_.data(), d_phase_log_, phase_log_host_.size() * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, stream_));
   NSL_CHECK(hipStreamSynchronize(stream_));
+  if (barrier_host_.timed_out) {
+    barrier_host_ = GridBarrier{};
+    NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
+    throw std::runtime_error("step kernel: grid barrier wait exceeded its limit (workgroups not co-resident?)");
+  }
   if (timing_) {
     for (int i = 0; i < timer_.used; ++i) {
       float ms = 0.0f;
@@ -743,6 +780,16 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
       kstats_.launches[timer_.classes[i]] += 1;
     }
     timer_.used = 0;
+    // marks: (class of the phase that just ended, wall clock); the first is the kernel start
+    int marks = 0;
+    for (int i = 0; i < kMaxPhaseLog; ++i, ++marks)
+      if (i > 0 && phase_log_host_[2 * i + 1] < phase_log_host_[2 * (i - 1) + 1]) break;
+    for (int i = 1; i < marks; ++i) {
+      const int cls = (int)phase_log_host_[2 * i];
+      if (cls < 0 || cls >= kPhaseClassCount) continue;
+      kstats_.phase_ms[cls] += (double)(phase_log_host_[2 * i + 1] - phase_log_host_[2 * (i - 1) + 1]) / wall_clock_khz_;
+      kstats_.phase_count[cls] += 1;
+    }
   }
   kstats_.steps += 1;
   kstats_.bodies += (int64_t)d_.n_awake;
@@ -754,17 +801,18 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   kstats_.recycled = counters_host_[3] - counters_base_[3];
   kstats_.points = counters_host_[4] - counters_base_[4];
   kstats_.overflow_points = counters_host_[5] - counters_base_[5];
-  for (size_t i = 0; i < nsh; ++i) {
-    if (moved_host_[i]) {
-      hshapes_[i].proxy_moved = true;
-      hshapes_[i].fat = fat_host_[i];
-    }
-  }
   const double t4 = now_ms();
-  kstats_.phase_ms[0] += t1 - t0;
-  kstats_.phase_ms[1] += t2 - t1;
-  kstats_.phase_ms[2] += t3 - t2;
-  kstats_.phase_ms[3] += t4 - t3;
+  kstats_.host_ms[0] += t1 - t0;
+  kstats_.host_ms[1] += t2 - t1;
+  kstats_.host_ms[2] += t3 - t2;
+  kstats_.host_ms[3] += t4 - t3;
+  kstats_.solver_blocks = blocks;
+  {
+    const int n_active = colors_used, ovf_phase = ovf.empty() ? 0 : 1;
+    int64_t barriers = 2 + (int64_t)p.sub_steps * (2 + 3 * (n_active + ovf_phase));
+    if (any_restitution) barriers += (int64_t)options.restitution_iterations * (n_active + ovf_phase);
+    kstats_.barriers += barriers;
+  }
   profile_.pairs = (float)(t1 - t0);
   profile_.collide = (float)(t2 - t1);
   profile_.state_pass = (float)(t3 - t2);

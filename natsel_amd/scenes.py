@@ -109,6 +109,87 @@ def many_pyramids(base_count: int = 10, rows: int = 14, cols: int = 14) -> Scene
     return sc
 
 
+PI = f32(3.14159265359)  # base/constants.rs:6
+
+
+def unwind_angle(radians: np.float32) -> np.float32:
+    """IEEE remainder by 2*pi, computed exactly in f64 (math/functions.rs:134-150)."""
+    x = float(radians)
+    period = float(f32(2.0) * PI)
+    n = float(round(x / period))  # ties to even
+    r = x - n * period
+    if r > 0.5 * period:
+        n += 1.0
+        r = x - n * period
+    elif r < -0.5 * period:
+        n -= 1.0
+        r = x - n * period
+    return f32(r)
+
+
+def compute_cos_sin(radians: np.float32) -> tuple:
+    """Rational cosine/sine approximation, normalised (math/functions.rs:101-130)."""
+    x = unwind_angle(radians)
+    pi2 = PI * PI
+    if x < f32(-0.5) * PI:
+        y = x + PI
+        y2 = y * y
+        c = -(pi2 - f32(4.0) * y2) / (pi2 + y2)
+    elif x > f32(0.5) * PI:
+        y = x - PI
+        y2 = y * y
+        c = -(pi2 - f32(4.0) * y2) / (pi2 + y2)
+    else:
+        y2 = x * x
+        c = (pi2 - f32(4.0) * y2) / (pi2 + y2)
+    if x < f32(0.0):
+        y = x + PI
+        s = f32(-16.0) * y * (PI - y) / (f32(5.0) * pi2 - f32(4.0) * y * (PI - y))
+    else:
+        s = f32(16.0) * x * (PI - x) / (f32(5.0) * pi2 - f32(4.0) * x * (PI - x))
+    mag = np.sqrt(s * s + c * c)
+    inv_mag = f32(1.0) / mag if mag > f32(0.0) else f32(0.0)
+    return f32(c * inv_mag), f32(s * inv_mag)
+
+
+def overflow_hub(ring_count: int = 5, per_ring: int = 5, enable_sleep: bool = True) -> Scene:
+    """A tall dynamic hub ringed by small boxes, so one body has more contacts than the
+    dynamic colours can hold and the overflow colour fills
+    (src/tests/simulation/overflow_color_tests.rs:16-79)."""
+    sc = Scene(name="overflow_hub", enable_sleep=enable_sleep)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -1.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (20.0, 1.0, 20.0)))
+    hx, hy, hz = f32(0.5), f32(2.5), f32(0.5)
+    hub = sc.add_body(BodySpec(DYNAMIC, (0.0, float(hy), 0.0)))
+    sc.add_shape(ShapeSpec(hub, BOX, (float(hx), float(hy), float(hz)), density=50.0))
+    nh = f32(0.2)
+    ring_radius = hx + nh - f32(0.03)
+    spacing = f32(0.5)
+    base_y = nh + f32(0.05)
+    for ring in range(ring_count):
+        y = base_y + spacing * f32(ring)
+        offset = PI / f32(per_ring) if ring & 1 else f32(0.0)
+        for slot in range(per_ring):
+            theta = offset + (f32(2.0) * PI * f32(slot)) / f32(per_ring)
+            c, s = compute_cos_sin(theta)
+            b = sc.add_body(BodySpec(DYNAMIC, (float(ring_radius * c), float(y), float(ring_radius * s))))
+            sc.add_shape(ShapeSpec(b, BOX, (float(nh),) * 3))
+    return sc
+
+
+def graph_scene() -> Scene:
+    """Three spheres and a box on a static slab (src/tests/graph_tests.rs:33-66)."""
+    sc = Scene(name="graph_scene", enable_sleep=True)
+    g = sc.add_body(BodySpec(STATIC, (0.0, 0.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (10.0, 0.5, 10.0), density=0.0))
+    for p in ((0.0, 0.9, 0.0), (0.6, 0.9, 0.0), (0.0, 2.0, 0.0)):
+        b = sc.add_body(BodySpec(DYNAMIC, tuple(float(f32(v)) for v in p)))
+        sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, 0.5), density=1000.0))
+    b = sc.add_body(BodySpec(DYNAMIC, (float(f32(-0.9)), float(f32(0.9)), 0.0)))
+    sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), density=1000.0))
+    return sc
+
+
 class XorShift32:
     """Seeded xorshift32 (Marsaglia) for the synthetic scenes."""
 

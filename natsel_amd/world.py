@@ -77,6 +77,7 @@ class WorldOptions:  # world_runtime.rs:1243-1304
     enable_continuous: bool = True
     enable_warm_starting: bool = True
     enable_speculative: bool = True
+    solver_blocks: int = 0  # workgroups of the step kernel; 0 = sized from the largest phase
 
 
 def _material(d: ShapeDef) -> capi.Material:
@@ -100,6 +101,7 @@ class World:
         c.restitution_iterations = o.restitution_iterations
         c.enable_sleep, c.enable_continuous = int(o.enable_sleep), int(o.enable_continuous)
         c.enable_warm_starting, c.enable_speculative = int(o.enable_warm_starting), int(o.enable_speculative)
+        c.solver_blocks = int(o.solver_blocks)
         return c
 
     @property
@@ -199,10 +201,12 @@ class World:
     def kernel_stats(self) -> dict:
         k = capi.KernelStats()
         capi.check(self._lib.nsl_world_get_kernel_stats(self._h, ctypes.byref(k)))
-        out = {n: getattr(k, n) for n, _ in capi.KernelStats._fields_[2:-1]}
-        out["phase_ms"] = dict(zip(["pairs", "collide", "state_pass", "solve_finalize"], list(k.phase_ms)))
+        out = {n: getattr(k, n) for n, _ in capi.KernelStats._fields_[4:-1]}
         out["ms"] = dict(zip(capi.KERNEL_CLASSES, list(k.ms)))
         out["launches"] = dict(zip(capi.KERNEL_CLASSES, list(k.launches)))
+        out["phase_ms"] = dict(zip(capi.PHASE_CLASSES, list(k.phase_ms)))
+        out["phase_count"] = dict(zip(capi.PHASE_CLASSES, list(k.phase_count)))
+        out["host_ms"] = dict(zip(capi.HOST_PHASES, list(k.host_ms)))
         return out
 
     def profile(self) -> dict:
@@ -211,9 +215,9 @@ class World:
         return {name: getattr(p, name) for name, _ in capi.Profile._fields_}
 
 
-def load_scene(scene: Scene, device: int = 0) -> World:
+def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0) -> World:
     """Build a World from a seeded scene description (natsel_amd.scenes)."""
-    opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous)
+    opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous, solver_blocks=solver_blocks)
     w = World(device, opts)
     ids = []
     for b in scene.bodies:

@@ -82,3 +82,48 @@ def test_small_mixed_pile_every_step_matches_oracle(gpu_available):
         o.step(1.0 / 60.0, 4)
         hg, ho = g.world_state_hash(), o.world_state_hash()
         assert hg == ho, f"step {k}: {hg:x} != {ho:x}; {_first_difference(g, o)}"
+
+
+def test_overflow_hub_every_step_matches_oracle(gpu_available):
+    """The overflow colour (solved in list order by one lane) against the oracle, and the
+    reference's known counts after 10 steps (overflow_color_tests.rs:82-103)."""
+    sc = scenes.overflow_hub(enable_sleep=False)
+    g, o = _pair(sc)
+    for k in range(60):
+        g.step(1.0 / 60.0, 4)
+        o.step(1.0 / 60.0, 4)
+        hg, ho = g.world_state_hash(), o.world_state_hash()
+        assert hg == ho, f"step {k}: {hg:x} != {ho:x}; {_first_difference(g, o)}"
+        if k == 9:
+            color, _, alive = g.contact_colors()
+            assert int(alive.sum()) == 65
+            assert int(((color == 23) & (alive == 1)).sum()) == 5
+
+
+def test_graph_scene_colours_on_gpu(gpu_available):
+    """graph_tests.rs:152-172 colours of the touching contacts after the first step."""
+    from natsel_amd.world import load_scene
+
+    w = load_scene(scenes.graph_scene())
+    w.step(1.0 / 60.0, 4)
+    color, local, alive = w.contact_colors()
+    assert len(color) == 8 and alive.all()
+    for cid, (c, l) in {0: (22, 0), 1: (22, 1), 2: (22, 2), 3: (0, 0), 5: (1, 0)}.items():
+        assert (int(color[cid]), int(local[cid])) == (c, l), cid
+
+
+def test_step_kernel_workgroup_count_does_not_change_results(gpu_available):
+    """The step kernel's grid-stride loops and grid barriers give the same bits for any
+    number of workgroups."""
+    from natsel_amd.world import load_scene
+
+    sc = scenes.pile(300, seed=7, size_x=6, size_y=10, size_z=6)
+    worlds = [load_scene(sc, solver_blocks=b) for b in (1, 2, 7, 32)]
+    for k in range(90):
+        hashes = set()
+        for w in worlds:
+            w.step(1.0 / 60.0, 4)
+            hashes.add(w.world_state_hash())
+        assert len(hashes) == 1, f"step {k}: {sorted(hashes)}"
+    assert worlds[0].kernel_stats()["solver_blocks"] == 1
+    assert worlds[3].kernel_stats()["solver_blocks"] == 32

@@ -1,0 +1,55 @@
+"""The oracle against the reference's own known-answer tests (CPU only).
+
+Each test rebuilds the reference test's scene through natsel_amd.scenes and checks the
+values the reference test asserts.
+"""
+import numpy as np
+
+from natsel_amd import scenes
+
+
+def _oracle(scene):
+    from oracle import pyoracle
+
+    return pyoracle.load_scene(scene)
+
+
+def test_overflow_hub_fills_overflow_colour():
+    """src/tests/simulation/overflow_color_tests.rs:82-103: after 10 steps the overflow
+    colour holds 5 contacts, with 27 bodies and 65 contacts."""
+    sc = scenes.overflow_hub()
+    assert len(sc.bodies) == 27
+    w = _oracle(sc)
+    for _ in range(10):
+        w.step(1.0 / 60.0, 4)
+    color, _, alive = w.contact_colors()
+    assert int(alive.sum()) == 65
+    assert int(((color == 23) & (alive == 1)).sum()) == 5
+
+
+def test_graph_colours_and_local_indices():
+    """src/tests/graph_tests.rs:152-172: colour and local index of each touching contact
+    after the first pair update and narrow phase. (The reference test adds contacts to
+    the graph without removing them from the non-touching list, so only the touching
+    contacts' indices carry over to a full step.)"""
+    w = _oracle(scenes.graph_scene())
+    w.step(1.0 / 60.0, 4)
+    color, local, alive = w.contact_colors()
+    assert len(color) == 8 and alive.all()
+    expected = {0: (22, 0), 1: (22, 1), 2: (22, 2), 3: (0, 0), 5: (1, 0)}
+    for cid, (c, l) in expected.items():
+        assert (int(color[cid]), int(local[cid])) == (c, l), cid
+    for cid in (4, 6, 7):
+        assert color[cid] == -1
+    used = np.unique(color[color >= 0])
+    assert list(used) == [0, 1, 22]
+
+
+def test_cos_sin_matches_closed_form():
+    """compute_cos_sin is a rational approximation: within 1e-3 of cos/sin, exact at the
+    quadrant points, unit length."""
+    for t in np.linspace(-7.0, 7.0, 141, dtype=np.float32):
+        c, s = scenes.compute_cos_sin(np.float32(t))
+        assert abs(float(c) - np.cos(float(t))) < 2e-3 and abs(float(s) - np.sin(float(t))) < 2e-3
+        assert abs(float(c) * float(c) + float(s) * float(s) - 1.0) < 1e-6
+    assert scenes.compute_cos_sin(np.float32(0.0)) == (1.0, 0.0)
