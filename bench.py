@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -62,25 +63,54 @@ def solve_bytes(ks: dict, substeps: int) -> float:
     return sum(stage_bytes(k, ks, substeps) for k in BYTE_MODEL if k != "collide")
 
 
-def cpu_baseline(scene, seconds: float, max_steps: int) -> dict:
+TRAFFIC_PROFILE = os.path.join("profiles", "k_solve_step_pile10k_pmc_bytes.md")
+
+
+def measured_traffic():
+    """HBM-side bytes per step-kernel launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by a copy kernel of known size), or None."""
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
+    return float(m.group(1)) if m else None
+
+
+def _oracle_rate(scene, threads: int, seconds: float, max_steps: int):
     from oracle import pyoracle
 
+    pyoracle.set_threads(threads)
     w = pyoracle.load_scene(scene)
-    n = scene.dynamic_count
     t0 = time.perf_counter()
     steps = 0
     while steps < max_steps and time.perf_counter() - t0 < seconds:
         w.step(scene.dt, scene.substeps)
         steps += 1
     dt = time.perf_counter() - t0
+    return scene.dynamic_count * scene.substeps * steps / dt, steps, dt, w.phase_ms()
+
+
+def cpu_baseline(scene, seconds: float, max_steps: int) -> dict:
+    """The oracle (a C++ restatement of the reference step, OpenMP over the loops whose
+    items are independent) on this host: all threads for ~70% of the budget, then one
+    thread for the rest, each from step 1 of a fresh world."""
+    from oracle import pyoracle
+
+    threads = pyoracle.max_threads()
+    value, steps, dt, phases = _oracle_rate(scene, threads, 0.7 * seconds, max_steps)
+    value1, steps1, dt1, _ = _oracle_rate(scene, 1, 0.3 * seconds, max_steps)
     return {
-        "value": n * scene.substeps * steps / dt,
+        "value": value,
         "unit": "body-substeps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle (C++ restatement of the reference step, g++ -O2, 1 thread), steps 1..{steps} of the same pile",
+        "sample": f"CPU oracle (C++ restatement of the reference step, g++ -O3, OpenMP, {threads} threads), "
+                  f"steps 1..{steps} of the same pile",
         "steps": steps,
         "seconds": dt,
+        "phase_ms_per_step": {k: round(v / max(steps, 1), 3) for k, v in phases.items()},
+        "single_thread": {"value": value1, "steps": steps1, "seconds": dt1},
     }
 
 
@@ -93,12 +123,24 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups of the step kernel (0: sized from the largest phase)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only copy this many MiB device to device 5 times (reference kernel for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)", file=sys.stderr)
         return 2
 
     import torch
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.calibrate_copy_mib, "launches": 5}))
+        return 0
 
     from natsel_amd import scenes
     from natsel_amd.world import load_scene
@@ -119,6 +161,7 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     world.set_kernel_timing(False)
     ks = world.kernel_stats()
+    world.close()  # release device memory now, not at interpreter exit
     n = scene.dynamic_count
     value = n * scene.substeps * args.steps / elapsed
 
@@ -134,7 +177,8 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(),
+        "traffic_source": TRAFFIC_PROFILE,
         "kernel": "k_solve_step",
         "launches": ks["launches"]["solve"],
         "avg_launch_us": 1e3 * ms["solve"] / launches,

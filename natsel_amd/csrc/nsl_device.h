@@ -90,7 +90,8 @@ constexpr int kMaxPhaseLog = 1024;
 void launch_init_contacts(const DeviceWorld& d, const int32_t* ids, const Contact* records, int n, hipStream_t s);
 void launch_collide(const DeviceWorld& d, int n, hipStream_t s, KernelTimer* timer);
 void launch_pairset_insert(PairSet set, const uint64_t* keys, int n, hipStream_t s);
-// One cooperative launch: prepare, the substep schedule, restitution, store and finalize.
+// One launch (all workgroups resident): prepare, the substep schedule, restitution, store
+// and finalize.
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
                        KernelTimer* timer);
 // Workgroups of the step kernel that fit on the device at once.

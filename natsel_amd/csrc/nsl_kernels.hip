@@ -498,18 +498,30 @@ __device__ __forceinline__ void integrate_positions_item(const DeviceWorld& d, i
 // coloured stages: one lane per constraint slot of one colour; the colour's slots touch
 // distinct dynamic bodies, so they run in any order
 
+// Each item takes its constraint's body indices from the caller, which reads them before
+// the barrier that opens the phase (they do not change during the solve): the body
+// states and the constraint's fields are then all requested at once. The manifold points
+// are read unconditionally (the slot always holds four) and used up to point_count.
+
 // contact_solver.rs:697-763
-__device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot) {
+__device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot, int32_t ia, int32_t ib) {
+  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
   const Constraint& c = d.constraints[slot];
+  const int pc = c.point_count;
+  ConstraintPoint pt[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pt[i] = c.points[i];
   Vec3 linear = zero3(), ang_a = zero3(), ang_b = zero3();
   if (c.manifold_count > 0) {
     float total = 0.0f;
     Vec3 moment_a = zero3(), moment_b = zero3();
-    for (int i = 0; i < c.point_count; ++i) {
-      float ni = c.points[i].normal_impulse;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= pc) break;
+      float ni = pt[i].normal_impulse;
       total += ni;
-      moment_a = mul_add(moment_a, ni, c.points[i].r_a);
-      moment_b = mul_add(moment_b, ni, c.points[i].r_b);
+      moment_a = mul_add(moment_a, ni, pt[i].r_a);
+      moment_b = mul_add(moment_b, ni, pt[i].r_b);
     }
     Vec3 normal = c.normal;
     Vec3 fi = scale(c.friction_impulse.x, c.tangent1);
@@ -526,19 +538,22 @@ __device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot) 
     ang_a = add(ang_a, a_a);
     ang_b = add(ang_b, a_b);
   }
-  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
   ba.w = sub(ba.w, sym_mv(c.inv_ia, ang_a));
   ba.v = sub(ba.v, scale(c.inv_mass_a, linear));
   bb.w = add(bb.w, sym_mv(c.inv_ib, ang_b));
   bb.v = add(bb.v, scale(c.inv_mass_b, linear));
-  store_state(d, c.index_a, ba);
-  store_state(d, c.index_b, bb);
+  store_state(d, ia, ba);
+  store_state(d, ib, bb);
 }
 
 // contact_solver.rs:796-894
-__device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const StepParams& p) {
+__device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib) {
+  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
   Constraint& c = d.constraints[slot];
-  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
+  const int pc = c.point_count;
+  ConstraintPoint pt[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pt[i] = c.points[i];
   const Softness sf = c.is_static ? p.static_softness : p.contact_softness;
   const float bias_rate = sf.mass_scale * sf.bias_rate;
   const float mass_scale = sf.mass_scale, impulse_scale = sf.impulse_scale;
@@ -551,8 +566,10 @@ __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const 
     Vec3 n_a = inv_rotate_wide(ba.dq, normal), n_b = inv_rotate_wide(bb.dq, normal);
     float lnv = dot(normal, sub(bb.v, ba.v));
     float manifold_impulse = 0.0f;
-    for (int i = 0; i < c.point_count; ++i) {
-      ConstraintPoint& cp = c.points[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= pc) break;
+      const ConstraintPoint& cp = pt[i];
       float s = (normal_sep + (dot(n_b, cp.r_b) - dot(n_a, cp.r_a))) + cp.base_separation;
       bool separated = s > 0.0f;
       float velocity_bias = separated ? s * inv_h : maxf(bias_rate * s, speed_neg);
@@ -563,7 +580,7 @@ __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const 
       float v_n = lnv + (dot(u_b, bb.w) - dot(u_a, ba.w));
       float neg_impulse = cp.normal_mass * (pm * v_n + velocity_bias) + pi * cp.normal_impulse;
       float impulse = maxf(-neg_impulse, -cp.normal_impulse);
-      cp.normal_impulse = cp.normal_impulse + impulse;
+      c.points[i].normal_impulse = cp.normal_impulse + impulse;
       lnv += inv_mass * impulse;
       manifold_impulse += impulse;
       ba.w = sub(ba.w, scale(impulse, g_a));
@@ -573,14 +590,18 @@ __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const 
     ba.v = sub(ba.v, scale(c.inv_mass_a, pp));
     bb.v = add(bb.v, scale(c.inv_mass_b, pp));
   }
-  store_state(d, c.index_a, ba);
-  store_state(d, c.index_b, bb);
+  store_state(d, ia, ba);
+  store_state(d, ib, bb);
 }
 
 // contact_solver.rs:927-1097
-__device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const StepParams& p) {
+__device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib) {
+  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
   Constraint& c = d.constraints[slot];
-  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
+  const int pc = c.point_count;
+  ConstraintPoint pt[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pt[i] = c.points[i];
   const float inv_h = p.inv_h, eps = kFltEps;
   Vec3 dp = sub(bb.dp, ba.dp);
   float inv_mass = c.inv_mass_a + c.inv_mass_b;
@@ -593,8 +614,10 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
     float ltv1 = dot(c.tangent1, dv) - c.tangent_velocity1;
     float ltv2 = dot(c.tangent2, dv) - c.tangent_velocity2;
     float manifold_impulse = 0.0f, total_normal = 0.0f, total_twist_limit = 0.0f;
-    for (int i = 0; i < c.point_count; ++i) {
-      ConstraintPoint& cp = c.points[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= pc) break;
+      const ConstraintPoint& cp = pt[i];
       float s = (normal_sep + (dot(n_b, cp.r_b) - dot(n_a, cp.r_a))) + cp.base_separation;
       float velocity_bias = maxf(0.0f, s * inv_h);
       Vec3 u_a = cross(cp.r_a, normal), u_b = cross(cp.r_b, normal);
@@ -603,8 +626,8 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       float neg_impulse = cp.normal_mass * (v_n + velocity_bias);
       float impulse = maxf(-neg_impulse, -cp.normal_impulse);
       float new_impulse = cp.normal_impulse + impulse;
-      cp.normal_impulse = new_impulse;
-      cp.total_normal_impulse = cp.total_normal_impulse + new_impulse;
+      c.points[i].normal_impulse = new_impulse;
+      c.points[i].total_normal_impulse = cp.total_normal_impulse + new_impulse;
       total_normal += new_impulse;
       total_twist_limit += cp.lever_arm * new_impulse;
       lnv += inv_mass * impulse;
@@ -615,13 +638,14 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
     if (c.rolling_resistance > 0.0f) {
       Vec3 delta = sym_mv(c.rolling_mass, sub(ba.w, bb.w));
       Vec3 old = c.rolling_impulse;
-      c.rolling_impulse = add(old, delta);
+      Vec3 ri = add(old, delta);
       float max_impulse = c.rolling_resistance * total_normal;
-      float lsq = dot(c.rolling_impulse, c.rolling_impulse);
+      float lsq = dot(ri, ri);
       bool mask = lsq > eps + max_impulse * max_impulse;
       float nrm = max_impulse / (sqrtf_(lsq) + eps);
-      c.rolling_impulse = scale(mask ? nrm : 1.0f, c.rolling_impulse);
-      delta = sub(c.rolling_impulse, old);
+      ri = scale(mask ? nrm : 1.0f, ri);
+      c.rolling_impulse = ri;
+      delta = sub(ri, old);
       ba.w = sub(ba.w, sym_mv(c.inv_ia, delta));
       bb.w = add(bb.w, sym_mv(c.inv_ib, delta));
     }
@@ -641,14 +665,15 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       Vec2 vt = {ltv1 + dot(arv, t1), ltv2 + dot(arv, t2)};
       Vec2 di = mul_mv2(c.tangent_mass, vt);
       di = Vec2{-di.x, -di.y};
-      Vec2 ni = {c.friction_impulse.x + di.x, c.friction_impulse.y + di.y};
+      const Vec2 fi_old = c.friction_impulse;
+      Vec2 ni = {fi_old.x + di.x, fi_old.y + di.y};
       float max_impulse = c.friction * total_normal;
       float lsq = ni.x * ni.x + ni.y * ni.y;
       bool mask = lsq > max_impulse * max_impulse;
       float nrm = max_impulse / (sqrtf_(lsq) + eps);
       float sc = mask ? nrm : 1.0f;
       ni = Vec2{sc * ni.x, sc * ni.y};
-      di = Vec2{ni.x - c.friction_impulse.x, ni.y - c.friction_impulse.y};
+      di = Vec2{ni.x - fi_old.x, ni.y - fi_old.y};
       c.friction_impulse = ni;
       Vec3 pp = scale(di.x, t1);
       pp = add(pp, scale(di.y, t2));
@@ -659,16 +684,20 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       bb.v = add(bb.v, scale(c.inv_mass_b, pp));
     }
   }
-  store_state(d, c.index_a, ba);
-  store_state(d, c.index_b, bb);
+  store_state(d, ia, ba);
+  store_state(d, ib, bb);
 }
 
 // contact_solver.rs:1135-1229
-__device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot, const StepParams& p) {
+__device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib) {
   Constraint& c = d.constraints[slot];
   const bool propagate = p.enable_restitution_propagation != 0;
   if (!propagate && c.restitution == 0.0f) return;
-  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
+  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
+  const int pc = c.point_count;
+  ConstraintPoint pt[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pt[i] = c.points[i];
   const bool rmask = c.restitution > 0.0f;
   const float neg_threshold = -p.restitution_threshold;
   Vec3 dp = sub(bb.dp, ba.dp);
@@ -676,8 +705,10 @@ __device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot,
     Vec3 normal = c.normal;
     float normal_sep = dot(normal, dp);
     Vec3 n_a = inv_rotate_wide(ba.dq, normal), n_b = inv_rotate_wide(bb.dq, normal);
-    for (int i = 0; i < c.point_count; ++i) {
-      ConstraintPoint& cp = c.points[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= pc) break;
+      const ConstraintPoint& cp = pt[i];
       float s = (normal_sep + (dot(n_b, cp.r_b) - dot(n_a, cp.r_a))) + cp.base_separation;
       float normal_mass = propagate ? cp.normal_mass : (rmask ? cp.normal_mass : 0.0f);
       float compression = cp.total_normal_impulse - cp.restitution_impulse;
@@ -692,9 +723,9 @@ __device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot,
       float allowance = c.restitution * (compression + approach) - cp.restitution_impulse;
       float max_impulse = approach + maxf(allowance, 0.0f);
       impulse = armed ? minf(impulse, max_impulse) : impulse;
-      cp.normal_impulse = cp.normal_impulse + impulse;
-      cp.restitution_impulse = cp.restitution_impulse + (impulse - approach);
-      cp.total_normal_impulse = cp.total_normal_impulse + impulse;
+      c.points[i].normal_impulse = cp.normal_impulse + impulse;
+      c.points[i].restitution_impulse = cp.restitution_impulse + (impulse - approach);
+      c.points[i].total_normal_impulse = cp.total_normal_impulse + impulse;
       Vec3 pp = scale(impulse, normal);
       ba.w = sub(ba.w, sym_mv(c.inv_ia, cross(cp.r_a, pp)));
       ba.v = sub(ba.v, scale(c.inv_mass_a, pp));
@@ -702,8 +733,8 @@ __device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot,
       bb.v = add(bb.v, scale(c.inv_mass_b, pp));
     }
   }
-  store_state(d, c.index_a, ba);
-  store_state(d, c.index_b, bb);
+  store_state(d, ia, ba);
+  store_state(d, ib, bb);
 }
 
 // contact_solver.rs:1258-1299
@@ -997,12 +1028,12 @@ __device__ __forceinline__ void finalize_item(const DeviceWorld& d, int i, const
 }
 
 // ------------------------------------------------------------------------------
-// the step kernel: one cooperative launch per step
+// the step kernel: one launch per step, all workgroups resident
 
 // Polls of a barrier wait before the kernel gives up (a few seconds; far above any phase).
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 
-// Grid-wide barrier of a cooperative launch (every workgroup is resident). The last
+// Grid-wide barrier over a launch whose workgroups are all resident. The last
 // workgroup to arrive resets the count and advances the generation; agent-scope
 // release/acquire make every store issued before the barrier visible to every load after
 // it, across the XCDs' L2 caches. A wait that exceeds the spin limit raises `timed_out`
@@ -1062,6 +1093,42 @@ struct PhaseClock {
   }
 };
 
+// The non-empty colours in order with their slot ranges, kept in LDS (a table indexed at
+// run time in private memory would cost a memory round trip per phase).
+struct ColourTable {
+  int n;
+  int begin[kOverflowColor];
+  int end[kOverflowColor];
+};
+
+// Body indices of a lane's first slot in active colour `a` ({-1, -1} past its end).
+__device__ __forceinline__ int2 first_indices(const DeviceWorld& d, const ColourTable& t, int a, int tid) {
+  const int slot = t.begin[a] + tid;
+  if (slot >= t.end[a]) return make_int2(kNullIndex, kNullIndex);
+  return make_int2(d.constraints[slot].index_a, d.constraints[slot].index_b);
+}
+
+// One stage over the active colours in order, a grid barrier after each. A lane's body
+// indices for its first slot in the next colour are read before the barrier, so each
+// phase opens with its body-state loads.
+template <class Item>
+__device__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid, int nthreads, GridBarrier* bar, PhaseClock& clock,
+                            int cls, Item item) {
+  const int n_active = t.n;
+  if (n_active == 0) return true;
+  int2 pf = first_indices(d, t, 0, tid);
+  for (int a = 0; a < n_active; ++a) {
+    const int end = t.end[a];
+    int slot = t.begin[a] + tid;
+    if (slot < end) item(slot, pf.x, pf.y);
+    for (slot += nthreads; slot < end; slot += nthreads) item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
+    if (a + 1 < n_active) pf = first_indices(d, t, a + 1, tid);
+    if (!grid_sync(bar)) return false;
+    clock.mark(cls);
+  }
+  return true;
+}
+
 // Stage schedule of one step (solver_stages.rs:543-589, solver_integration.rs:807-1015):
 // prepare; per substep: integrate velocities, warm start, push, integrate positions,
 // relax, each stage running the overflow list first and then colours 0..22 in order;
@@ -1076,11 +1143,18 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   PhaseClock clock{plan.phase_log, 0};
   clock.mark(-1);
 
-  // colours with work, in order
-  int active[kOverflowColor];
-  int n_active = 0;
-  for (int ci = 0; ci < kOverflowColor; ++ci)
-    if (plan.cr.end[ci] > plan.cr.begin[ci]) active[n_active++] = ci;
+  __shared__ ColourTable colours;
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int ci = 0; ci < kOverflowColor; ++ci)
+      if (plan.cr.end[ci] > plan.cr.begin[ci]) {
+        colours.begin[n] = plan.cr.begin[ci];
+        colours.end[n] = plan.cr.end[ci];
+        ++n;
+      }
+    colours.n = n;
+  }
+  __syncthreads();
 
   {
     int points = 0, ovf_points = 0;
@@ -1102,24 +1176,18 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
-    for (int a = 0; a < n_active; ++a) {
-      const int ci = active[a];
-      for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) warm_start_item(d, slot);
-      if (!grid_sync(bar)) return;
-      clock.mark(kPWarmStart);
-    }
+    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPWarmStart,
+                     [&](int slot, int32_t ia, int32_t ib) { warm_start_item(d, slot, ia, ib); }))
+      return;
 
     if (n_ovf > 0) {
       if (lane0) overflow_solve_all(d, n_ovf, p, 0);
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
-    for (int a = 0; a < n_active; ++a) {
-      const int ci = active[a];
-      for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) push_item(d, slot, p);
-      if (!grid_sync(bar)) return;
-      clock.mark(kPPush);
-    }
+    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPPush,
+                     [&](int slot, int32_t ia, int32_t ib) { push_item(d, slot, p, ia, ib); }))
+      return;
 
     for (int i = tid; i < d.n_awake; i += nthreads) integrate_positions_item(d, i, p);
     if (!grid_sync(bar)) return;
@@ -1130,12 +1198,9 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
-    for (int a = 0; a < n_active; ++a) {
-      const int ci = active[a];
-      for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) relax_item(d, slot, p);
-      if (!grid_sync(bar)) return;
-      clock.mark(kPRelax);
-    }
+    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRelax,
+                     [&](int slot, int32_t ia, int32_t ib) { relax_item(d, slot, p, ia, ib); }))
+      return;
   }
 
   if (plan.any_restitution) {
@@ -1145,12 +1210,9 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         if (!grid_sync(bar)) return;
         clock.mark(kPOverflow);
       }
-      for (int a = 0; a < n_active; ++a) {
-        const int ci = active[a];
-        for (int slot = plan.cr.begin[ci] + tid; slot < plan.cr.end[ci]; slot += nthreads) restitution_item(d, slot, p);
-        if (!grid_sync(bar)) return;
-        clock.mark(kPRestitution);
-      }
+      if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRestitution,
+                       [&](int slot, int32_t ia, int32_t ib) { restitution_item(d, slot, p, ia, ib); }))
+        return;
     }
   }
 
@@ -1197,13 +1259,12 @@ int solve_step_max_blocks(int device) {
 
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
                        KernelTimer* t) {
-  DeviceWorld dw = d;
-  SolvePlan pl = plan;
-  StepParams pp = p;
-  void* args[] = {&dw, &pl, &pp};
-  hipError_t err = hipSuccess;
-  NSL_TIMED(t, kKSolve, s,
-            err = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_solve_step), dim3(blocks), dim3(kSolveBlock), args, 0, s));
+  // An ordinary launch: `blocks` is clamped by the caller to what fits on the device at
+  // once, so every workgroup is resident while the step runs on its own stream (a
+  // workgroup that never arrives at a barrier would end the launch with the timeout
+  // error, not a hang).
+  NSL_TIMED(t, kKSolve, s, hipLaunchKernelGGL(k_solve_step, dim3(blocks), dim3(kSolveBlock), 0, s, d, plan, p));
+  const hipError_t err = hipGetLastError();
   if (err != hipSuccess) throw std::runtime_error(std::string("step kernel launch: ") + hipGetErrorString(err));
 }
 

@@ -51,8 +51,20 @@ def lib():
         L.nslo_contact_capacity.argtypes = [P]
         ip = ctypes.POINTER(ctypes.c_int32)
         L.nslo_get_contact_colors.argtypes = [P, ip, ip, ip]
+        L.nslo_set_threads.argtypes = [ctypes.c_int]
+        L.nslo_get_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+        L.nslo_max_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP loops (its results do not depend on this)."""
+    lib().nslo_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().nslo_max_threads())
 
 
 class OracleWorld:
@@ -114,6 +126,11 @@ class OracleWorld:
         keys = ["pairs_created", "contacts_destroyed", "begin_touch", "end_touch", "recycled", "sat_calls", "sat_hits",
                 "fast_bodies", "overflow_constraints"]
         return dict(zip(keys, list(out)))
+
+    def phase_ms(self) -> dict:
+        out = (ctypes.c_double * 5)()
+        self.L.nslo_get_phase_ms(self.h, out)
+        return dict(zip(["pairs", "collide", "prepare", "solve", "finalize"], list(out)))
 
     def contact_colors(self):
         n = self.L.nslo_contact_capacity(self.h)

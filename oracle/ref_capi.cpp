@@ -3,11 +3,17 @@
 #include <cstdint>
 #include <cstring>
 
+#include <omp.h>
+
 #include "ref_world.h"
 
 using namespace nslref;
 
 extern "C" {
+
+// Threads for the OpenMP loops (results do not depend on it).
+void nslo_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int nslo_max_threads(void) { return omp_get_max_threads(); }
 
 void* nslo_create(void) { return new World(); }
 void nslo_destroy(void* w) { delete static_cast<World*>(w); }
@@ -85,6 +91,12 @@ void nslo_get_stats(void* wp, int64_t* out) {
   const StepStats& s = static_cast<World*>(wp)->stats;
   out[0] = s.pairs_created; out[1] = s.contacts_destroyed; out[2] = s.begin_touch; out[3] = s.end_touch;
   out[4] = s.recycled; out[5] = s.sat_calls; out[6] = s.sat_hits; out[7] = s.fast_bodies; out[8] = s.overflow_constraints;
+}
+
+// Cumulative phase wall times (ms): pairs, collide, prepare, solve, finalize.
+void nslo_get_phase_ms(void* wp, double* out) {
+  const StepStats& s = static_cast<World*>(wp)->stats;
+  out[0] = s.ms_pairs; out[1] = s.ms_collide; out[2] = s.ms_prepare; out[3] = s.ms_solve; out[4] = s.ms_finalize;
 }
 
 // Graph coloring snapshot: per contact id, its color (-1 when not in the graph) and local index.

@@ -565,7 +565,7 @@ void collide_hulls(LocalManifold& m, const Hull& ha, const Hull& hb, Xf b_to_a, 
   const float spec = speculative_distance();
   const float ls = linear_slop();
   cache.hit = 0;
-  if (sat_calls) *sat_calls += 1;
+  if (sat_calls) __atomic_fetch_add(sat_calls, 1, __ATOMIC_RELAXED);
 
   switch (cache.type) {
     case kInvalidAxis:
@@ -576,11 +576,11 @@ void collide_hulls(LocalManifold& m, const Hull& ha, const Hull& hb, Xf b_to_a, 
       int vi = find_support_vertex(hb, dir);
       Vec3 support = xf_point(b_to_a, hb.points[vi]);
       float separation = plane_sep(pl, support);
-      if (separation >= spec) { cache.hit = 1; if (sat_hits) *sat_hits += 1; return; }
+      if (separation >= spec) { cache.hit = 1; if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED); return; }
       SepAxis fq{pl.n, 0.0f, cache.index_a, vi, kFaceAxisA};
       SATCache local{};
       bool touching = build_face_a_contact(m, ha, hb, b_to_a, fq, local);
-      if (touching && absf(cache.separation - local.separation) < ls) { cache.hit = 1; if (sat_hits) *sat_hits += 1; return; }
+      if (touching && absf(cache.separation - local.separation) < ls) { cache.hit = 1; if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED); return; }
       break;
     }
     case kFaceAxisB: {
@@ -589,11 +589,11 @@ void collide_hulls(LocalManifold& m, const Hull& ha, const Hull& hb, Xf b_to_a, 
       int vi = find_support_vertex(ha, dir);
       Vec3 support = inv_xf_point(b_to_a, ha.points[vi]);
       float separation = plane_sep(pl, support);
-      if (separation >= spec) { cache.hit = 1; if (sat_hits) *sat_hits += 1; return; }
+      if (separation >= spec) { cache.hit = 1; if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED); return; }
       SepAxis fq{neg(pl.n), 0.0f, vi, cache.index_b, kFaceAxisB};
       SATCache local{};
       bool touching = build_face_b_contact(m, ha, hb, b_to_a, fq, local);
-      if (touching && absf(cache.separation - local.separation) < ls) { cache.hit = 1; if (sat_hits) *sat_hits += 1; return; }
+      if (touching && absf(cache.separation - local.separation) < ls) { cache.hit = 1; if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED); return; }
       break;
     }
     case kEdgePairAxis: {
@@ -619,11 +619,11 @@ void collide_hulls(LocalManifold& m, const Hull& ha, const Hull& hb, Xf b_to_a, 
           float t = cba / (cba - dba);
           Vec3 axis = normalize(lerp(u_b, v_b, t));
           float separation = dot(axis, sub(q_a, q_b));
-          if (separation > spec) { cache.hit = 1; if (sat_hits) *sat_hits += 1; return; }
+          if (separation > spec) { cache.hit = 1; if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED); return; }
           SepAxis eq{neg(axis), 0.0f, cache.index_a, cache.index_b, kEdgePairAxis};
           SATCache local{};
           bool touching = build_edge_contact(m, ha, hb, b_to_a, eq, local);
-          if (touching && absf(cache.separation - local.separation) < ls) { cache.hit = 1; if (sat_hits) *sat_hits += 1; return; }
+          if (touching && absf(cache.separation - local.separation) < ls) { cache.hit = 1; if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED); return; }
         }
       }
       break;

@@ -264,21 +264,28 @@ void World::prepare_contacts(const StepContext& ctx) {
   if (any_restitution) solve_any_restitution = true;
   if ((int32_t)manifold_pool.size() < manifold_total) manifold_pool.resize(manifold_total);
   const float warm = enable_warm_starting ? 1.0f : 0.0f;
+  // manifold slots are assigned in colour order first; the constraints are then independent
+  struct Item { int32_t ci, k, start; };
+  std::vector<Item> items;
   int32_t start = 0;
   for (int ci = 0; ci < kGraphColorCount - 1; ++ci) {
-    std::vector<ContactConstraint>& list = color_constraints[ci];
-    list.resize(colors[ci].convex_contacts.size());
+    color_constraints[ci].resize(colors[ci].convex_contacts.size());
     for (size_t k = 0; k < colors[ci].convex_contacts.size(); ++k) {
-      int32_t id = colors[ci].convex_contacts[k];
-      prepare_contact_constraint(id, list[k], start, soft, static_soft, warm);
-      start += contacts[id].manifold_count;
+      items.push_back(Item{ci, (int32_t)k, start});
+      start += contacts[colors[ci].convex_contacts[k]].manifold_count;
     }
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (size_t t = 0; t < items.size(); ++t) {
+    const Item& it = items[t];
+    prepare_contact_constraint(colors[it.ci].convex_contacts[it.k], color_constraints[it.ci][it.k], it.start, soft, static_soft, warm);
   }
 }
 
 // solver_integration.rs:263-358
 void World::integrate_velocities(float h) {
   Vec3 g = gravity;
+#pragma omp for schedule(static)
   for (size_t i = 0; i < awake_sims.size(); ++i) {
     const BodySim& sim = awake_sims[i];
     BodyState& state = awake_states[i];
@@ -323,6 +330,7 @@ void World::integrate_positions(float h, float inv_dt) {
   float max_lin = max_linear_speed;
   float max_ang = kMaxRotation * inv_dt;
   float max_lin_sq = max_lin * max_lin, max_ang_sq = max_ang * max_ang;
+#pragma omp for schedule(static)
   for (size_t i = 0; i < awake_states.size(); ++i) {
     BodyState& st = awake_states[i];
     Vec3 v = st.v, w = st.w;
@@ -351,7 +359,10 @@ void World::integrate_positions(float h, float inv_dt) {
 
 // contact_solver.rs:697-763
 void World::warm_start_color(int ci) {
-  for (const ContactConstraint& c : color_constraints[ci]) {
+  const std::vector<ContactConstraint>& list_ = color_constraints[ci];
+#pragma omp for schedule(static)
+  for (size_t k_ = 0; k_ < list_.size(); ++k_) {
+    const ContactConstraint& c = list_[k_];
     Vec3 linear = kZero3, ang_a = kZero3, ang_b = kZero3;
     for (int mo = 0; mo < c.manifold_count; ++mo) {
       ManifoldConstraint& cm = manifold_pool[c.manifold_start + mo];
@@ -397,7 +408,10 @@ void World::push_color(int ci, const StepContext& ctx) {
   const float dyn_mass = ctx.contact_softness.mass_scale, dyn_imp = ctx.contact_softness.impulse_scale;
   const float st_bias = ctx.static_softness.mass_scale * ctx.static_softness.bias_rate;
   const float st_mass = ctx.static_softness.mass_scale, st_imp = ctx.static_softness.impulse_scale;
-  for (const ContactConstraint& c : color_constraints[ci]) {
+  const std::vector<ContactConstraint>& list_ = color_constraints[ci];
+#pragma omp for schedule(static)
+  for (size_t k_ = 0; k_ < list_.size(); ++k_) {
+    const ContactConstraint& c = list_[k_];
     BodyState ba = state_at(awake_states, c.index_a);
     BodyState bb = state_at(awake_states, c.index_b);
     float bias_rate = c.is_static ? st_bias : dyn_bias;
@@ -446,7 +460,10 @@ void World::push_color(int ci, const StepContext& ctx) {
 void World::relax_color(int ci, const StepContext& ctx) {
   const float inv_h = ctx.inv_h;
   const float eps = FLT_EPSILON;
-  for (const ContactConstraint& c : color_constraints[ci]) {
+  const std::vector<ContactConstraint>& list_ = color_constraints[ci];
+#pragma omp for schedule(static)
+  for (size_t k_ = 0; k_ < list_.size(); ++k_) {
+    const ContactConstraint& c = list_[k_];
     BodyState ba = state_at(awake_states, c.index_a);
     BodyState bb = state_at(awake_states, c.index_b);
     bool any_rolling = c.rolling_resistance > 0.0f;
@@ -543,7 +560,10 @@ void World::restitution_color(int ci, const StepContext& ctx) {
   const float inv_h = ctx.inv_h;
   const float neg_threshold = -restitution_threshold;
   const bool propagate = enable_restitution_propagation;
-  for (const ContactConstraint& c : color_constraints[ci]) {
+  const std::vector<ContactConstraint>& list_ = color_constraints[ci];
+#pragma omp for schedule(static)
+  for (size_t k_ = 0; k_ < list_.size(); ++k_) {
+    const ContactConstraint& c = list_[k_];
     if (!propagate && c.restitution == 0.0f) continue;
     BodyState ba = state_at(awake_states, c.index_a);
     BodyState bb = state_at(awake_states, c.index_b);
@@ -590,7 +610,10 @@ void World::restitution_color(int ci, const StepContext& ctx) {
 
 // contact_solver.rs:1258-1299
 void World::store_color(int ci) {
-  for (const ContactConstraint& c : color_constraints[ci]) {
+  const std::vector<ContactConstraint>& list_ = color_constraints[ci];
+#pragma omp for schedule(static)
+  for (size_t k_ = 0; k_ < list_.size(); ++k_) {
+    const ContactConstraint& c = list_[k_];
     Contact& contact = contacts[c.contact_id];
     if (contact.manifold_count <= 0) continue;
     bool hit_events = (contact.flags & CF::SIM_ENABLE_HIT_EVENT) != 0;
@@ -862,25 +885,36 @@ void World::store_overflow() {
 }
 
 // solver_stages.rs:543-589
+// One parallel region: every thread walks the schedule; each stage's loop over one
+// colour (or over bodies) is shared out with an implicit barrier at its end, and the
+// overflow list runs on one thread.
 void World::solve_stages(const StepContext& ctx) {
-  for (int sub_i = 0; sub_i < ctx.sub_step_count; ++sub_i) {
-    integrate_velocities(ctx.h);
-    warm_start_overflow();
-    for (int ci = 0; ci < kGraphColorCount - 1; ++ci) warm_start_color(ci);
-    push_overflow(ctx);
-    for (int ci = 0; ci < kGraphColorCount - 1; ++ci) push_color(ci, ctx);
-    integrate_positions(ctx.h, ctx.inv_dt);
-    relax_overflow(ctx);
-    for (int ci = 0; ci < kGraphColorCount - 1; ++ci) relax_color(ci, ctx);
-  }
-  if (solve_any_restitution && restitution_iterations > 0) {
-    for (int it = 0; it < restitution_iterations; ++it) {
-      restitution_overflow(ctx);
-      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) restitution_color(ci, ctx);
+#pragma omp parallel
+  {
+    for (int sub_i = 0; sub_i < ctx.sub_step_count; ++sub_i) {
+      integrate_velocities(ctx.h);
+#pragma omp single
+      warm_start_overflow();
+      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) warm_start_color(ci);
+#pragma omp single
+      push_overflow(ctx);
+      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) push_color(ci, ctx);
+      integrate_positions(ctx.h, ctx.inv_dt);
+#pragma omp single
+      relax_overflow(ctx);
+      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) relax_color(ci, ctx);
     }
+    if (solve_any_restitution && restitution_iterations > 0) {
+      for (int it = 0; it < restitution_iterations; ++it) {
+#pragma omp single
+        restitution_overflow(ctx);
+        for (int ci = 0; ci < kGraphColorCount - 1; ++ci) restitution_color(ci, ctx);
+      }
+    }
+#pragma omp single
+    store_overflow();
+    for (int ci = 0; ci < kGraphColorCount - 1; ++ci) store_color(ci);
   }
-  store_overflow();
-  for (int ci = 0; ci < kGraphColorCount - 1; ++ci) store_color(ci);
 }
 
 }  // namespace nslref

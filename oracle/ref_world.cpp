@@ -4,6 +4,10 @@
 #include <algorithm>
 #include <cstdio>
 
+#include <omp.h>
+
+#include <algorithm>
+
 #include "ref_world.h"
 
 namespace nslref {
@@ -423,29 +427,38 @@ void World::update_broad_phase_pairs() {
   }
   if (!any_moved) return;
   std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.lo < y.lo; });
-  std::vector<uint64_t> keys;
-  std::vector<int32_t> active;
-  for (const Item& it : items) {
-    const AABB& box = fat_aabbs[it.shape];
-    size_t w = 0;
-    for (size_t k = 0; k < active.size(); ++k) {
-      int32_t o = active[k];
-      if (fat_aabbs[o].hi.x < box.lo.x) continue;  // drop: ends before this one starts
-      active[w++] = o;
-      const Shape& sa = shapes[it.shape];
-      const Shape& sb = shapes[o];
-      if (!(sa.proxy_moved || sb.proxy_moved)) continue;
-      if (sa.proxy_type != kDynamic && sb.proxy_type != kDynamic) continue;
-      if (!aabb_overlaps(box, fat_aabbs[o])) continue;
-      int32_t lo = mini(it.shape, o), hi = maxi(it.shape, o);
-      if (sa.body_id == sb.body_id) continue;
-      uint64_t key = shape_pair_key(lo, hi, 0);
-      if (pair_set.count(key)) continue;
-      if (should_create_pair(lo, hi)) keys.push_back(key);
+  // Sweep along x: the later proxies that overlap proxy p along x are the positions after
+  // p whose lower x does not exceed p's upper x. Positions are independent, so they are
+  // shared out between threads; the keys are sorted afterwards.
+  const int n = (int)items.size();
+  std::vector<float> los((size_t)n);
+  for (int i = 0; i < n; ++i) los[i] = items[i].lo;
+  std::vector<std::vector<uint64_t>> found((size_t)omp_get_max_threads());
+#pragma omp parallel
+  {
+    std::vector<uint64_t>& mine = found[(size_t)omp_get_thread_num()];
+#pragma omp for schedule(dynamic, 32)
+    for (int p = 0; p < n; ++p) {
+      const int32_t sp = items[p].shape;
+      const AABB& box = fat_aabbs[sp];
+      const Shape& sa = shapes[sp];
+      const int end = (int)(std::upper_bound(los.begin() + p + 1, los.end(), box.hi.x) - los.begin());
+      for (int q = p + 1; q < end; ++q) {
+        const int32_t o = items[q].shape;
+        const Shape& sb = shapes[o];
+        if (!(sa.proxy_moved || sb.proxy_moved)) continue;
+        if (sa.proxy_type != kDynamic && sb.proxy_type != kDynamic) continue;
+        if (!aabb_overlaps(box, fat_aabbs[o])) continue;
+        int32_t lo = mini(sp, o), hi = maxi(sp, o);
+        if (sa.body_id == sb.body_id) continue;
+        uint64_t key = shape_pair_key(lo, hi, 0);
+        if (pair_set.count(key)) continue;
+        if (should_create_pair(lo, hi)) mine.push_back(key);
+      }
     }
-    active.resize(w);
-    active.push_back(it.shape);
   }
+  std::vector<uint64_t> keys;
+  for (const std::vector<uint64_t>& f : found) keys.insert(keys.end(), f.begin(), f.end());
   for (Shape& s : shapes) s.proxy_moved = false;
   std::sort(keys.begin(), keys.end());
   for (size_t i = 0; i < keys.size(); ++i) {
@@ -759,7 +772,7 @@ void World::collide_one_contact(int32_t id, std::vector<uint8_t>& state_bits) {
             m.points[i].persisted = true;
           }
         }
-        stats.recycled += 1;
+        __atomic_fetch_add(&stats.recycled, 1, __ATOMIC_RELAXED);
         return;
       }
     }
@@ -792,7 +805,9 @@ void World::collide() {
   ids.insert(ids.end(), awake_contact_indices.begin(), awake_contact_indices.end());
   if (ids.empty()) return;
   std::vector<uint8_t> state_bits(std::max<size_t>(contacts.size(), 8), 0);
-  for (int32_t id : ids) collide_one_contact(id, state_bits);
+  // each item reads shared state and writes only its own contact and state byte
+#pragma omp parallel for schedule(dynamic, 64)
+  for (size_t k = 0; k < ids.size(); ++k) collide_one_contact(ids[k], state_bits);
   // collide_pass.rs:386-478: state transitions in contact-id order
   for (size_t id = 0; id < state_bits.size(); ++id) {
     if (!state_bits[id]) continue;
@@ -826,6 +841,8 @@ void World::finalize_bodies(const StepContext& ctx) {
   const float spec = speculative_distance();
   const float time_step = ctx.dt, inv_time_step = ctx.inv_dt;
   std::vector<uint8_t> moved_flags(awake_sims.size(), 0);
+  // each item writes its own body, its own shapes and fat AABBs
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < awake_sims.size(); ++i) {
     BodyState& state = awake_states[i];
     BodySim& sim = awake_sims[i];
@@ -858,7 +875,7 @@ void World::finalize_bodies(const StepContext& ctx) {
         // Continuous collision is not restated; callers that pin results assert this
         // counter stays zero or run with continuous collision off.
         sim.flags |= BF::IS_FAST;
-        stats.fast_bodies += 1;
+        __atomic_fetch_add(&stats.fast_bodies, 1, __ATOMIC_RELAXED);
       } else {
         sim.center0 = sim.center;
         sim.rotation0 = sim.transform.q;
@@ -888,7 +905,10 @@ void World::finalize_bodies(const StepContext& ctx) {
 // ---------------------------------------------------------------------------------
 // Step driver (solver_integration.rs:807-1015)
 void World::step(float time_step, int sub_step_count) {
+  double t0 = omp_get_wtime();
   update_broad_phase_pairs();
+  double t1 = omp_get_wtime();
+  stats.ms_pairs += 1e3 * (t1 - t0);
   StepContext ctx;
   ctx.dt = time_step;
   ctx.sub_step_count = maxi(1, sub_step_count);
@@ -903,11 +923,19 @@ void World::step(float time_step, int sub_step_count) {
   ctx.max_linear_velocity = max_linear_speed;
   ctx.enable_warm_starting = enable_warm_starting;
   if (time_step > 0.0f) collide();
+  double t2 = omp_get_wtime();
+  stats.ms_collide += 1e3 * (t2 - t1);
   if (time_step > 0.0f) {
     if (awake_sims.empty()) return;
     prepare_contacts(ctx);
+    double t3 = omp_get_wtime();
     solve_stages(ctx);
+    double t4 = omp_get_wtime();
     finalize_bodies(ctx);
+    double t5 = omp_get_wtime();
+    stats.ms_prepare += 1e3 * (t3 - t2);
+    stats.ms_solve += 1e3 * (t4 - t3);
+    stats.ms_finalize += 1e3 * (t5 - t4);
   }
 }
 

@@ -289,6 +289,9 @@ struct ShapeDef {
 struct StepStats {
   int64_t pairs_created = 0, contacts_destroyed = 0, begin_touch = 0, end_touch = 0;
   int64_t recycled = 0, sat_calls = 0, sat_hits = 0, fast_bodies = 0, overflow_constraints = 0;
+  // cumulative wall time in ms (types/world_def.rs:158-184 phases): pairs, collide
+  // (narrow phase and state pass), prepare, solve, finalize
+  double ms_pairs = 0, ms_collide = 0, ms_prepare = 0, ms_solve = 0, ms_finalize = 0;
 };
 
 class World {

@@ -114,16 +114,20 @@ def test_graph_scene_colours_on_gpu(gpu_available):
 
 def test_step_kernel_workgroup_count_does_not_change_results(gpu_available):
     """The step kernel's grid-stride loops and grid barriers give the same bits for any
-    number of workgroups."""
+    number of workgroups, equal to the oracle's."""
     from natsel_amd.world import load_scene
+    from oracle import pyoracle
 
     sc = scenes.pile(300, seed=7, size_x=6, size_y=10, size_z=6)
-    worlds = [load_scene(sc, solver_blocks=b) for b in (1, 2, 7, 32)]
+    counts = (1, 2, 7, 32, 48)
+    worlds = [load_scene(sc, solver_blocks=b) for b in counts]
+    o = pyoracle.load_scene(sc)
     for k in range(90):
+        o.step(1.0 / 60.0, 4)
         hashes = set()
         for w in worlds:
             w.step(1.0 / 60.0, 4)
             hashes.add(w.world_state_hash())
-        assert len(hashes) == 1, f"step {k}: {sorted(hashes)}"
-    assert worlds[0].kernel_stats()["solver_blocks"] == 1
-    assert worlds[3].kernel_stats()["solver_blocks"] == 32
+        assert hashes == {o.world_state_hash()}, f"step {k}: {sorted(hashes)}"
+    for b, w in zip(counts, worlds):
+        assert w.kernel_stats()["solver_blocks"] == b
