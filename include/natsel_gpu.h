@@ -19,6 +19,8 @@
  *   nsl_world_get_profile          World::profile               src/types/world_def.rs:158-184
  *   nsl_world_*kernel_*            per-kernel device timing (no counterpart: the reference times phases
  *                                  on the host, solver_stages.rs:551-586)
+ *   nsl_batch_*                    many Worlds at once: create_body / create_*_shape / step /
+ *                                  world_state_hash per world, as above (no batch type in the reference)
  *
  * Conventions: every call returns 0 on success or a negative status; on failure
  * nsl_last_error() returns a message. Pointers are plain host memory owned by the
@@ -133,6 +135,55 @@ int nsl_world_get_profile(nsl_world* world, nsl_profile* out);
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on);
 int nsl_world_reset_kernel_stats(nsl_world* world);
 int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out);
+
+/* ---------------------------------------------------------------------------------
+ * Batches of independent worlds. Each world is built and stepped exactly as a World of
+ * its own (the calls mirror the World methods above, applied to world `world`), and
+ * its state hash equals the hash the same world reaches alone. All worlds of a batch
+ * share the options and step together; worlds, bodies and shapes can be added until the
+ * first nsl_batch_step. A world holds at most 256 dynamic bodies; a world whose contact
+ * capacity runs out stops stepping and is reported by nsl_batch_get_stats. Sleep and
+ * continuous collision are not run. */
+typedef struct nsl_batch nsl_batch;
+
+typedef struct nsl_batch_stats {
+  double last_step_ms;      /* device time of the last nsl_batch_step call */
+  int64_t body_steps;       /* dynamic bodies x steps since creation */
+  int64_t bodies;           /* dynamic bodies in the batch */
+  int64_t contact_capacity; /* contact slots over all worlds */
+  int32_t error_worlds;     /* worlds stopped by a full capacity */
+  int32_t first_error;      /* 1: contacts full, 2: overflow colour full, 3: new pairs of a step over 2048 */
+} nsl_batch_stats;
+
+int nsl_batch_create(int device, const nsl_options* options, nsl_batch** out);
+int nsl_batch_destroy(nsl_batch* batch);
+int nsl_batch_add_world(nsl_batch* batch, int32_t* out_world);
+int nsl_batch_create_body(nsl_batch* batch, int32_t world, const nsl_body_def* def, int32_t* out_body);
+int nsl_batch_create_box(nsl_batch* batch, int32_t world, int32_t body, const float half_extents[3], const nsl_material* m,
+                         int32_t* out_shape);
+int nsl_batch_create_sphere(nsl_batch* batch, int32_t world, int32_t body, const float center[3], float radius,
+                            const nsl_material* m, int32_t* out_shape);
+int nsl_batch_create_capsule(nsl_batch* batch, int32_t world, int32_t body, const float c1[3], const float c2[3], float radius,
+                             const nsl_material* m, int32_t* out_shape);
+/* `steps` steps of every world, with no host work between them. */
+int nsl_batch_step(nsl_batch* batch, float dt, int32_t sub_steps, int32_t steps);
+int nsl_batch_world_count(nsl_batch* batch, int32_t* out);
+int nsl_batch_body_count(nsl_batch* batch, int32_t world, int32_t* out);
+/* out[world_count]: each world's World::world_state_hash. */
+int nsl_batch_state_hashes(nsl_batch* batch, uint64_t* out);
+/* Arrays sized by nsl_batch_body_count(world), as nsl_world_get_bodies. */
+int nsl_batch_get_bodies(nsl_batch* batch, int32_t world, double* pos, float* rot, float* v, float* w);
+/* Contact slots of a world (0 before the first step) and, per slot, its constraint-graph
+ * colour (-1 when not in the graph), local index and liveness, as nsl_world_get_contact_colors. */
+int nsl_batch_contact_capacity(nsl_batch* batch, int32_t world, int32_t* out);
+int nsl_batch_get_contact_colors(nsl_batch* batch, int32_t world, int32_t* color, int32_t* local, int32_t* alive);
+/* Per contact slot of a world: its shape ids (primary first; -1 for a free slot). */
+int nsl_batch_get_contact_shapes(nsl_batch* batch, int32_t world, int32_t* shape_a, int32_t* shape_b);
+/* Per shape of a world (shape count = shapes created in it): fat AABB as lower x, y, z,
+ * upper x, y, z (shape_runtime.rs:398-408), and whether its proxy moved since the last
+ * pair update. */
+int nsl_batch_get_shape_bounds(nsl_batch* batch, int32_t world, float* fat, uint8_t* moved);
+int nsl_batch_get_stats(nsl_batch* batch, nsl_batch_stats* out);
 
 #ifdef __cplusplus
 }

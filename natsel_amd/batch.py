@@ -1,0 +1,166 @@
+"""Batches of independent worlds over the HIP library (include/natsel_gpu.h, nsl_batch_*).
+
+Each world of a batch is built with the reference's World calls (create_body,
+create_hull_shape(make_box_hull), create_sphere_shape, create_capsule_shape) and steps
+exactly as that World would alone (World::step, src/simulation/solver_integration.rs:807),
+so its world_state_hash (src/simulation/body_queries.rs:432) matches. All worlds step
+together on the device with no host work between steps.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .scenes import BOX, SPHERE, Scene
+from .world import BodyDef, BoxHull, Capsule, ShapeDef, Sphere, WorldOptions, _material, c_options, make_box_hull
+
+
+class BatchWorld:
+    """World `index` of a batch, with the reference World's creation calls."""
+
+    def __init__(self, batch: "Batch", index: int):
+        self._b = batch
+        self.index = index
+
+    def create_body(self, d: BodyDef) -> int:
+        c = capi.BodyDef()
+        c.type = d.type
+        c.position[:] = d.position
+        c.rotation[:] = d.rotation
+        c.linear_velocity[:] = d.linear_velocity
+        c.angular_velocity[:] = d.angular_velocity
+        c.linear_damping, c.angular_damping, c.gravity_scale = d.linear_damping, d.angular_damping, d.gravity_scale
+        c.enable_sleep = int(d.enable_sleep)
+        out = ctypes.c_int32()
+        capi.check(self._b._lib.nsl_batch_create_body(self._b._h, self.index, ctypes.byref(c), ctypes.byref(out)))
+        return out.value
+
+    def create_hull_shape(self, body: int, d: ShapeDef, hull: BoxHull) -> int:
+        out = ctypes.c_int32()
+        h = (ctypes.c_float * 3)(hull.hx, hull.hy, hull.hz)
+        capi.check(self._b._lib.nsl_batch_create_box(self._b._h, self.index, body, h, ctypes.byref(_material(d)), ctypes.byref(out)))
+        return out.value
+
+    def create_sphere_shape(self, body: int, d: ShapeDef, s: Sphere) -> int:
+        out = ctypes.c_int32()
+        c = (ctypes.c_float * 3)(*s.center)
+        capi.check(self._b._lib.nsl_batch_create_sphere(self._b._h, self.index, body, c, s.radius, ctypes.byref(_material(d)),
+                                                        ctypes.byref(out)))
+        return out.value
+
+    def create_capsule_shape(self, body: int, d: ShapeDef, c: Capsule) -> int:
+        out = ctypes.c_int32()
+        c1 = (ctypes.c_float * 3)(*c.center1)
+        c2 = (ctypes.c_float * 3)(*c.center2)
+        capi.check(self._b._lib.nsl_batch_create_capsule(self._b._h, self.index, body, c1, c2, c.radius,
+                                                         ctypes.byref(_material(d)), ctypes.byref(out)))
+        return out.value
+
+    def body_arrays(self):
+        return self._b.body_arrays(self.index)
+
+
+class Batch:
+    def __init__(self, device: int = 0, options: WorldOptions | None = None):
+        self._lib = capi.load_library()
+        self._opts = options or WorldOptions()
+        handle = ctypes.c_void_p()
+        capi.check(self._lib.nsl_batch_create(device, ctypes.byref(c_options(self._opts)), ctypes.byref(handle)))
+        self._h = handle
+        self.worlds: list[BatchWorld] = []
+
+    def close(self) -> None:
+        if self._h:
+            capi.check(self._lib.nsl_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_world(self) -> BatchWorld:
+        out = ctypes.c_int32()
+        capi.check(self._lib.nsl_batch_add_world(self._h, ctypes.byref(out)))
+        w = BatchWorld(self, out.value)
+        self.worlds.append(w)
+        return w
+
+    def step(self, time_step: float, sub_step_count: int, steps: int = 1) -> None:
+        capi.check(self._lib.nsl_batch_step(self._h, time_step, sub_step_count, steps))
+
+    def state_hashes(self) -> np.ndarray:
+        out = np.zeros(len(self.worlds), np.uint64)
+        capi.check(self._lib.nsl_batch_state_hashes(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return out
+
+    def body_arrays(self, world: int):
+        n = ctypes.c_int32()
+        capi.check(self._lib.nsl_batch_body_count(self._h, world, ctypes.byref(n)))
+        n = n.value
+        pos = np.zeros((n, 3), np.float64)
+        rot = np.zeros((n, 4), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        w = np.zeros((n, 3), np.float32)
+        f = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))  # noqa: E731
+        capi.check(self._lib.nsl_batch_get_bodies(self._h, world, f(pos, ctypes.c_double), f(rot, ctypes.c_float),
+                                                  f(v, ctypes.c_float), f(w, ctypes.c_float)))
+        return pos, rot, v, w
+
+    def contact_colors(self, world: int):
+        """Per contact slot of world `world`: colour (-1 when not in the graph), local index, liveness."""
+        n = ctypes.c_int32()
+        capi.check(self._lib.nsl_batch_contact_capacity(self._h, world, ctypes.byref(n)))
+        color, local, alive = (np.zeros(n.value, np.int32) for _ in range(3))
+        f = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        capi.check(self._lib.nsl_batch_get_contact_colors(self._h, world, f(color), f(local), f(alive)))
+        return color, local, alive
+
+    def contact_shapes(self, world: int):
+        """Per contact slot of world `world`: its shape ids (primary first; -1 when free)."""
+        n = ctypes.c_int32()
+        capi.check(self._lib.nsl_batch_contact_capacity(self._h, world, ctypes.byref(n)))
+        a, b = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        f = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        capi.check(self._lib.nsl_batch_get_contact_shapes(self._h, world, f(a), f(b)))
+        return a, b
+
+    def shape_bounds(self, world: int, n_shapes: int):
+        """(fat AABBs [n, 6] as lower xyz, upper xyz; moved flags [n]) of world `world`."""
+        fat = np.zeros((n_shapes, 6), np.float32)
+        moved = np.zeros(n_shapes, np.uint8)
+        capi.check(self._lib.nsl_batch_get_shape_bounds(self._h, world, fat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                        moved.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return fat, moved
+
+    def stats(self) -> dict:
+        s = capi.BatchStats()
+        capi.check(self._lib.nsl_batch_get_stats(self._h, ctypes.byref(s)))
+        return {n: getattr(s, n) for n, _ in capi.BatchStats._fields_}
+
+
+def load_batch(scene_list: list[Scene], device: int = 0) -> Batch:
+    """One batch world per scene. The worlds of a batch share their options, so the scenes
+    must agree on sleep, continuous collision, time step and substeps."""
+    first = scene_list[0]
+    for s in scene_list:
+        if (s.enable_sleep, s.enable_continuous, s.dt, s.substeps) != (first.enable_sleep, first.enable_continuous, first.dt,
+                                                                     first.substeps):
+            raise ValueError(f"scene {s.name} differs from {first.name} in sleep, continuous collision, dt or substeps")
+    b = Batch(device, WorldOptions(enable_sleep=first.enable_sleep, enable_continuous=first.enable_continuous))
+    for scene in scene_list:
+        w = b.add_world()
+        ids = [w.create_body(BodyDef(s.body_type, s.position, s.rotation, s.linear_velocity, s.angular_velocity, s.linear_damping,
+                                     s.angular_damping, s.gravity_scale, s.enable_sleep)) for s in scene.bodies]
+        for s in scene.shapes:
+            d = ShapeDef(s.density, s.friction, s.restitution)
+            if s.kind == BOX:
+                w.create_hull_shape(ids[s.body], d, make_box_hull(*s.params))
+            elif s.kind == SPHERE:
+                w.create_sphere_shape(ids[s.body], d, Sphere(tuple(s.params[:3]), s.params[3]))
+            else:
+                w.create_capsule_shape(ids[s.body], d, Capsule(tuple(s.params[:3]), tuple(s.params[3:6]), s.params[6]))
+    return b

@@ -78,6 +78,17 @@ class KernelStats(ctypes.Structure):
     ] + [("host_ms", ctypes.c_double * len(HOST_PHASES))]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("last_step_ms", ctypes.c_double),
+        ("body_steps", ctypes.c_int64),
+        ("bodies", ctypes.c_int64),
+        ("contact_capacity", ctypes.c_int64),
+        ("error_worlds", ctypes.c_int32),
+        ("first_error", ctypes.c_int32),
+    ]
+
+
 # Every entry point declared in include/natsel_gpu.h, with its argument types.
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -109,6 +120,32 @@ SIGNATURES = {
     "nsl_world_set_kernel_timing": (ctypes.c_int, [_P, _I]),
     "nsl_world_reset_kernel_stats": (ctypes.c_int, [_P]),
     "nsl_world_get_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStats)]),
+    "nsl_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Options), ctypes.POINTER(_P)]),
+    "nsl_batch_destroy": (ctypes.c_int, [_P]),
+    "nsl_batch_add_world": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "nsl_batch_create_body": (ctypes.c_int, [_P, _I, ctypes.POINTER(BodyDef), ctypes.POINTER(_I)]),
+    "nsl_batch_create_box": (ctypes.c_int, [_P, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(Material), ctypes.POINTER(_I)]),
+    "nsl_batch_create_sphere": (
+        ctypes.c_int,
+        [_P, _I, _I, ctypes.POINTER(_F), _F, ctypes.POINTER(Material), ctypes.POINTER(_I)],
+    ),
+    "nsl_batch_create_capsule": (
+        ctypes.c_int,
+        [_P, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), _F, ctypes.POINTER(Material), ctypes.POINTER(_I)],
+    ),
+    "nsl_batch_step": (ctypes.c_int, [_P, _F, _I, _I]),
+    "nsl_batch_world_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "nsl_batch_body_count": (ctypes.c_int, [_P, _I, ctypes.POINTER(_I)]),
+    "nsl_batch_state_hashes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "nsl_batch_get_bodies": (
+        ctypes.c_int,
+        [_P, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)],
+    ),
+    "nsl_batch_contact_capacity": (ctypes.c_int, [_P, _I, ctypes.POINTER(_I)]),
+    "nsl_batch_get_contact_colors": (ctypes.c_int, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "nsl_batch_get_contact_shapes": (ctypes.c_int, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "nsl_batch_get_shape_bounds": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(ctypes.c_uint8)]),
+    "nsl_batch_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(BatchStats)]),
 }
 
 

@@ -1,0 +1,161 @@
+// Batched stepper: thousands of independent worlds of up to 256 dynamic bodies each,
+// stepped together on the device with no host work between steps.
+//
+// The reference steps one World at a time (World::step, solver_integration.rs:807); a
+// batch holds many such worlds and steps each exactly as the reference would step it
+// alone: the same pair order, contact ids, colours and float operations, so every
+// world's state hash equals its standalone hash. A step is three launches over all
+// worlds, one workgroup per world, with no host work in between: the pair update and
+// contact creation (one wavefront), the collide pass, and the contact-state transitions
+// and graph colouring (one lane, in contact-id order) followed by preparation, the
+// substeps with the world's body states in LDS, impulse storage and finalisation (one
+// wavefront). Worlds never synchronise with each other.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "nsl_broadphase.h"
+#include "nsl_device.h"
+#include "nsl_scene.h"
+
+namespace nsl {
+
+constexpr int kBatchMaxAwake = 256;    // dynamic bodies per world (their states sit in LDS)
+constexpr int kBatchMaxShapes = 4096;  // shapes per world (pair keys pack two 16-bit ids)
+constexpr int kBatchMaxNewPairs = 2048;  // new pairs per world and step (sorted in LDS)
+constexpr int kBatchLanes = 64;        // one wavefront per world (pair update, solve)
+constexpr int kBatchCollideBlock = 128;  // collide pass: one workgroup per world
+
+// Where one world's records live in the concatenated device arrays.
+struct BatchWorldDesc {
+  int32_t awake_off, n_awake;
+  int32_t static_off, n_static;
+  int32_t body_off, n_bodies;
+  int32_t shape_off, n_shapes;
+  int32_t hull_off;
+  int32_t contact_off, contact_cap;    // contacts, constraints, solve ids, free ids, non-touching list
+  int32_t overflow_off, overflow_cap;  // overflow constraints and list
+  int32_t color_off, color_cap;        // lists of colours 0..22: [23][color_cap] contact ids
+  int32_t bits_off, bits_words;        // body bits of colours 0..22: [23][bits_words] u32
+  int32_t pair_off;                    // pair-exists bits: n_shapes * n_shapes bits (u32 words)
+};
+
+// The integer bookkeeping of one world (what the single-world stepper keeps on the host).
+struct BatchWorldState {
+  int32_t next_id;                      // contact id pool (id_pool.rs:28-46)
+  int32_t n_free;
+  int32_t n_non_touching;               // awake non-touching contacts (collide_pass.rs:743-774)
+  int32_t color_count[kOverflowColor];  // contacts in colours 0..22
+  int32_t n_overflow;
+  int32_t error;                        // non-zero: a capacity was exceeded; the world stops
+  int32_t steps;                        // steps taken
+};
+
+enum BatchError : int32_t { kBatchOk = 0, kBatchContactsFull = 1, kBatchOverflowFull = 2, kBatchPairsFull = 3 };
+
+struct BodyMeta {  // per body id: type, solver-set encoding, flags
+  int32_t type, enc;
+  uint32_t flags;
+};
+
+struct ContactMeta {  // per contact id: shapes, graph colour and index, liveness
+  int32_t shape_a, shape_b, color, local;
+  int32_t alive;
+  float restitution;
+};
+
+// Device arrays of a batch, passed by value to the kernel.
+struct BatchDevice {
+  BodySim* sims;
+  BodyState* states;
+  BodySim* static_sims;
+  BodyMeta* body_meta;
+  Shape* shapes;
+  Aabb* fat_aabbs;
+  uint8_t* shape_moved;
+  Proxy* proxies;  // pad holds the shape's enable_speculative flag
+  BoxHull* hulls;
+  Contact* contacts;
+  uint8_t* contact_change;
+  ContactMeta* contact_meta;
+  Constraint* constraints;
+  int32_t* solve_ids;
+  OverflowConstraint* overflow;
+  int32_t* overflow_ids;
+  int32_t* free_ids;
+  int32_t* non_touching;
+  int32_t* overflow_list;
+  int32_t* color_lists;
+  uint32_t* color_bits;
+  uint32_t* pair_bits;
+  BatchWorldDesc* worlds;
+  BatchWorldState* wstate;
+  int32_t* order;     // worlds in launch order (most bodies first)
+  int32_t* counters;  // as DeviceWorld::counters
+  int32_t n_worlds;
+  int32_t enable_speculative;
+  int32_t restitution_iterations;
+};
+
+void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hipStream_t s);
+
+class BatchStepper {
+ public:
+  explicit BatchStepper(int device);
+  ~BatchStepper();
+  BatchStepper(const BatchStepper&) = delete;
+  BatchStepper& operator=(const BatchStepper&) = delete;
+
+  WorldOptions options;  // shared by every world
+
+  // Worlds and their bodies and shapes can be added until the first step.
+  int32_t add_world();
+  HostScene& scene(int32_t w);
+  int32_t world_count() const { return (int32_t)scenes_.size(); }
+  int32_t body_count(int32_t w) const { return (int32_t)scenes_.at(w)->bodies_.size(); }
+
+  void step(float dt, int32_t sub_steps, int32_t steps);
+
+  // World w's reference state hash (body_queries.rs:432), for every world.
+  void state_hashes(uint64_t* out);
+  void get_bodies(int32_t w, double* pos, float* rot, float* v, float* wv);
+  // Worlds stopped by a full capacity, and the first such world's error code.
+  int32_t error_worlds(int32_t* first_error);
+  // World w's contact slots (its contact capacity) and, per slot, graph colour (-1: not in
+  // the graph), index in its list and liveness.
+  int32_t contact_slots(int32_t w) const;
+  void get_contact_colors(int32_t w, int32_t* color, int32_t* local, int32_t* alive);
+  // Per shape of world w: fat AABB (lower x, y, z, upper x, y, z) and the moved flag.
+  void get_shape_bounds(int32_t w, float* fat, uint8_t* moved);
+  // Per contact slot: its two shape ids (-1 when the slot is free).
+  void get_contact_shapes(int32_t w, int32_t* shape_a, int32_t* shape_b);
+  // Device time of the last step call (HIP events), and totals since creation.
+  float last_step_ms() const { return last_ms_; }
+  int64_t total_body_steps() const { return body_steps_; }
+  int64_t total_bodies() const { return total_awake_; }
+  int64_t contact_capacity() const { return total_contact_cap_; }
+
+ private:
+  void upload();
+  void pull();
+  template <class T> T* alloc(size_t n, bool zero);
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  std::vector<std::unique_ptr<HostScene>> scenes_;
+  std::vector<BatchWorldDesc> desc_;
+  BatchDevice d_{};
+  std::vector<void*> allocations_;
+  bool uploaded_ = false;
+  bool device_newer_ = false;
+  float last_ms_ = 0.0f;
+  int64_t body_steps_ = 0;
+  int64_t total_awake_ = 0;
+  int64_t total_contact_cap_ = 0;
+};
+
+}  // namespace nsl

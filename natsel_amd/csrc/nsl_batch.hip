@@ -1,0 +1,801 @@
+// Batched stepper (see nsl_batch.h): per step, three launches over all worlds (pair
+// update, collide pass, transitions and solve), each world in its own workgroup.
+#include "nsl_batch.h"
+
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+#include "nsl_stages.h"
+
+namespace nsl {
+
+#define BATCH_CHECK(expr)                                                                                \
+  do {                                                                                                   \
+    hipError_t e_ = (expr);                                                                              \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr ": ") + hipGetErrorString(e_));     \
+  } while (0)
+
+namespace {
+
+// One world's pointers into the batch arrays.
+struct WorldRefs {
+  const BodyMeta* body;
+  const Proxy* proxy;
+  const Shape* shapes;
+  const Aabb* fat;
+  uint8_t* moved;
+  ContactMeta* meta;
+  Contact* contacts;
+  uint8_t* change;
+  int32_t* free_ids;
+  int32_t* non_touching;
+  int32_t* overflow_list;
+  int32_t* colors;
+  uint32_t* bits;
+  uint32_t* pairs;
+  BatchWorldState* ws;
+  int32_t n_shapes, contact_cap, overflow_cap, color_cap, bits_words;
+};
+
+__device__ WorldRefs world_refs(const BatchDevice& b, const BatchWorldDesc& w, int world) {
+  WorldRefs r;
+  r.body = b.body_meta + w.body_off;
+  r.proxy = b.proxies + w.shape_off;
+  r.shapes = b.shapes + w.shape_off;
+  r.fat = b.fat_aabbs + w.shape_off;
+  r.moved = b.shape_moved + w.shape_off;
+  r.meta = b.contact_meta + w.contact_off;
+  r.contacts = b.contacts + w.contact_off;
+  r.change = b.contact_change + w.contact_off;
+  r.free_ids = b.free_ids + w.contact_off;
+  r.non_touching = b.non_touching + w.contact_off;
+  r.overflow_list = b.overflow_list + w.overflow_off;
+  r.colors = b.color_lists + w.color_off;
+  r.bits = b.color_bits + w.bits_off;
+  r.pairs = b.pair_bits + w.pair_off;
+  r.ws = b.wstate + world;
+  r.n_shapes = w.n_shapes;
+  r.contact_cap = w.contact_cap;
+  r.overflow_cap = w.overflow_cap;
+  r.color_cap = w.color_cap;
+  r.bits_words = w.bits_words;
+  return r;
+}
+
+// The world as a DeviceWorld over the shared per-item functions; `states` is the world's
+// slice of the global states or its LDS copy.
+__device__ DeviceWorld world_view(const BatchDevice& b, const BatchWorldDesc& w, BodyState* states) {
+  DeviceWorld d;
+  d.sims = b.sims + w.awake_off;
+  d.states = states;
+  d.static_sims = b.static_sims + w.static_off;
+  d.shapes = b.shapes + w.shape_off;
+  d.fat_aabbs = b.fat_aabbs + w.shape_off;
+  d.shape_moved = b.shape_moved + w.shape_off;
+  d.hulls = b.hulls + w.hull_off;
+  d.contacts = b.contacts + w.contact_off;
+  d.contact_change = b.contact_change + w.contact_off;
+  d.constraints = b.constraints + w.contact_off;
+  d.overflow = b.overflow + w.overflow_off;
+  d.collide_ids = nullptr;
+  d.solve_ids = b.solve_ids + w.contact_off;
+  d.overflow_ids = b.overflow_ids + w.overflow_off;
+  d.counters = b.counters;
+  d.pairs = PairSet{nullptr, 0};
+  d.n_awake = w.n_awake;
+  return d;
+}
+
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lanes_below(uint64_t mask, int lane) {
+  return __popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }  // the workgroup is one wavefront
+
+__device__ __forceinline__ bool pair_bit(const WorldRefs& r, int a, int b) {
+  const uint32_t k = (uint32_t)a * (uint32_t)r.n_shapes + (uint32_t)b;
+  return (r.pairs[k >> 5] >> (k & 31)) & 1u;
+}
+__device__ __forceinline__ void set_pair_bit(const WorldRefs& r, int a, int b, bool on) {
+  const uint32_t k = (uint32_t)a * (uint32_t)r.n_shapes + (uint32_t)b;
+  if (on) atomicOr(&r.pairs[k >> 5], 1u << (k & 31));
+  else atomicAnd(&r.pairs[k >> 5], ~(1u << (k & 31)));
+}
+
+__device__ __forceinline__ bool colour_has(const WorldRefs& r, int c, int body) {
+  return (r.bits[c * r.bits_words + (body >> 5)] >> (body & 31)) & 1u;
+}
+__device__ __forceinline__ void colour_set(const WorldRefs& r, int c, int body, bool on) {
+  uint32_t& w = r.bits[c * r.bits_words + (body >> 5)];
+  if (on) w |= 1u << (body & 31);
+  else w &= ~(1u << (body & 31));
+}
+
+// contact_creation.rs:219-425 (one lane): the primary shape is hull > capsule > sphere.
+__device__ void create_contact(const WorldRefs& r, int32_t a, int32_t b, bool enable_speculative) {
+  auto rank = [](int32_t kind) { return kind == kShapeHull ? 2 : (kind == kShapeCapsule ? 1 : 0); };
+  if (rank(r.shapes[a].kind) < rank(r.shapes[b].kind)) {
+    const int32_t t = a;
+    a = b;
+    b = t;
+  }
+  BatchWorldState* ws = r.ws;
+  int32_t id;
+  if (ws->n_free > 0) {
+    id = r.free_ids[--ws->n_free];
+  } else {
+    if (ws->next_id >= r.contact_cap) {
+      ws->error = kBatchContactsFull;
+      return;
+    }
+    id = ws->next_id++;
+  }
+  const Shape& sa = r.shapes[a];
+  const Shape& sb = r.shapes[b];
+  const Proxy& pa = r.proxy[a];
+  const Proxy& pb = r.proxy[b];
+  const BodyMeta& ba = r.body[pa.body];
+  const BodyMeta& bb = r.body[pb.body];
+  ContactMeta m;
+  m.shape_a = a;
+  m.shape_b = b;
+  m.color = kNullIndex;
+  m.local = ws->n_non_touching;
+  m.alive = 1;
+  m.restitution = maxf(sa.restitution, sb.restitution);
+  r.meta[id] = m;
+  r.non_touching[ws->n_non_touching++] = id;
+  Contact c{};
+  c.shape_a = a;
+  c.shape_b = b;
+  c.enc_a = ba.enc;
+  c.enc_b = bb.enc;
+  if (ba.type == kBodyStatic || bb.type == kBodyStatic) c.flags |= cflag::kStatic;
+  if (enable_speculative && pa.pad && pb.pad) c.flags |= cflag::kSpeculative;
+  if ((ba.flags & bflag::kEnableRecycling) && (bb.flags & bflag::kEnableRecycling) && (c.flags & cflag::kSpeculative))
+    c.flags |= cflag::kRecycle;
+  c.rolling_resistance = maxf(sa.rolling_resistance, sb.rolling_resistance) * maxf(sa.radius, sb.radius);
+  c.cached_rot_a = quat_identity();
+  c.cached_rot_b = quat_identity();
+  r.contacts[id] = c;
+  set_pair_bit(r, a < b ? a : b, a < b ? b : a, true);
+}
+
+// Swap-remove of entry `local` of `list` (length *n); returns the moved contact id or -1.
+__device__ __forceinline__ int32_t swap_remove(int32_t* list, int32_t* n, int32_t local) {
+  const int32_t last = --(*n);
+  if (local != last) {
+    list[local] = list[last];
+    return list[local];
+  }
+  return kNullIndex;
+}
+
+// constraint_graph.rs:200-251
+__device__ void remove_from_graph(const WorldRefs& r, int32_t body_a, int32_t body_b, int32_t color, int32_t local) {
+  BatchWorldState* ws = r.ws;
+  if (color != kOverflowColor) {
+    colour_set(r, color, body_a, false);
+    colour_set(r, color, body_b, false);
+    const int32_t moved = swap_remove(r.colors + color * r.color_cap, &ws->color_count[color], local);
+    if (moved != kNullIndex) r.meta[moved].local = local;
+  } else {
+    const int32_t moved = swap_remove(r.overflow_list, &ws->n_overflow, local);
+    if (moved != kNullIndex) r.meta[moved].local = local;
+  }
+}
+
+// constraint_graph.rs:113-197
+__device__ void add_to_graph(const WorldRefs& r, int32_t id) {
+  ContactMeta& c = r.meta[id];
+  const int32_t a = r.proxy[c.shape_a].body, b = r.proxy[c.shape_b].body;
+  const int32_t ta = r.body[a].type, tb = r.body[b].type;
+  int32_t color = kOverflowColor;
+  if (ta == kBodyDynamic && tb == kBodyDynamic) {
+    for (int i = 0; i < kDynamicColorCount; ++i) {
+      if (colour_has(r, i, a) || colour_has(r, i, b)) continue;
+      colour_set(r, i, a, true);
+      colour_set(r, i, b, true);
+      color = i;
+      break;
+    }
+  } else if (ta == kBodyDynamic || tb == kBodyDynamic) {
+    const int32_t dyn = ta == kBodyDynamic ? a : b;
+    for (int i = kOverflowColor - 1; i >= 1; --i) {
+      if (!colour_has(r, i, dyn)) {
+        colour_set(r, i, dyn, true);
+        color = i;
+        break;
+      }
+    }
+  }
+  BatchWorldState* ws = r.ws;
+  c.color = color;
+  if (color == kOverflowColor) {
+    if (ws->n_overflow >= r.overflow_cap) {
+      ws->error = kBatchOverflowFull;
+      return;
+    }
+    c.local = ws->n_overflow;
+    r.overflow_list[ws->n_overflow++] = id;
+  } else {
+    c.local = ws->color_count[color];
+    r.colors[color * r.color_cap + ws->color_count[color]++] = id;
+  }
+}
+
+// contact_creation.rs:428-552 and id_pool.rs:36-45
+__device__ void destroy_contact(const WorldRefs& r, int32_t id) {
+  ContactMeta& c = r.meta[id];
+  BatchWorldState* ws = r.ws;
+  if (c.color == kNullIndex) {
+    const int32_t moved = swap_remove(r.non_touching, &ws->n_non_touching, c.local);
+    if (moved != kNullIndex) r.meta[moved].local = c.local;
+  } else {
+    remove_from_graph(r, r.proxy[c.shape_a].body, r.proxy[c.shape_b].body, c.color, c.local);
+  }
+  c.alive = 0;
+  c.color = kNullIndex;
+  c.local = kNullIndex;
+  if (id == ws->next_id - 1) ws->next_id -= 1;
+  else r.free_ids[ws->n_free++] = id;
+}
+
+// collide_pass.rs:386-478: one contact's transition (contacts are visited in id order).
+__device__ void apply_change(const WorldRefs& r, int32_t id, uint8_t ch) {
+  if (ch == kChangeDisjoint) {
+    destroy_contact(r, id);
+  } else if (ch == kChangeBegan) {
+    const int32_t old_local = r.meta[id].local;
+    add_to_graph(r, id);
+    const int32_t moved = swap_remove(r.non_touching, &r.ws->n_non_touching, old_local);
+    if (moved != kNullIndex) r.meta[moved].local = old_local;
+  } else if (ch == kChangeEnded) {
+    ContactMeta& c = r.meta[id];
+    const int32_t color = c.color, local = c.local;
+    c.color = kNullIndex;
+    c.local = r.ws->n_non_touching;
+    r.non_touching[r.ws->n_non_touching++] = id;
+    remove_from_graph(r, r.proxy[c.shape_a].body, r.proxy[c.shape_b].body, color, local);
+  }
+}
+
+// Pair test of the update (contact_creation.rs:901-1042, shape_runtime.rs:285-292).
+__device__ __forceinline__ bool new_pair(const WorldRefs& r, int m, const Proxy& pm, const Aabb& fm, int j) {
+  const Proxy pj = r.proxy[j];
+  if (pm.body_type != kBodyDynamic && pj.body_type != kBodyDynamic) return false;
+  if (pm.body == pj.body) return false;
+  if (!aabb_overlaps(fm, r.fat[j])) return false;
+  if (pm.group == pj.group && pm.group != 0) {
+    if (pm.group < 0) return false;
+  } else if (!((pm.mask & pj.category) != 0 && (pm.category & pj.mask) != 0)) {
+    return false;
+  }
+  return !pair_bit(r, m < j ? m : j, m < j ? j : m);
+}
+
+// Ascending sort of n keys in LDS by one wavefront (bitonic over the next power of two;
+// the padding holds all-ones keys).
+__device__ void sort_keys(uint32_t* keys, int n, int lane) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  for (int i = n + lane; i < p; i += kBatchLanes) keys[i] = ~0u;
+  wave_sync();
+  for (int k = 2; k <= p; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < p; i += kBatchLanes) {
+        const int x = i ^ j;
+        if (x > i) {
+          const uint32_t a = keys[i], b = keys[x];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[x] = a;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+struct Colours {
+  int n;
+  int begin[kOverflowColor];
+  int end[kOverflowColor];
+};
+
+// Pair update of one world (contact_creation.rs:901-1042): every moved proxy against
+// every shape, the new keys sorted, contacts created in key order by one lane.
+__global__ __launch_bounds__(kBatchLanes) void k_batch_pairs(BatchDevice b) {
+  __shared__ uint32_t s_keys[kBatchMaxNewPairs];
+  __shared__ int s_count;
+  const int lane = threadIdx.x;
+  const int world = b.order[blockIdx.x];
+  const BatchWorldDesc wd = b.worlds[world];
+  const WorldRefs r = world_refs(b, wd, world);
+  BatchWorldState* ws = r.ws;
+  if (ws->error != kBatchOk) return;
+  if (lane == 0) s_count = 0;
+  wave_sync();
+  for (int base = 0; base < r.n_shapes; base += kBatchLanes) {
+    const bool mv = base + lane < r.n_shapes && r.moved[base + lane] != 0;
+    uint64_t mset = wave_ballot(mv);
+    while (mset) {
+      const int m = base + (__ffsll((unsigned long long)mset) - 1);
+      mset &= mset - 1;
+      const Proxy pm = r.proxy[m];
+      const Aabb fm = r.fat[m];
+      for (int j0 = 0; j0 < r.n_shapes; j0 += kBatchLanes) {
+        const int j = j0 + lane;
+        // a pair of two moved proxies is found from the lower one
+        const bool hit = j < r.n_shapes && j != m && (j > m || r.moved[j] == 0) && new_pair(r, m, pm, fm, j);
+        const uint64_t hits = wave_ballot(hit);
+        if (hits == 0) continue;
+        const int base_count = s_count;
+        const int total = __popcll(hits);
+        if (base_count + total > kBatchMaxNewPairs) {
+          if (lane == 0) ws->error = kBatchPairsFull;
+          return;
+        }
+        if (hit) {
+          const int lo = m < j ? m : j, hi = m < j ? j : m;
+          s_keys[base_count + lanes_below(hits, lane)] = ((uint32_t)lo << 16) | (uint32_t)hi;
+        }
+        wave_sync();
+        if (lane == 0) s_count = base_count + total;
+        wave_sync();
+      }
+    }
+  }
+  const int n_keys = s_count;
+  if (n_keys > 0) {
+    sort_keys(s_keys, n_keys, lane);
+    if (lane == 0)
+      for (int k = 0; k < n_keys && ws->error == kBatchOk; ++k)
+        create_contact(r, (int32_t)(s_keys[k] >> 16), (int32_t)(s_keys[k] & 0xffffu), b.enable_speculative != 0);
+  }
+  for (int s = lane; s < r.n_shapes; s += kBatchLanes) r.moved[s] = 0;
+}
+
+// Collide pass of one world over its live contacts (collide_pass.rs:541-742).
+__global__ __launch_bounds__(kBatchCollideBlock) void k_batch_collide(BatchDevice b) {
+  const int world = b.order[blockIdx.x];
+  const BatchWorldDesc wd = b.worlds[world];
+  const WorldRefs r = world_refs(b, wd, world);
+  if (r.ws->error != kBatchOk) return;
+  const DeviceWorld dg = world_view(b, wd, b.states + wd.awake_off);
+  const int next_id = r.ws->next_id;
+  for (int id = threadIdx.x; id < next_id; id += blockDim.x) {
+    uint8_t ch = kChangeNone;
+    if (r.meta[id].alive) {
+      ch = collide_contact(dg, id);
+      if (ch == kChangeDisjoint) {
+        const int a = r.meta[id].shape_a, c = r.meta[id].shape_b;
+        set_pair_bit(r, a < c ? a : c, a < c ? c : a, false);
+      }
+    }
+    r.change[id] = ch;
+  }
+}
+
+// Contact transitions in id order (one lane), the solve order, then prepare, the
+// substeps with the world's body states in LDS, restitution, impulse storage and
+// finalisation (solver_stages.rs:543-589, solver_integration.rs:587-799).
+__global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, StepParams p) {
+  __shared__ BodyState s_states[kBatchMaxAwake];
+  __shared__ Colours colours;
+  __shared__ int s_count, s_any;
+  const int lane = threadIdx.x;
+  const int world = b.order[blockIdx.x];
+  const BatchWorldDesc wd = b.worlds[world];
+  const WorldRefs r = world_refs(b, wd, world);
+  const DeviceWorld dg = world_view(b, wd, b.states + wd.awake_off);
+  const DeviceWorld dl = world_view(b, wd, s_states);
+  BatchWorldState* ws = r.ws;
+  if (ws->error != kBatchOk) return;
+  const int next_id = ws->next_id;
+  // ---- transitions in contact-id order (one lane walks the changed ids)
+  for (int base = 0; base < next_id; base += kBatchLanes) {
+    const int id = base + lane;
+    uint64_t changed = wave_ballot(id < next_id && r.change[id] != kChangeNone);
+    if (lane == 0)
+      while (changed) {
+        const int k = base + (__ffsll((unsigned long long)changed) - 1);
+        changed &= changed - 1;
+        apply_change(r, k, r.change[k]);
+      }
+    wave_sync();
+  }
+  if (ws->error != kBatchOk) return;
+  if (lane == 0) ws->steps += 1;
+  if (wd.n_awake == 0) return;
+
+  // ---- solve order: colours 0..22 in graph order, then the overflow list
+  if (lane == 0) {
+    int n = 0, slots = 0;
+    for (int ci = 0; ci < kOverflowColor; ++ci) {
+      const int cnt = ws->color_count[ci];
+      if (cnt > 0) {
+        colours.begin[n] = slots;
+        colours.end[n] = slots + cnt;
+        ++n;
+      }
+      slots += cnt;
+    }
+    colours.n = n;
+    s_count = slots;
+    s_any = 0;
+  }
+  wave_sync();
+  const int n_constraints = s_count;
+  const int n_ovf = ws->n_overflow;
+  {
+    int slot = 0;
+    bool any_rest = false;
+    for (int ci = 0; ci < kOverflowColor; ++ci) {
+      const int cnt = ws->color_count[ci];
+      for (int k = lane; k < cnt; k += kBatchLanes) {
+        const int32_t id = r.colors[ci * r.color_cap + k];
+        dl.solve_ids[slot + k] = id;
+        any_rest = any_rest || r.meta[id].restitution > 0.0f;
+      }
+      slot += cnt;
+    }
+    for (int k = lane; k < n_ovf; k += kBatchLanes) {
+      const int32_t id = r.overflow_list[k];
+      dl.overflow_ids[k] = id;
+      any_rest = any_rest || r.meta[id].restitution > 0.0f;
+    }
+    if (wave_ballot(any_rest) != 0 && lane == 0) s_any = 1;
+  }
+  for (int i = lane; i < wd.n_awake; i += kBatchLanes) s_states[i] = dg.states[i];
+  wave_sync();
+  const bool any_restitution = s_any != 0;
+
+  // ---- prepare, substeps, restitution, store, finalize (solver_stages.rs:543-589)
+  for (int slot = lane; slot < n_constraints; slot += kBatchLanes) prepare_item(dl, slot, p);
+  for (int k = lane; k < n_ovf; k += kBatchLanes) prepare_overflow_item(dl, k, p);
+  wave_sync();
+  for (int sub = 0; sub < p.sub_steps; ++sub) {
+    for (int i = lane; i < wd.n_awake; i += kBatchLanes) integrate_velocities_item(dl, i, p);
+    wave_sync();
+    if (n_ovf > 0) {
+      if (lane == 0) overflow_warm_all(dl, n_ovf);
+      wave_sync();
+    }
+    for (int a = 0; a < colours.n; ++a) {
+      for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
+        warm_start_item(dl, slot, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
+      wave_sync();
+    }
+    if (n_ovf > 0) {
+      if (lane == 0) overflow_solve_all(dl, n_ovf, p, 0);
+      wave_sync();
+    }
+    for (int a = 0; a < colours.n; ++a) {
+      for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
+        push_item(dl, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
+      wave_sync();
+    }
+    for (int i = lane; i < wd.n_awake; i += kBatchLanes) integrate_positions_item(dl, i, p);
+    wave_sync();
+    if (n_ovf > 0) {
+      if (lane == 0) overflow_solve_all(dl, n_ovf, p, 1);
+      wave_sync();
+    }
+    for (int a = 0; a < colours.n; ++a) {
+      for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
+        relax_item(dl, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
+      wave_sync();
+    }
+  }
+  if (any_restitution) {
+    for (int it = 0; it < b.restitution_iterations; ++it) {
+      if (n_ovf > 0) {
+        if (lane == 0) overflow_solve_all(dl, n_ovf, p, 2);
+        wave_sync();
+      }
+      for (int a = 0; a < colours.n; ++a) {
+        for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
+          restitution_item(dl, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
+        wave_sync();
+      }
+    }
+  }
+  for (int k = lane; k < n_ovf; k += kBatchLanes) overflow_store_item(dl, k);
+  for (int slot = lane; slot < n_constraints; slot += kBatchLanes) store_item(dl, slot);
+  for (int i = lane; i < wd.n_awake; i += kBatchLanes) finalize_item(dl, i, p);
+  wave_sync();
+  for (int i = lane; i < wd.n_awake; i += kBatchLanes) dg.states[i] = s_states[i];
+}
+
+}  // namespace
+
+void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hipStream_t s) {
+  if (b.n_worlds <= 0) return;
+  for (int k = 0; k < steps; ++k) {
+    hipLaunchKernelGGL(k_batch_pairs, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, b);
+    hipLaunchKernelGGL(k_batch_collide, dim3(b.n_worlds), dim3(kBatchCollideBlock), 0, s, b);
+    hipLaunchKernelGGL(k_batch_solve, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, b, p);
+  }
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) throw std::runtime_error(std::string("batch step launch: ") + hipGetErrorString(err));
+}
+
+// ---------------------------------------------------------------------------------
+// host side
+
+BatchStepper::BatchStepper(int device) : device_(device) {
+  BATCH_CHECK(hipSetDevice(device_));
+  BATCH_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  BATCH_CHECK(hipEventCreate(&ev0_));
+  BATCH_CHECK(hipEventCreate(&ev1_));
+}
+
+BatchStepper::~BatchStepper() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (void* p : allocations_) (void)hipFree(p);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int32_t BatchStepper::add_world() {
+  if (uploaded_) throw std::invalid_argument("worlds can only be added before the first step");
+  scenes_.push_back(std::make_unique<HostScene>());
+  scenes_.back()->isotropic_inertia_tolerance = options.isotropic_inertia_tolerance;
+  return (int32_t)scenes_.size() - 1;
+}
+
+HostScene& BatchStepper::scene(int32_t w) {
+  if (w < 0 || w >= (int32_t)scenes_.size()) throw std::invalid_argument("bad world index");
+  if (uploaded_) throw std::invalid_argument("bodies and shapes can only be added before the first step");
+  return *scenes_[w];
+}
+
+template <class T>
+T* BatchStepper::alloc(size_t n, bool zero) {
+  T* p = nullptr;
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  BATCH_CHECK(hipMalloc(&p, bytes));
+  allocations_.push_back(p);
+  if (zero) BATCH_CHECK(hipMemsetAsync(p, 0, bytes, stream_));
+  return p;
+}
+
+template <class T>
+static T* upload_vec(hipStream_t s, T* dst, const std::vector<T>& v) {
+  if (!v.empty()) {
+    const hipError_t e = hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) throw std::runtime_error(std::string("batch upload: ") + hipGetErrorString(e));
+  }
+  return dst;
+}
+
+void BatchStepper::upload() {
+  const int nw = (int)scenes_.size();
+  desc_.assign(nw, BatchWorldDesc{});
+  std::vector<BodySim> sims, statics;
+  std::vector<BodyState> states;
+  std::vector<BodyMeta> bmeta;
+  std::vector<Shape> shapes;
+  std::vector<Aabb> fat;
+  std::vector<uint8_t> moved;
+  std::vector<Proxy> proxies;
+  std::vector<BoxHull> hulls;
+  int64_t contact_total = 0, overflow_total = 0, color_total = 0, bits_total = 0, pair_total = 0;
+  for (int w = 0; w < nw; ++w) {
+    const HostScene& sc = *scenes_[w];
+    BatchWorldDesc& d = desc_[w];
+    d.n_awake = (int32_t)sc.awake_host_.size();
+    d.n_static = (int32_t)sc.static_host_.size();
+    d.n_bodies = (int32_t)sc.bodies_.size();
+    d.n_shapes = (int32_t)sc.shapes_.size();
+    if (d.n_awake > kBatchMaxAwake) throw std::invalid_argument("a batched world holds at most 256 dynamic bodies");
+    if (d.n_shapes > kBatchMaxShapes) throw std::invalid_argument("a batched world holds at most 4096 shapes");
+    for (const Shape& s : sc.shapes_)
+      if (s.kind == kShapeHull && s.hull < 0) throw std::invalid_argument("bad hull");
+    d.awake_off = (int32_t)sims.size();
+    d.static_off = (int32_t)statics.size();
+    d.body_off = (int32_t)bmeta.size();
+    d.shape_off = (int32_t)shapes.size();
+    d.hull_off = (int32_t)hulls.size();
+    sims.insert(sims.end(), sc.awake_host_.begin(), sc.awake_host_.end());
+    states.insert(states.end(), sc.states_host_.begin(), sc.states_host_.end());
+    statics.insert(statics.end(), sc.static_host_.begin(), sc.static_host_.end());
+    for (const HostScene::HBody& hb : sc.bodies_) bmeta.push_back(BodyMeta{hb.type, sc.encode(hb), hb.flags});
+    shapes.insert(shapes.end(), sc.shapes_.begin(), sc.shapes_.end());
+    for (const HostScene::HShape& hs : sc.hshapes_) {
+      fat.push_back(hs.fat);
+      moved.push_back(hs.proxy_moved ? 1 : 0);
+      proxies.push_back(Proxy{hs.body, sc.bodies_[hs.body].type, hs.group, hs.speculative ? 1 : 0, hs.category, hs.mask});
+    }
+    hulls.insert(hulls.end(), sc.hulls_.begin(), sc.hulls_.end());
+    // capacities: a pile keeps a few contacts per shape; the overflow colour is rare
+    d.contact_cap = std::max(64, 12 * d.n_shapes);
+    d.overflow_cap = std::max(16, d.contact_cap / 4);
+    d.color_cap = std::max(1, d.n_bodies);
+    d.bits_words = (d.n_bodies + 31) / 32;
+    d.contact_off = (int32_t)contact_total;
+    d.overflow_off = (int32_t)overflow_total;
+    d.color_off = (int32_t)color_total;
+    d.bits_off = (int32_t)bits_total;
+    d.pair_off = (int32_t)pair_total;
+    contact_total += d.contact_cap;
+    overflow_total += d.overflow_cap;
+    color_total += (int64_t)kOverflowColor * d.color_cap;
+    bits_total += (int64_t)kOverflowColor * std::max(1, d.bits_words);
+    pair_total += ((int64_t)d.n_shapes * d.n_shapes + 31) / 32;
+    if (contact_total > INT32_MAX || color_total > INT32_MAX || pair_total > INT32_MAX)
+      throw std::invalid_argument("batch too large for 32-bit offsets");
+  }
+  std::vector<int32_t> order(nw);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return desc_[a].n_awake > desc_[b].n_awake; });
+
+  BatchDevice& b = d_;
+  b.n_worlds = nw;
+  b.enable_speculative = options.enable_speculative ? 1 : 0;
+  b.restitution_iterations = options.restitution_iterations;
+  b.sims = upload_vec(stream_, alloc<BodySim>(sims.size(), false), sims);
+  b.states = upload_vec(stream_, alloc<BodyState>(states.size(), false), states);
+  b.static_sims = upload_vec(stream_, alloc<BodySim>(statics.size(), false), statics);
+  b.body_meta = upload_vec(stream_, alloc<BodyMeta>(bmeta.size(), false), bmeta);
+  b.shapes = upload_vec(stream_, alloc<Shape>(shapes.size(), false), shapes);
+  b.fat_aabbs = upload_vec(stream_, alloc<Aabb>(fat.size(), false), fat);
+  b.shape_moved = upload_vec(stream_, alloc<uint8_t>(moved.size(), false), moved);
+  b.proxies = upload_vec(stream_, alloc<Proxy>(proxies.size(), false), proxies);
+  b.hulls = upload_vec(stream_, alloc<BoxHull>(hulls.size(), false), hulls);
+  b.contacts = alloc<Contact>(contact_total, false);
+  b.contact_change = alloc<uint8_t>(contact_total, true);
+  b.contact_meta = alloc<ContactMeta>(contact_total, true);
+  b.constraints = alloc<Constraint>(contact_total, false);
+  b.solve_ids = alloc<int32_t>(contact_total, false);
+  b.free_ids = alloc<int32_t>(contact_total, false);
+  b.non_touching = alloc<int32_t>(contact_total, false);
+  b.overflow = alloc<OverflowConstraint>(overflow_total, false);
+  b.overflow_ids = alloc<int32_t>(overflow_total, false);
+  b.overflow_list = alloc<int32_t>(overflow_total, false);
+  b.color_lists = alloc<int32_t>(color_total, false);
+  b.color_bits = alloc<uint32_t>(bits_total, true);
+  b.pair_bits = alloc<uint32_t>(pair_total, true);
+  b.worlds = upload_vec(stream_, alloc<BatchWorldDesc>(nw, false), desc_);
+  b.wstate = alloc<BatchWorldState>(nw, true);
+  b.order = upload_vec(stream_, alloc<int32_t>(nw, false), order);
+  b.counters = alloc<int32_t>(8, true);
+  BATCH_CHECK(hipStreamSynchronize(stream_));  // the host vectors are read by the copies
+  total_awake_ = (int64_t)sims.size();
+  total_contact_cap_ = contact_total;
+  uploaded_ = true;
+}
+
+void BatchStepper::step(float dt, int32_t sub_steps, int32_t steps) {
+  if (!(dt > 0.0f)) throw std::invalid_argument("the batched stepper needs a positive time step");
+  if (steps <= 0) return;
+  BATCH_CHECK(hipSetDevice(device_));
+  if (!uploaded_) upload();
+  StepParams p{};
+  p.dt = dt;
+  p.sub_steps = std::max(1, sub_steps);
+  p.inv_dt = 1.0f / dt;
+  p.h = dt / (float)p.sub_steps;
+  p.inv_h = (float)p.sub_steps * p.inv_dt;
+  const float contact_hz = minf(options.contact_hertz, 0.125f * p.inv_h);
+  p.contact_softness = make_soft(contact_hz, options.contact_damping_ratio, p.h);
+  p.static_softness = make_soft(2.0f * contact_hz, 0.5f * options.contact_damping_ratio, p.h);
+  p.contact_speed = options.contact_speed;
+  p.max_linear_speed = options.max_linear_speed;
+  p.restitution_threshold = options.restitution_threshold;
+  p.gravity = options.gravity;
+  p.warm_start_scale = options.enable_warm_starting ? 1.0f : 0.0f;
+  p.enable_sleep = options.enable_sleep;
+  p.enable_continuous = options.enable_continuous;
+  p.enable_restitution_propagation = options.enable_restitution_propagation;
+  BATCH_CHECK(hipEventRecord(ev0_, stream_));
+  launch_batch_step(d_, p, steps, stream_);
+  BATCH_CHECK(hipEventRecord(ev1_, stream_));
+  BATCH_CHECK(hipEventSynchronize(ev1_));
+  BATCH_CHECK(hipEventElapsedTime(&last_ms_, ev0_, ev1_));
+  body_steps_ += total_awake_ * (int64_t)steps;
+  device_newer_ = true;
+}
+
+void BatchStepper::pull() {
+  if (!device_newer_) return;
+  std::vector<BodySim> sims((size_t)total_awake_);
+  std::vector<BodyState> states((size_t)total_awake_);
+  if (total_awake_ > 0) {
+    BATCH_CHECK(hipMemcpyAsync(sims.data(), d_.sims, sims.size() * sizeof(BodySim), hipMemcpyDeviceToHost, stream_));
+    BATCH_CHECK(hipMemcpyAsync(states.data(), d_.states, states.size() * sizeof(BodyState), hipMemcpyDeviceToHost, stream_));
+  }
+  BATCH_CHECK(hipStreamSynchronize(stream_));
+  for (size_t w = 0; w < scenes_.size(); ++w) {
+    HostScene& sc = *scenes_[w];
+    const BatchWorldDesc& d = desc_[w];
+    std::copy(sims.begin() + d.awake_off, sims.begin() + d.awake_off + d.n_awake, sc.awake_host_.begin());
+    std::copy(states.begin() + d.awake_off, states.begin() + d.awake_off + d.n_awake, sc.states_host_.begin());
+  }
+  device_newer_ = false;
+}
+
+void BatchStepper::state_hashes(uint64_t* out) {
+  pull();
+  for (size_t w = 0; w < scenes_.size(); ++w) out[w] = scenes_[w]->state_hash();
+}
+
+void BatchStepper::get_bodies(int32_t w, double* pos, float* rot, float* v, float* wv) {
+  if (w < 0 || w >= (int32_t)scenes_.size()) throw std::invalid_argument("bad world index");
+  pull();
+  scenes_[w]->get_bodies(pos, rot, v, wv);
+}
+
+int32_t BatchStepper::contact_slots(int32_t w) const {
+  if (w < 0 || w >= (int32_t)scenes_.size()) throw std::invalid_argument("bad world index");
+  return uploaded_ ? desc_[w].contact_cap : 0;
+}
+
+void BatchStepper::get_contact_colors(int32_t w, int32_t* color, int32_t* local, int32_t* alive) {
+  const int32_t n = contact_slots(w);
+  if (n == 0) return;
+  std::vector<ContactMeta> meta((size_t)n);
+  BATCH_CHECK(hipMemcpy(meta.data(), d_.contact_meta + desc_[w].contact_off, (size_t)n * sizeof(ContactMeta), hipMemcpyDeviceToHost));
+  for (int32_t i = 0; i < n; ++i) {
+    color[i] = meta[i].alive ? meta[i].color : kNullIndex;
+    local[i] = meta[i].alive ? meta[i].local : kNullIndex;
+    alive[i] = meta[i].alive;
+  }
+}
+
+void BatchStepper::get_shape_bounds(int32_t w, float* fat, uint8_t* moved) {
+  if (w < 0 || w >= (int32_t)scenes_.size()) throw std::invalid_argument("bad world index");
+  const HostScene& sc = *scenes_[w];
+  const size_t n = sc.hshapes_.size();
+  std::vector<Aabb> boxes(n);
+  std::vector<uint8_t> flags(n);
+  if (uploaded_ && n) {
+    BATCH_CHECK(hipMemcpy(boxes.data(), d_.fat_aabbs + desc_[w].shape_off, n * sizeof(Aabb), hipMemcpyDeviceToHost));
+    BATCH_CHECK(hipMemcpy(flags.data(), d_.shape_moved + desc_[w].shape_off, n, hipMemcpyDeviceToHost));
+  } else {
+    for (size_t i = 0; i < n; ++i) {
+      boxes[i] = sc.hshapes_[i].fat;
+      flags[i] = sc.hshapes_[i].proxy_moved ? 1 : 0;
+    }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const Aabb& b = boxes[i];
+    const float v[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+    for (int k = 0; k < 6; ++k) fat[6 * i + k] = v[k];
+    moved[i] = flags[i];
+  }
+}
+
+void BatchStepper::get_contact_shapes(int32_t w, int32_t* shape_a, int32_t* shape_b) {
+  const int32_t n = contact_slots(w);
+  if (n == 0) return;
+  std::vector<ContactMeta> meta((size_t)n);
+  BATCH_CHECK(hipMemcpy(meta.data(), d_.contact_meta + desc_[w].contact_off, (size_t)n * sizeof(ContactMeta), hipMemcpyDeviceToHost));
+  for (int32_t i = 0; i < n; ++i) {
+    shape_a[i] = meta[i].alive ? meta[i].shape_a : kNullIndex;
+    shape_b[i] = meta[i].alive ? meta[i].shape_b : kNullIndex;
+  }
+}
+
+int32_t BatchStepper::error_worlds(int32_t* first_error) {
+  *first_error = 0;
+  if (!uploaded_) return 0;
+  std::vector<BatchWorldState> ws(scenes_.size());
+  BATCH_CHECK(hipMemcpy(ws.data(), d_.wstate, ws.size() * sizeof(BatchWorldState), hipMemcpyDeviceToHost));
+  int32_t n = 0;
+  for (const BatchWorldState& s : ws)
+    if (s.error != kBatchOk) {
+      if (n == 0) *first_error = s.error;
+      ++n;
+    }
+  return n;
+}
+
+}  // namespace nsl

@@ -13,14 +13,14 @@ namespace nsl {
 
 // hull.rs:736-750, (next, twin, origin, face) per half-edge. Feature ids index into
 // this table, so it is the reference's box topology.
-__constant__ uint8_t c_box_edge[24][4] = {
+static __constant__ uint8_t c_box_edge[24][4] = {
     {2, 1, 2, 0},   {17, 0, 1, 5},  {4, 3, 1, 0},   {20, 2, 5, 3},  {6, 5, 5, 0},   {23, 4, 6, 4},
     {0, 7, 6, 0},   {18, 6, 2, 2},  {10, 9, 0, 1},  {21, 8, 3, 5},  {12, 11, 3, 1}, {16, 10, 7, 2},
     {14, 13, 7, 1}, {19, 12, 4, 4}, {8, 15, 4, 1},  {22, 14, 0, 3}, {7, 17, 3, 2},  {9, 16, 2, 5},
     {11, 19, 6, 2}, {5, 18, 7, 4},  {15, 21, 1, 3}, {1, 20, 0, 5},  {3, 23, 4, 3},  {13, 22, 5, 4},
 };
-__constant__ uint8_t c_box_vertex_edge[8] = {8, 1, 0, 9, 13, 3, 5, 11};
-__constant__ uint8_t c_box_face_edge[6] = {0, 8, 16, 20, 19, 21};
+static __constant__ uint8_t c_box_vertex_edge[8] = {8, 1, 0, 9, 13, 3, 5, 11};
+static __constant__ uint8_t c_box_face_edge[6] = {0, 8, 16, 20, 19, 21};
 
 #define E_NEXT(i) c_box_edge[i][0]
 #define E_TWIN(i) c_box_edge[i][1]

@@ -3,11 +3,18 @@
 #include <new>
 #include <string>
 
+#include <stdexcept>
+
 #include "../../include/natsel_gpu.h"
+#include "nsl_batch.h"
 #include "nsl_world.h"
 
 struct nsl_world {
   nsl::GpuWorld* impl;
+};
+
+struct nsl_batch {
+  nsl::BatchStepper* impl;
 };
 
 namespace {
@@ -25,6 +32,8 @@ int guarded(F&& f) {
     return NSL_OK;
   } catch (const std::bad_alloc&) {
     return fail(NSL_ERR_DEVICE, "out of memory");
+  } catch (const std::invalid_argument& e) {
+    return fail(NSL_ERR_INVALID, e.what());
   } catch (const std::exception& e) {
     return fail(NSL_ERR_DEVICE, e.what());
   } catch (...) {
@@ -59,6 +68,28 @@ nsl::Material material(const nsl_material* m) {
 }
 
 bool valid_body(nsl_world* w, int32_t body) { return w && body >= 0 && body < w->impl->body_capacity(); }
+
+nsl::BodyDesc body_desc(const nsl_body_def* def) {
+  nsl::BodyDesc d;
+  d.type = def->type;
+  d.position = nsl::Pos{def->position[0], def->position[1], def->position[2]};
+  d.rotation = nsl::Quat{{def->rotation[0], def->rotation[1], def->rotation[2]}, def->rotation[3]};
+  d.linear_velocity = nsl::Vec3{def->linear_velocity[0], def->linear_velocity[1], def->linear_velocity[2]};
+  d.angular_velocity = nsl::Vec3{def->angular_velocity[0], def->angular_velocity[1], def->angular_velocity[2]};
+  d.linear_damping = def->linear_damping;
+  d.angular_damping = def->angular_damping;
+  d.gravity_scale = def->gravity_scale;
+  d.enable_sleep = def->enable_sleep != 0;
+  return d;
+}
+
+// World `w` of a batch with body `body` (or no body check when body < -1 is passed as -2).
+nsl::HostScene* batch_scene(nsl_batch* b, int32_t w, int32_t body) {
+  if (!b || w < 0 || w >= b->impl->world_count()) return nullptr;
+  nsl::HostScene* sc = &b->impl->scene(w);
+  if (body != -2 && (body < 0 || body >= (int32_t)sc->bodies_.size())) return nullptr;
+  return sc;
+}
 }  // namespace
 
 extern "C" {
@@ -110,19 +141,7 @@ int nsl_world_set_options(nsl_world* world, const nsl_options* options) {
 int nsl_create_body(nsl_world* world, const nsl_body_def* def, int32_t* out_body) {
   if (!world || !def || !out_body) return fail(NSL_ERR_INVALID, "null argument");
   if (def->type < NSL_BODY_STATIC || def->type > NSL_BODY_DYNAMIC) return fail(NSL_ERR_INVALID, "bad body type");
-  return guarded([&] {
-    nsl::BodyDesc d;
-    d.type = def->type;
-    d.position = nsl::Pos{def->position[0], def->position[1], def->position[2]};
-    d.rotation = nsl::Quat{{def->rotation[0], def->rotation[1], def->rotation[2]}, def->rotation[3]};
-    d.linear_velocity = nsl::Vec3{def->linear_velocity[0], def->linear_velocity[1], def->linear_velocity[2]};
-    d.angular_velocity = nsl::Vec3{def->angular_velocity[0], def->angular_velocity[1], def->angular_velocity[2]};
-    d.linear_damping = def->linear_damping;
-    d.angular_damping = def->angular_damping;
-    d.gravity_scale = def->gravity_scale;
-    d.enable_sleep = def->enable_sleep != 0;
-    *out_body = world->impl->create_body(d);
-  });
+  return guarded([&] { *out_body = world->impl->create_body(body_desc(def)); });
 }
 
 int nsl_create_box(nsl_world* world, int32_t body, const float h[3], const nsl_material* m, int32_t* out_shape) {
@@ -243,6 +262,130 @@ int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out) {
   out->pair_tests = k.pair_tests;
   for (int i = 0; i < 4; ++i) out->host_ms[i] = k.host_ms[i];
   return NSL_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// batches of independent worlds
+
+int nsl_batch_create(int device, const nsl_options* options, nsl_batch** out) {
+  if (!out) return fail(NSL_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  return guarded([&] {
+    auto* b = new nsl_batch{new nsl::BatchStepper(device)};
+    if (options) apply(b->impl->options, *options);
+    *out = b;
+  });
+}
+
+int nsl_batch_destroy(nsl_batch* batch) {
+  if (!batch) return fail(NSL_ERR_INVALID, "null batch");
+  return guarded([&] {
+    delete batch->impl;
+    delete batch;
+  });
+}
+
+int nsl_batch_add_world(nsl_batch* batch, int32_t* out_world) {
+  if (!batch || !out_world) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { *out_world = batch->impl->add_world(); });
+}
+
+int nsl_batch_create_body(nsl_batch* batch, int32_t world, const nsl_body_def* def, int32_t* out_body) {
+  if (!def || !out_body) return fail(NSL_ERR_INVALID, "null argument");
+  if (def->type < NSL_BODY_STATIC || def->type > NSL_BODY_DYNAMIC) return fail(NSL_ERR_INVALID, "bad body type");
+  return guarded([&] {
+    nsl::HostScene* sc = batch_scene(batch, world, -2);
+    if (!sc) throw std::invalid_argument("bad batch or world");
+    *out_body = sc->create_body(body_desc(def));
+  });
+}
+
+int nsl_batch_create_box(nsl_batch* batch, int32_t world, int32_t body, const float h[3], const nsl_material* m,
+                         int32_t* out_shape) {
+  if (!h || !out_shape) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] {
+    nsl::HostScene* sc = batch_scene(batch, world, body);
+    if (!sc) throw std::invalid_argument("bad batch, world or body");
+    *out_shape = sc->create_box(body, nsl::Vec3{h[0], h[1], h[2]}, material(m));
+  });
+}
+
+int nsl_batch_create_sphere(nsl_batch* batch, int32_t world, int32_t body, const float c[3], float radius,
+                            const nsl_material* m, int32_t* out_shape) {
+  if (!c || !out_shape) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] {
+    nsl::HostScene* sc = batch_scene(batch, world, body);
+    if (!sc) throw std::invalid_argument("bad batch, world or body");
+    *out_shape = sc->create_sphere(body, nsl::Vec3{c[0], c[1], c[2]}, radius, material(m));
+  });
+}
+
+int nsl_batch_create_capsule(nsl_batch* batch, int32_t world, int32_t body, const float c1[3], const float c2[3], float radius,
+                             const nsl_material* m, int32_t* out_shape) {
+  if (!c1 || !c2 || !out_shape) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] {
+    nsl::HostScene* sc = batch_scene(batch, world, body);
+    if (!sc) throw std::invalid_argument("bad batch, world or body");
+    *out_shape = sc->create_capsule(body, nsl::Vec3{c1[0], c1[1], c1[2]}, nsl::Vec3{c2[0], c2[1], c2[2]}, radius, material(m));
+  });
+}
+
+int nsl_batch_step(nsl_batch* batch, float dt, int32_t sub_steps, int32_t steps) {
+  if (!batch || !(dt > 0.0f) || steps < 0) return fail(NSL_ERR_INVALID, "null batch, non-positive time step or negative steps");
+  return guarded([&] { batch->impl->step(dt, sub_steps, steps); });
+}
+
+int nsl_batch_world_count(nsl_batch* batch, int32_t* out) {
+  if (!batch || !out) return fail(NSL_ERR_INVALID, "null argument");
+  *out = batch->impl->world_count();
+  return NSL_OK;
+}
+
+int nsl_batch_body_count(nsl_batch* batch, int32_t world, int32_t* out) {
+  if (!batch || !out || world < 0 || world >= batch->impl->world_count()) return fail(NSL_ERR_INVALID, "bad argument");
+  return guarded([&] { *out = batch->impl->body_count(world); });
+}
+
+int nsl_batch_state_hashes(nsl_batch* batch, uint64_t* out) {
+  if (!batch || !out) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { batch->impl->state_hashes(out); });
+}
+
+int nsl_batch_get_bodies(nsl_batch* batch, int32_t world, double* pos, float* rot, float* v, float* w) {
+  if (!batch || !pos || !rot || !v || !w) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { batch->impl->get_bodies(world, pos, rot, v, w); });
+}
+
+int nsl_batch_contact_capacity(nsl_batch* batch, int32_t world, int32_t* out) {
+  if (!batch || !out || world < 0 || world >= batch->impl->world_count()) return fail(NSL_ERR_INVALID, "bad argument");
+  return guarded([&] { *out = batch->impl->contact_slots(world); });
+}
+
+int nsl_batch_get_contact_colors(nsl_batch* batch, int32_t world, int32_t* color, int32_t* local, int32_t* alive) {
+  if (!batch || !color || !local || !alive || world < 0 || world >= batch->impl->world_count())
+    return fail(NSL_ERR_INVALID, "bad argument");
+  return guarded([&] { batch->impl->get_contact_colors(world, color, local, alive); });
+}
+
+int nsl_batch_get_contact_shapes(nsl_batch* batch, int32_t world, int32_t* shape_a, int32_t* shape_b) {
+  if (!batch || !shape_a || !shape_b || world < 0 || world >= batch->impl->world_count()) return fail(NSL_ERR_INVALID, "bad argument");
+  return guarded([&] { batch->impl->get_contact_shapes(world, shape_a, shape_b); });
+}
+
+int nsl_batch_get_shape_bounds(nsl_batch* batch, int32_t world, float* fat, uint8_t* moved) {
+  if (!batch || !fat || !moved || world < 0 || world >= batch->impl->world_count()) return fail(NSL_ERR_INVALID, "bad argument");
+  return guarded([&] { batch->impl->get_shape_bounds(world, fat, moved); });
+}
+
+int nsl_batch_get_stats(nsl_batch* batch, nsl_batch_stats* out) {
+  if (!batch || !out) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] {
+    out->last_step_ms = batch->impl->last_step_ms();
+    out->body_steps = batch->impl->total_body_steps();
+    out->bodies = batch->impl->total_bodies();
+    out->contact_capacity = batch->impl->contact_capacity();
+    out->error_worlds = batch->impl->error_worlds(&out->first_error);
+  });
 }
 
 }  // extern "C"

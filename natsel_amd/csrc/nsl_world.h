@@ -12,38 +12,9 @@
 
 #include "nsl_broadphase.h"
 #include "nsl_device.h"
+#include "nsl_scene.h"
 
 namespace nsl {
-
-struct BodyDesc {
-  int32_t type = kBodyStatic;
-  Pos position{0, 0, 0};
-  Quat rotation{{0, 0, 0}, 1};
-  Vec3 linear_velocity{0, 0, 0}, angular_velocity{0, 0, 0};
-  float linear_damping = 0, angular_damping = 0, gravity_scale = 1.0f;
-  float sleep_threshold = 0.05f, safety_factor = 0.5f;
-  uint32_t lock_flags = 0;
-  bool enable_sleep = true, is_bullet = false, allow_fast_rotation = false, enable_contact_recycling = true;
-};
-
-struct Material {
-  float density = 1000.0f, friction = 0.6f, restitution = 0.0f, rolling_resistance = 0.0f;
-  bool enable_speculative = true, enable_hit_events = false;
-  uint64_t category_bits = 1, mask_bits = ~0ull;
-  int32_t group_index = 0;
-};
-
-struct WorldOptions {
-  Vec3 gravity{0.0f, -10.0f, 0.0f};
-  float restitution_threshold = 1.0f;
-  int32_t restitution_iterations = 2;
-  bool enable_restitution_propagation = false;
-  float max_linear_speed = 400.0f;
-  float contact_speed = 3.0f, contact_hertz = 30.0f, contact_damping_ratio = 10.0f;
-  float isotropic_inertia_tolerance = 0.2f;
-  bool enable_sleep = true, enable_warm_starting = true, enable_continuous = true, enable_speculative = true;
-  int32_t solver_blocks = 0;  // workgroups of the step kernel; 0 = sized from the largest phase
-};
 
 // Wall-clock phases of the last step, in milliseconds (types/world_def.rs:158-184).
 struct StepProfile {
@@ -52,7 +23,7 @@ struct StepProfile {
   int32_t fast_bodies, sat_calls, sat_hits, recycled;
 };
 
-class GpuWorld {
+class GpuWorld : public HostScene {
  public:
   explicit GpuWorld(int device);
   ~GpuWorld();
@@ -61,10 +32,7 @@ class GpuWorld {
 
   WorldOptions options;
 
-  int32_t create_body(const BodyDesc& desc);
-  int32_t create_box(int32_t body, Vec3 half_extents, const Material& m);
-  int32_t create_sphere(int32_t body, Vec3 center, float radius, const Material& m);
-  int32_t create_capsule(int32_t body, Vec3 c1, Vec3 c2, float radius, const Material& m);
+  int32_t create_body(const BodyDesc& desc) override;
 
   void step(float dt, int32_t sub_steps);
 
@@ -95,19 +63,6 @@ class GpuWorld {
   void reset_kernel_stats();
 
  private:
-  struct HBody {
-    int32_t type = kBodyStatic, set = 0, local = -1, head_shape = -1;
-    uint32_t flags = 0;
-  };
-  struct HShape {
-    int32_t id = -1, body = -1, kind = kShapeHull, hull = -1;
-    uint64_t category = 1, mask = ~0ull;
-    int32_t group = 0;
-    float density = 0, restitution = 0, rolling_resistance = 0, radius = 0;
-    bool speculative = true;
-    bool proxy_moved = false;
-    Aabb fat{};
-  };
   struct HContact {
     int32_t shape_a = -1, shape_b = -1, color = -1, local = -1, set = -1;
     int32_t alive = 0;
@@ -122,10 +77,7 @@ class GpuWorld {
     void clear(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w < bits.size()) bits[w] &= ~(1ull << (b & 63)); }
   };
 
-  int32_t add_shape(int32_t body, Shape shape, const Material& m, float radius);
-  void update_mass(int32_t body);
-  Aabb shape_local_aabb(const Shape& s, Quat q) const;
-  int32_t encode(const HBody& b) const { return b.set == 2 ? b.local : -(b.local + 2); }
+  int32_t add_shape(int32_t body, Shape shape, const Material& m, float radius) override;
 
   void upload_bodies();
   void pull_bodies();
@@ -153,14 +105,6 @@ class GpuWorld {
   int32_t counters_host_[8] = {};
   int32_t counters_base_[8] = {};
 
-  std::vector<HBody> bodies_;
-  std::vector<BodySim> awake_host_, static_host_;
-  std::vector<BodyState> states_host_;
-  std::vector<Shape> shapes_;
-  std::vector<HShape> hshapes_;
-  std::vector<BoxHull> hulls_;
-  std::vector<Mat3> hull_inertia_;
-  std::vector<float> hull_volume_;
   std::vector<HContact> hcontacts_;
   std::vector<int32_t> contact_free_;
   int32_t contact_pool_next_ = 0;

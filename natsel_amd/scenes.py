@@ -212,6 +212,12 @@ class XorShift32:
         return self.next_u32() % n
 
 
+def _ground(sc: Scene, half_x: float, half_z: float) -> None:
+    """Static ground box, top at y = 0, a metre wider than the footprint on each side."""
+    g = sc.add_body(BodySpec(STATIC, (0.0, -0.5, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (half_x + 1.0, 0.5, half_z + 1.0)))
+
+
 def _container(sc: Scene, half_x: float, half_z: float, height: float) -> None:
     """Static floor (top at y = 0) and four walls around a half_x x half_z footprint."""
     g = sc.add_body(BodySpec(STATIC, (0.0, -0.5, 0.0)))
@@ -226,7 +232,7 @@ def _container(sc: Scene, half_x: float, half_z: float, height: float) -> None:
 
 
 def pile(count: int = 10_000, seed: int = 0x5EED, size_x: int = 20, size_y: int = 60, size_z: int = 20,
-         base_y: float = 1.0) -> Scene:
+         base_y: float = 1.0, container: bool = True) -> Scene:
     """Random pile: `count` dynamic bodies, a third each boxes, spheres and capsules.
 
     Sizes s ~ U[0.25, 0.75] m are full extents: box edge lengths (per axis), sphere
@@ -234,10 +240,14 @@ def pile(count: int = 10_000, seed: int = 0x5EED, size_x: int = 20, size_y: int 
     1000, friction ~ U[0.4, 0.8]. Bodies take distinct cells of a 1 m lattice over a
     size_x x size_y x size_z region above the floor (seeded shuffle), jittered by at
     most (1 - s)/2 so no two start overlapping. The container is a static floor and
-    four walls. Continuous collision and sleeping are off.
+    four walls (or only the floor, `container=False`). Continuous collision and sleeping
+    are off.
     """
     sc = Scene(name=f"pile_{count}", enable_sleep=False, enable_continuous=False)
-    _container(sc, size_x / 2.0, size_z / 2.0, float(size_y) + base_y + 2.0)
+    if container:
+        _container(sc, size_x / 2.0, size_z / 2.0, float(size_y) + base_y + 2.0)
+    else:
+        _ground(sc, size_x / 2.0, size_z / 2.0)
     rng = XorShift32(seed)
     cells = size_x * size_y * size_z
     if count > cells:
@@ -272,3 +282,55 @@ def pile(count: int = 10_000, seed: int = 0x5EED, size_x: int = 20, size_y: int 
             q = float(f32(s) * f32(0.25))
             sc.add_shape(ShapeSpec(b, CAPSULE, (0.0, -q, 0.0, 0.0, q, 0.0, q), density=1000.0, friction=friction))
     return sc
+
+
+def stacks(count: int, seed: int, base_y: float = 0.5) -> Scene:
+    """Towers of up to 8 mixed bodies (boxes, spheres, capsules in turn; sizes as in
+    `pile`) on a 1.5 m grid, 1 m apart in height, on a ground box."""
+    sc = Scene(name=f"stacks_{count}", enable_sleep=False, enable_continuous=False)
+    towers = -(-count // 8)
+    side = max(1, int(np.ceil(np.sqrt(towers))))
+    half = 0.75 * side
+    _ground(sc, half, half)
+    rng = XorShift32(seed)
+    for k in range(count):
+        t, level = k // 8, k % 8
+        x = -half + 0.75 + 1.5 * (t % side)
+        z = -half + 0.75 + 1.5 * (t // side)
+        kind = (BOX, SPHERE, CAPSULE)[k % 3]
+        friction = float(rng.uniform(0.4, 0.8))
+        pos = (float(f32(x)), float(f32(base_y + level + 0.5)), float(f32(z)))
+        b = sc.add_body(BodySpec(DYNAMIC, pos, enable_sleep=False))
+        if kind == BOX:
+            h = tuple(float(rng.uniform(0.25, 0.75) * f32(0.5)) for _ in range(3))
+            sc.add_shape(ShapeSpec(b, BOX, h, density=1000.0, friction=friction))
+        elif kind == SPHERE:
+            sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, float(rng.uniform(0.25, 0.75) * f32(0.5))), density=1000.0,
+                                   friction=friction))
+        else:
+            q = float(rng.uniform(0.25, 0.75) * f32(0.25))
+            sc.add_shape(ShapeSpec(b, CAPSULE, (0.0, -q, 0.0, 0.0, q, 0.0, q), density=1000.0, friction=friction))
+    return sc
+
+
+def batch_body_count(rng: XorShift32, lo: int = 8, hi: int = 256) -> int:
+    """round(exp(U[ln lo, ln hi])): log-uniform body counts."""
+    u = float(rng.uniform(0.0, 1.0))
+    return int(round(float(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo))))))
+
+
+def batched(count: int = 4096, seed: int = 0xB47C4, lo: int = 8, hi: int = 256) -> list:
+    """Independent worlds with log-uniform body counts in [lo, hi]: even worlds are piles
+    (the random-pile generator on a ground box, no walls), odd worlds are stacks."""
+    out = []
+    for k in range(count):
+        rng = XorShift32((seed + 0x9E3779B9 * (k + 1)) & 0xFFFFFFFF)
+        n = batch_body_count(rng, lo, hi)
+        world_seed = rng.next_u32()
+        if k % 2 == 0:
+            side = max(2, int(np.ceil(np.sqrt(n / 3.0))))
+            height = -(-n // (side * side)) + 1
+            out.append(pile(n, seed=world_seed, size_x=side, size_y=height, size_z=side, base_y=0.5, container=False))
+        else:
+            out.append(stacks(n, seed=world_seed))
+    return out

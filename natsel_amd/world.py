@@ -84,6 +84,19 @@ def _material(d: ShapeDef) -> capi.Material:
     return capi.Material(d.density, d.friction, d.restitution, d.rolling_resistance)
 
 
+def c_options(o: WorldOptions) -> capi.Options:
+    """WorldOptions as the C ABI's nsl_options."""
+    c = capi.Options()
+    c.gravity[:] = o.gravity
+    c.contact_hertz, c.contact_damping_ratio, c.contact_speed = o.contact_hertz, o.contact_damping_ratio, o.contact_speed
+    c.max_linear_speed, c.restitution_threshold = o.max_linear_speed, o.restitution_threshold
+    c.restitution_iterations = o.restitution_iterations
+    c.enable_sleep, c.enable_continuous = int(o.enable_sleep), int(o.enable_continuous)
+    c.enable_warm_starting, c.enable_speculative = int(o.enable_warm_starting), int(o.enable_speculative)
+    c.solver_blocks = int(o.solver_blocks)
+    return c
+
+
 class World:
     def __init__(self, device: int = 0, options: WorldOptions | None = None):
         self._lib = capi.load_library()
@@ -93,16 +106,7 @@ class World:
         self._h = handle
 
     def _c_options(self) -> capi.Options:
-        o = self._opts
-        c = capi.Options()
-        c.gravity[:] = o.gravity
-        c.contact_hertz, c.contact_damping_ratio, c.contact_speed = o.contact_hertz, o.contact_damping_ratio, o.contact_speed
-        c.max_linear_speed, c.restitution_threshold = o.max_linear_speed, o.restitution_threshold
-        c.restitution_iterations = o.restitution_iterations
-        c.enable_sleep, c.enable_continuous = int(o.enable_sleep), int(o.enable_continuous)
-        c.enable_warm_starting, c.enable_speculative = int(o.enable_warm_starting), int(o.enable_speculative)
-        c.solver_blocks = int(o.solver_blocks)
-        return c
+        return c_options(self._opts)
 
     @property
     def options(self) -> WorldOptions:

@@ -52,6 +52,8 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int32)
         L.nslo_get_contact_colors.argtypes = [P, ip, ip, ip]
         L.nslo_set_threads.argtypes = [ctypes.c_int]
+        L.nslo_get_contact_shapes.argtypes = [P, ip, ip]
+        L.nslo_get_shape_bounds.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint8)]
         L.nslo_get_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.nslo_max_threads.restype = ctypes.c_int
         _LIB = L
@@ -131,6 +133,20 @@ class OracleWorld:
         out = (ctypes.c_double * 5)()
         self.L.nslo_get_phase_ms(self.h, out)
         return dict(zip(["pairs", "collide", "prepare", "solve", "finalize"], list(out)))
+
+    def shape_bounds(self, n_shapes: int):
+        fat = np.zeros((n_shapes, 6), np.float32)
+        moved = np.zeros(n_shapes, np.uint8)
+        self.L.nslo_get_shape_bounds(self.h, fat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                     moved.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return fat, moved
+
+    def contact_shapes(self):
+        n = self.L.nslo_contact_capacity(self.h)
+        a, b = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        ip = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        self.L.nslo_get_contact_shapes(self.h, ip(a), ip(b))
+        return a, b
 
     def contact_colors(self):
         n = self.L.nslo_contact_capacity(self.h)

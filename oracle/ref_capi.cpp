@@ -99,6 +99,27 @@ void nslo_get_phase_ms(void* wp, double* out) {
   out[0] = s.ms_pairs; out[1] = s.ms_collide; out[2] = s.ms_prepare; out[3] = s.ms_solve; out[4] = s.ms_finalize;
 }
 
+// Per shape id: fat AABB (lower xyz, upper xyz) and the moved flag.
+void nslo_get_shape_bounds(void* wp, float* fat, uint8_t* moved) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->shapes.size(); ++i) {
+    const AABB& b = w->fat_aabbs[i];
+    const float v[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+    for (int k = 0; k < 6; ++k) fat[6 * i + k] = v[k];
+    moved[i] = w->shapes[i].proxy_moved ? 1 : 0;
+  }
+}
+
+// Per contact id: its two shape ids (-1 when the id is free).
+void nslo_get_contact_shapes(void* wp, int32_t* a, int32_t* b) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->contacts.size(); ++i) {
+    const bool live = w->contacts[i].contact_id != kNullIndex;
+    a[i] = live ? w->contacts[i].shape_id_a : kNullIndex;
+    b[i] = live ? w->contacts[i].shape_id_b : kNullIndex;
+  }
+}
+
 // Graph coloring snapshot: per contact id, its color (-1 when not in the graph) and local index.
 int nslo_contact_capacity(void* wp) { return (int)static_cast<World*>(wp)->contacts.size(); }
 void nslo_get_contact_colors(void* wp, int32_t* color, int32_t* local, int32_t* alive) {

@@ -53,3 +53,14 @@ def test_cos_sin_matches_closed_form():
         assert abs(float(c) - np.cos(float(t))) < 2e-3 and abs(float(s) - np.sin(float(t))) < 2e-3
         assert abs(float(c) * float(c) + float(s) * float(s) - 1.0) < 1e-6
     assert scenes.compute_cos_sin(np.float32(0.0)) == (1.0, 0.0)
+
+
+def test_batched_scene_generator_is_seeded():
+    """The batched-world generator: log-uniform counts within bounds, deterministic."""
+    a = scenes.batched(count=64, seed=5)
+    b = scenes.batched(count=64, seed=5)
+    counts = [s.dynamic_count for s in a]
+    assert counts == [s.dynamic_count for s in b]
+    assert min(counts) >= 8 and max(counts) <= 256
+    assert [s.bodies[1].position for s in a] == [s.bodies[1].position for s in b]
+    assert sum(1 for s in a if s.name.startswith("pile")) == 32
