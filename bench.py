@@ -58,6 +58,12 @@ def stage_bytes(stage: str, ks: dict, substeps: int) -> float:
     return reps * (m["per_constraint"] * ks["constraints"] + m["per_point"] * ks["points"])
 
 
+def step_bytes(counts: dict, substeps: int) -> float:
+    """Algorithmic bytes of whole steps (collide + prepare .. finalize) for the given work
+    counts (bodies, collided, constraints, points summed over the steps)."""
+    return sum(stage_bytes(k, counts, substeps) for k in BYTE_MODEL)
+
+
 def solve_bytes(ks: dict, substeps: int) -> float:
     """Algorithmic bytes of the step kernel (prepare .. finalize) over the measured steps."""
     return sum(stage_bytes(k, ks, substeps) for k in BYTE_MODEL if k != "collide")
@@ -114,6 +120,47 @@ def cpu_baseline(scene, seconds: float, max_steps: int) -> dict:
     }
 
 
+def batched_measurement(worlds: int, steps: int) -> dict:
+    """Batched worlds (the log-uniform 8..256-body piles and stacks of
+    natsel_amd.scenes.batched): every world stepped together, `steps` steps in one call
+    after one untimed step; device time from HIP events around the call."""
+    from natsel_amd import scenes
+    from natsel_amd.batch import load_batch
+
+    scene_list = scenes.batched(worlds)
+    dt, sub = scene_list[0].dt, scene_list[0].substeps
+    b = load_batch(scene_list)
+    b.step(dt, sub, 1)  # uploads the worlds
+    s0 = b.stats()
+    b.step(dt, sub, steps)
+    s1 = b.stats()
+    b.close()
+    seconds = s1["last_step_ms"] / 1e3
+    bodies = s1["bodies"]
+    counts = {"bodies": bodies * steps, "steps": steps, "collided": s1["contacts_collided"] - s0["contacts_collided"],
+              "constraints": s1["constraints"] - s0["constraints"], "points": s1["points"] - s0["points"]}
+    total = step_bytes(counts, sub)
+    achieved = total / seconds / 1e9
+    return {
+        "metric": "world-steps/s (batched worlds)",
+        "value": worlds * steps / seconds,
+        "unit": "world-steps/s",
+        "body_substeps_per_s": bodies * sub * steps / seconds,
+        "ms_per_step": 1e3 * seconds / steps,
+        "steps": steps,
+        "config": {"workload": f"{worlds} independent worlds, {bodies} dynamic bodies (8..256 per world, log-uniform; "
+                               "piles and stacks of boxes/spheres/capsules on ground boxes), dt 1/60, 4 substeps",
+                   "worlds": worlds, "bodies": bodies},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_batch_pairs + k_batch_collide + k_batch_solve (one step)",
+                     "algorithmic_bytes_per_step": total / steps},
+        "contacts_collided_per_step": counts["collided"] / steps,
+        "constraints_per_step": counts["constraints"] / steps,
+        "points_per_step": counts["points"] / steps,
+        "error_worlds": s1["error_worlds"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +170,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups of the step kernel (0: sized from the largest phase)")
+    ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched measurement (0: skip it)")
+    ap.add_argument("--batched-steps", type=int, default=240)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (reference kernel for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
@@ -220,6 +269,8 @@ def main() -> int:
         cb["gpu_value_same_steps"] = n * scene.substeps * k / (marks[k - 1] - t0)
         cb["gpu_over_cpu_same_steps"] = cb["gpu_value_same_steps"] / cb["value"]
         out["cpu_baseline"] = cb
+    if args.batched_worlds > 0:
+        out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps)
     print(json.dumps(out))
     return 0
 

@@ -153,6 +153,10 @@ typedef struct nsl_batch_stats {
   int64_t contact_capacity; /* contact slots over all worlds */
   int32_t error_worlds;     /* worlds stopped by a full capacity */
   int32_t first_error;      /* 1: contacts full, 2: overflow colour full, 3: new pairs of a step over 2048 */
+  int64_t contacts_collided;    /* since creation, over all worlds and steps */
+  int64_t constraints;          /* coloured contact constraints prepared */
+  int64_t points;               /* their manifold points */
+  int64_t overflow_constraints; /* overflow-colour constraints prepared */
 } nsl_batch_stats;
 
 int nsl_batch_create(int device, const nsl_options* options, nsl_batch** out);

@@ -86,6 +86,10 @@ class BatchStats(ctypes.Structure):
         ("contact_capacity", ctypes.c_int64),
         ("error_worlds", ctypes.c_int32),
         ("first_error", ctypes.c_int32),
+        ("contacts_collided", ctypes.c_int64),
+        ("constraints", ctypes.c_int64),
+        ("points", ctypes.c_int64),
+        ("overflow_constraints", ctypes.c_int64),
     ]
 
 

@@ -95,6 +95,7 @@ struct BatchDevice {
   BatchWorldState* wstate;
   int32_t* order;     // worlds in launch order (most bodies first)
   int32_t* counters;  // as DeviceWorld::counters
+  unsigned long long* work;  // [0] contacts collided, [1] constraints prepared, [2] their points, [3] overflow constraints
   int32_t n_worlds;
   int32_t enable_speculative;
   int32_t restitution_iterations;
@@ -137,6 +138,9 @@ class BatchStepper {
   int64_t total_body_steps() const { return body_steps_; }
   int64_t total_bodies() const { return total_awake_; }
   int64_t contact_capacity() const { return total_contact_cap_; }
+  // Work since creation: contacts collided, constraints prepared, their manifold points,
+  // overflow constraints.
+  void work_counts(int64_t out[4]);
 
  private:
   void upload();

@@ -368,9 +368,11 @@ __global__ __launch_bounds__(kBatchCollideBlock) void k_batch_collide(BatchDevic
   if (r.ws->error != kBatchOk) return;
   const DeviceWorld dg = world_view(b, wd, b.states + wd.awake_off);
   const int next_id = r.ws->next_id;
+  int collided = 0;
   for (int id = threadIdx.x; id < next_id; id += blockDim.x) {
     uint8_t ch = kChangeNone;
     if (r.meta[id].alive) {
+      ++collided;
       ch = collide_contact(dg, id);
       if (ch == kChangeDisjoint) {
         const int a = r.meta[id].shape_a, c = r.meta[id].shape_b;
@@ -379,6 +381,7 @@ __global__ __launch_bounds__(kBatchCollideBlock) void k_batch_collide(BatchDevic
     }
     r.change[id] = ch;
   }
+  if (collided) atomicAdd(&b.work[0], (unsigned long long)collided);
 }
 
 // Contact transitions in id order (one lane), the solve order, then prepare, the
@@ -456,8 +459,17 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
   const bool any_restitution = s_any != 0;
 
   // ---- prepare, substeps, restitution, store, finalize (solver_stages.rs:543-589)
-  for (int slot = lane; slot < n_constraints; slot += kBatchLanes) prepare_item(dl, slot, p);
-  for (int k = lane; k < n_ovf; k += kBatchLanes) prepare_overflow_item(dl, k, p);
+  {
+    int points = 0;
+    for (int slot = lane; slot < n_constraints; slot += kBatchLanes) points += prepare_item(dl, slot, p);
+    for (int k = lane; k < n_ovf; k += kBatchLanes) prepare_overflow_item(dl, k, p);
+    for (int off = 32; off > 0; off >>= 1) points += __shfl_xor(points, off, 64);
+    if (lane == 0) {
+      atomicAdd(&b.work[1], (unsigned long long)n_constraints);
+      atomicAdd(&b.work[2], (unsigned long long)points);
+      if (n_ovf) atomicAdd(&b.work[3], (unsigned long long)n_ovf);
+    }
+  }
   wave_sync();
   for (int sub = 0; sub < p.sub_steps; ++sub) {
     for (int i = lane; i < wd.n_awake; i += kBatchLanes) integrate_velocities_item(dl, i, p);
@@ -667,6 +679,7 @@ void BatchStepper::upload() {
   b.wstate = alloc<BatchWorldState>(nw, true);
   b.order = upload_vec(stream_, alloc<int32_t>(nw, false), order);
   b.counters = alloc<int32_t>(8, true);
+  b.work = alloc<unsigned long long>(4, true);
   BATCH_CHECK(hipStreamSynchronize(stream_));  // the host vectors are read by the copies
   total_awake_ = (int64_t)sims.size();
   total_contact_cap_ = contact_total;
@@ -731,6 +744,12 @@ void BatchStepper::get_bodies(int32_t w, double* pos, float* rot, float* v, floa
   if (w < 0 || w >= (int32_t)scenes_.size()) throw std::invalid_argument("bad world index");
   pull();
   scenes_[w]->get_bodies(pos, rot, v, wv);
+}
+
+void BatchStepper::work_counts(int64_t out[4]) {
+  unsigned long long w[4] = {0, 0, 0, 0};
+  if (uploaded_) BATCH_CHECK(hipMemcpy(w, d_.work, sizeof(w), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 4; ++i) out[i] = (int64_t)w[i];
 }
 
 int32_t BatchStepper::contact_slots(int32_t w) const {

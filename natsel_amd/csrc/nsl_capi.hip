@@ -385,6 +385,12 @@ int nsl_batch_get_stats(nsl_batch* batch, nsl_batch_stats* out) {
     out->bodies = batch->impl->total_bodies();
     out->contact_capacity = batch->impl->contact_capacity();
     out->error_worlds = batch->impl->error_worlds(&out->first_error);
+    int64_t w[4];
+    batch->impl->work_counts(w);
+    out->contacts_collided = w[0];
+    out->constraints = w[1];
+    out->points = w[2];
+    out->overflow_constraints = w[3];
   });
 }
 
