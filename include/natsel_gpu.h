@@ -61,6 +61,7 @@ typedef struct nsl_options {
   int32_t enable_warm_starting;      /* 1 */
   int32_t enable_speculative;        /* 1 */
   int32_t solver_blocks;             /* workgroups of the step kernel; 0 = sized from the largest phase */
+  int32_t enable_restitution_propagation; /* 0 (world_runtime.rs:1055) */
 } nsl_options;
 
 typedef struct nsl_body_def {

@@ -147,10 +147,15 @@ def load_batch(scene_list: list[Scene], device: int = 0) -> Batch:
     must agree on sleep, continuous collision, time step and substeps."""
     first = scene_list[0]
     for s in scene_list:
-        if (s.enable_sleep, s.enable_continuous, s.dt, s.substeps) != (first.enable_sleep, first.enable_continuous, first.dt,
-                                                                     first.substeps):
-            raise ValueError(f"scene {s.name} differs from {first.name} in sleep, continuous collision, dt or substeps")
-    b = Batch(device, WorldOptions(enable_sleep=first.enable_sleep, enable_continuous=first.enable_continuous))
+        key = lambda x: (x.enable_sleep, x.enable_continuous, x.dt, x.substeps, tuple(x.gravity), x.restitution_iterations,  # noqa: E731
+                         x.restitution_threshold, x.restitution_propagation)
+        if key(s) != key(first):
+            raise ValueError(f"scene {s.name} differs from {first.name} in its world options (sleep, continuous collision, "
+                             "dt, substeps, gravity or restitution settings)")
+    b = Batch(device, WorldOptions(enable_sleep=first.enable_sleep, enable_continuous=first.enable_continuous,
+                                   gravity=tuple(first.gravity), restitution_iterations=first.restitution_iterations,
+                                   restitution_threshold=first.restitution_threshold,
+                                   enable_restitution_propagation=first.restitution_propagation))
     for scene in scene_list:
         w = b.add_world()
         ids = [w.create_body(BodyDef(s.body_type, s.position, s.rotation, s.linear_velocity, s.angular_velocity, s.linear_damping,

@@ -26,6 +26,7 @@ class Options(ctypes.Structure):
         ("enable_warm_starting", ctypes.c_int32),
         ("enable_speculative", ctypes.c_int32),
         ("solver_blocks", ctypes.c_int32),
+        ("enable_restitution_propagation", ctypes.c_int32),
     ]
 
 

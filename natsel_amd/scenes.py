@@ -55,6 +55,10 @@ class Scene:
     enable_continuous: bool = True
     dt: float = 1.0 / 60.0
     substeps: int = 4
+    gravity: tuple = (0.0, -10.0, 0.0)
+    restitution_iterations: int = 2
+    restitution_threshold: float = 1.0
+    restitution_propagation: bool = False
 
     def add_body(self, spec: BodySpec) -> int:
         self.bodies.append(spec)
@@ -334,3 +338,101 @@ def batched(count: int = 4096, seed: int = 0xB47C4, lo: int = 8, hi: int = 256) 
         else:
             out.append(stacks(n, seed=world_seed))
     return out
+
+
+
+# Restitution scenes: src/tests/simulation/restitution_phase_tests.rs. The ground is a
+# 80 x 2 x 80 box whose top face is y = 0; balls are spheres of radius 0.5.
+IMPACT_SPEED = f32(5.0)  # restitution_phase_tests.rs:22
+RESTITUTION_TIME_STEP = f32(1.0 / 60.0)
+
+
+def _bounce_ground(sc: Scene, restitution: float) -> None:
+    """make_ground (restitution_phase_tests.rs:35-44)."""
+    g = sc.add_body(BodySpec(STATIC, (0.0, -1.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (40.0, 1.0, 40.0), friction=0.0, restitution=restitution))
+
+
+def _ball(sc: Scene, x: float, y, vy: float, restitution: float) -> int:
+    """make_ball (restitution_phase_tests.rs:46-58): f32 position widened to f64."""
+    b = sc.add_body(BodySpec(DYNAMIC, (float(f32(x)), float(f32(y)), 0.0), linear_velocity=(0.0, float(f32(vy)), 0.0)))
+    sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, 0.5), friction=0.0, restitution=restitution))
+    return b
+
+
+def ground_bounce(restitution: float, gap) -> Scene:
+    """A ball driven at 5 m/s into the ground from `gap` above it, gravity off
+    (measure_ground_bounce, restitution_phase_tests.rs:62-72)."""
+    sc = Scene(name=f"ground_bounce_{restitution}", enable_sleep=False, gravity=(0.0, 0.0, 0.0))
+    _bounce_ground(sc, restitution)
+    _ball(sc, 0.0, f32(0.5) + f32(gap), -IMPACT_SPEED, restitution)
+    return sc
+
+
+def supported_bounce(restitution: float, support_count: int, propagate: bool) -> Scene:
+    """A ball driven onto a column of resting balls on the ground, four restitution
+    iterations (measure_supported_bounce, restitution_phase_tests.rs:77-96). The impactor is
+    the last body."""
+    sc = Scene(name=f"supported_{restitution}_{support_count}", enable_sleep=False, gravity=(0.0, 0.0, 0.0),
+               restitution_iterations=4, restitution_propagation=propagate)
+    _bounce_ground(sc, 0.0)
+    for i in range(support_count):
+        _ball(sc, 0.0, f32(0.5) + f32(i), 0.0, 0.0)
+    y = f32(0.5) + f32(support_count) + f32(0.5) * IMPACT_SPEED * RESTITUTION_TIME_STEP
+    _ball(sc, 0.0, y, -IMPACT_SPEED, restitution)
+    return sc
+
+
+def threshold_bounce(threshold: float, speed: float) -> Scene:
+    """A ball resting on a bouncy ground at approach speed `speed` under a restitution
+    threshold (threshold / threshold_above, restitution_phase_tests.rs:160-193)."""
+    sc = Scene(name=f"threshold_{threshold}", enable_sleep=False, gravity=(0.0, 0.0, 0.0), restitution_threshold=threshold)
+    _bounce_ground(sc, 0.9)
+    _ball(sc, 0.0, 0.5, -speed, 0.9)
+    return sc
+
+
+def head_on(restitution: float, density_b: float) -> Scene:
+    """Two free unit spheres closing head on at 2.5 m/s each, gravity off; the second has
+    density `density_b` (measure_head_on, restitution_phase_tests.rs:202-238)."""
+    sc = Scene(name=f"head_on_{restitution}_{density_b}", enable_sleep=False, gravity=(0.0, 0.0, 0.0))
+    half = float(f32(0.5) * IMPACT_SPEED)
+    for x, vx, rho in ((-1.0, half, 1.0), (1.0, -half, density_b)):
+        b = sc.add_body(BodySpec(DYNAMIC, (x, 0.0, 0.0), linear_velocity=(vx, 0.0, 0.0)))
+        sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, 0.5), density=rho, friction=0.0, restitution=restitution))
+    return sc
+
+
+def drop_ball(drop_height: float) -> Scene:
+    """A perfectly elastic ball dropped from rest under gravity onto a dead ground
+    (measure_apexes, restitution_phase_tests.rs:937-958). The ball is the last body."""
+    sc = Scene(name=f"drop_{drop_height}", enable_sleep=False)
+    _bounce_ground(sc, 0.0)
+    _ball(sc, 0.0, drop_height, 0.0, 1.0)
+    return sc
+
+
+def resting_stack(box_count: int = 10) -> Scene:
+    """Ten unit boxes stacked on a bouncy ground, all with restitution 0.9, gravity on
+    (resting_stack, restitution_phase_tests.rs:1001-1049)."""
+    sc = Scene(name="resting_stack", enable_sleep=False)
+    _bounce_ground(sc, 0.9)
+    for i in range(box_count):
+        b = sc.add_body(BodySpec(DYNAMIC, (0.0, 0.5 + i, 0.0)))
+        sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), density=1.0, friction=0.6, restitution=0.9))
+    return sc
+
+
+def flat_landing(restitution: float, sub_steps: int = 4, spin: float = 0.0) -> Scene:
+    """A flat 2 x 0.5 x 2 box driven at 5 m/s into frictionless ground with gravity off and
+    four restitution iterations, optionally spinning about x (measure_flat_bounce and
+    measure_spinning_bounce, restitution_phase_tests.rs:824-855, 889-918). The box is the
+    last body."""
+    sc = Scene(name=f"flat_landing_{restitution}_{spin}", enable_sleep=False, gravity=(0.0, 0.0, 0.0),
+               restitution_iterations=4, substeps=sub_steps)
+    _bounce_ground(sc, 0.0)
+    y = f32(0.25) + f32(0.5) * IMPACT_SPEED * RESTITUTION_TIME_STEP
+    b = sc.add_body(BodySpec(DYNAMIC, (0.0, float(y), 0.0), linear_velocity=(0.0, -float(IMPACT_SPEED), 0.0),
+                             angular_velocity=(float(f32(spin)), 0.0, 0.0)))
+    sc.add_shape(ShapeSpec(b, BOX, (1.0, 0.25, 1.0), density=1.0, friction=0.0, restitution=restitution))
+    return sc

@@ -78,6 +78,7 @@ class WorldOptions:  # world_runtime.rs:1243-1304
     enable_warm_starting: bool = True
     enable_speculative: bool = True
     solver_blocks: int = 0  # workgroups of the step kernel; 0 = sized from the largest phase
+    enable_restitution_propagation: bool = False
 
 
 def _material(d: ShapeDef) -> capi.Material:
@@ -94,6 +95,7 @@ def c_options(o: WorldOptions) -> capi.Options:
     c.enable_sleep, c.enable_continuous = int(o.enable_sleep), int(o.enable_continuous)
     c.enable_warm_starting, c.enable_speculative = int(o.enable_warm_starting), int(o.enable_speculative)
     c.solver_blocks = int(o.solver_blocks)
+    c.enable_restitution_propagation = int(o.enable_restitution_propagation)
     return c
 
 
@@ -221,7 +223,10 @@ class World:
 
 def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0) -> World:
     """Build a World from a seeded scene description (natsel_amd.scenes)."""
-    opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous, solver_blocks=solver_blocks)
+    opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous, solver_blocks=solver_blocks,
+                        gravity=tuple(scene.gravity), restitution_iterations=scene.restitution_iterations,
+                        restitution_threshold=scene.restitution_threshold,
+                        enable_restitution_propagation=scene.restitution_propagation)
     w = World(device, opts)
     ids = []
     for b in scene.bodies:

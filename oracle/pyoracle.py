@@ -37,6 +37,7 @@ def lib():
         L.nslo_destroy.argtypes = [P]
         L.nslo_set_options.argtypes = [P, I, I, I, I]
         L.nslo_set_gravity.argtypes = [P, F, F, F]
+        L.nslo_set_restitution.argtypes = [P, I, F, I]
         L.nslo_create_body.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), fp, fp, fp, I, F, F, F]
         L.nslo_create_box.argtypes = [P, I, F, F, F, F, F, F]
         L.nslo_create_sphere.argtypes = [P, I, fp, F, F, F, F]
@@ -159,6 +160,9 @@ class OracleWorld:
 def load_scene(scene) -> OracleWorld:
     """Build an OracleWorld from a natsel_amd.scenes.Scene."""
     w = OracleWorld(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous)
+    w.L.nslo_set_gravity(w.h, *(float(v) for v in scene.gravity))
+    w.L.nslo_set_restitution(w.h, int(scene.restitution_iterations), float(scene.restitution_threshold),
+                             int(scene.restitution_propagation))
     ids = [w.create_body(b.body_type, b.position, b.rotation, b.linear_velocity, b.angular_velocity, b.enable_sleep,
                          b.linear_damping, b.angular_damping, b.gravity_scale) for b in scene.bodies]
     for s in scene.shapes:

@@ -27,6 +27,12 @@ void nslo_set_options(void* wp, int enable_sleep, int enable_continuous, int ena
 }
 
 void nslo_set_gravity(void* wp, float x, float y, float z) { static_cast<World*>(wp)->gravity = Vec3{x, y, z}; }
+void nslo_set_restitution(void* wp, int iterations, float threshold, int propagate) {
+  World* w = static_cast<World*>(wp);
+  w->restitution_iterations = iterations;
+  w->restitution_threshold = threshold;
+  w->enable_restitution_propagation = propagate != 0;
+}
 
 int nslo_create_body(void* wp, int type, const double* pos, const float* rot, const float* v, const float* av,
                      int enable_sleep, float linear_damping, float angular_damping, float gravity_scale) {

@@ -36,13 +36,26 @@ def _first_difference(g, o):
 
 
 def test_pyramid20_every_step_matches_oracle(gpu_available):
+    """The 210-box pyramid over 600 steps: the GPU's hash equals the oracle's after every
+    step, and the GPU's own top box settles and stays put (the resting-stack pattern of
+    restitution_phase_tests.rs:1036-1049: drift under 1 cm, speeds under 0.05 m/s once
+    settled)."""
     sc = scenes.pyramid(20, density=1000.0)
     g, o = _pair(sc)
-    for k in range(120):
+    top = len(sc.bodies) - 1
+    heights, speeds = [], []
+    for k in range(600):
         g.step(1.0 / 60.0, 4)
         o.step(1.0 / 60.0, 4)
         hg, ho = g.world_state_hash(), o.world_state_hash()
         assert hg == ho, f"step {k}: {hg:x} != {ho:x}; {_first_difference(g, o)}"
+        pos, _, v, _ = g.body_arrays()
+        heights.append(pos[top, 1])
+        speeds.append(float(np.linalg.norm(v[1:].astype(np.float64), axis=1).max()))
+    settled = heights[239]
+    assert max(abs(h - settled) for h in heights[240:]) < 0.01
+    assert max(speeds[240:]) < 0.05
+    assert abs(settled - sc.bodies[top].position[1]) < 0.03  # 20 soft contact layers compress 2.76 cm
 
 
 def test_pyramid20_colors_match_oracle(gpu_available):

@@ -64,3 +64,24 @@ def test_batched_scene_generator_is_seeded():
     assert min(counts) >= 8 and max(counts) <= 256
     assert [s.bodies[1].position for s in a] == [s.bodies[1].position for s in b]
     assert sum(1 for s in a if s.name.startswith("pile")) == 32
+
+
+def test_pyramid_settles_and_rests():
+    """The 210-box pyramid (scenes.rs:105-139, base 20, density 1000) settles within 60
+    steps and then rests: over steps 240-600 the top box moves less than 1 cm and no body
+    exceeds 0.05 m/s (the resting-stack pattern of restitution_phase_tests.rs:1036-1049).
+    The 20 soft contact layers compress the stack by 2.76 cm while it settles."""
+    sc = scenes.pyramid(20, density=1000.0)
+    w = _oracle(sc)
+    top = len(sc.bodies) - 1
+    assert sc.bodies[top].position[1] == 19.5
+    heights, speeds = [], []
+    for _ in range(600):
+        w.step(1.0 / 60.0, 4)
+        pos, _, v, _ = w.body_arrays()
+        heights.append(pos[top, 1])
+        speeds.append(float(np.linalg.norm(v[1:].astype(np.float64), axis=1).max()))
+    settled = heights[239]
+    assert max(abs(h - settled) for h in heights[240:]) < 0.01
+    assert max(speeds[240:]) < 0.05
+    assert 0.02 < 19.5 - settled < 0.03
