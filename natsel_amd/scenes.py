@@ -47,10 +47,34 @@ class ShapeSpec:
 
 
 @dataclass
+class JointSpec:
+    """A revolute joint (RevoluteJointDef, types/joint_defs.rs:30-58, :154-167). Frames are
+    (px, py, pz, qx, qy, qz, qs) in the body's frame; the hinge axis is frame A's z axis."""
+    body_a: int
+    body_b: int
+    frame_a: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+    frame_b: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+    constraint_hertz: float = 60.0
+    constraint_damping_ratio: float = 2.0
+    collide_connected: bool = False
+    enable_spring: bool = False
+    hertz: float = 0.0
+    damping_ratio: float = 0.0
+    target_angle: float = 0.0
+    enable_limit: bool = False
+    lower_angle: float = 0.0
+    upper_angle: float = 0.0
+    enable_motor: bool = False
+    max_motor_torque: float = 0.0
+    motor_speed: float = 0.0
+
+
+@dataclass
 class Scene:
     name: str
     bodies: list = field(default_factory=list)
     shapes: list = field(default_factory=list)
+    joints: list = field(default_factory=list)
     enable_sleep: bool = True
     enable_continuous: bool = True
     dt: float = 1.0 / 60.0
@@ -67,6 +91,10 @@ class Scene:
     def add_shape(self, spec: ShapeSpec) -> int:
         self.shapes.append(spec)
         return len(self.shapes) - 1
+
+    def add_joint(self, spec: JointSpec) -> int:
+        self.joints.append(spec)
+        return len(self.joints) - 1
 
     @property
     def dynamic_count(self) -> int:
@@ -236,7 +264,7 @@ def _container(sc: Scene, half_x: float, half_z: float, height: float) -> None:
 
 
 def pile(count: int = 10_000, seed: int = 0x5EED, size_x: int = 20, size_y: int = 60, size_z: int = 20,
-         base_y: float = 1.0, container: bool = True) -> Scene:
+         base_y: float = 1.0, container: bool = True, wall_height: float | None = None) -> Scene:
     """Random pile: `count` dynamic bodies, a third each boxes, spheres and capsules.
 
     Sizes s ~ U[0.25, 0.75] m are full extents: box edge lengths (per axis), sphere
@@ -249,7 +277,7 @@ def pile(count: int = 10_000, seed: int = 0x5EED, size_x: int = 20, size_y: int 
     """
     sc = Scene(name=f"pile_{count}", enable_sleep=False, enable_continuous=False)
     if container:
-        _container(sc, size_x / 2.0, size_z / 2.0, float(size_y) + base_y + 2.0)
+        _container(sc, size_x / 2.0, size_z / 2.0, wall_height or float(size_y) + base_y + 2.0)
     else:
         _ground(sc, size_x / 2.0, size_z / 2.0)
     rng = XorShift32(seed)
@@ -435,4 +463,132 @@ def flat_landing(restitution: float, sub_steps: int = 4, spin: float = 0.0) -> S
     b = sc.add_body(BodySpec(DYNAMIC, (0.0, float(y), 0.0), linear_velocity=(0.0, -float(IMPACT_SPEED), 0.0),
                              angular_velocity=(float(f32(spin)), 0.0, 0.0)))
     sc.add_shape(ShapeSpec(b, BOX, (1.0, 0.25, 1.0), density=1.0, friction=0.0, restitution=restitution))
+    return sc
+
+
+# Revolute-joint scenes. The reference's own joint tests (src/tests/revolute_joint_tests.rs,
+# src/tests/simulation/joint_inspection_tests.rs) check replay and contact filtering; the
+# other scenes below are built so a closed form fixes what the joint must do.
+IDENTITY_FRAME = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+
+
+def _frame(p, q=(0.0, 0.0, 0.0, 1.0)) -> tuple:
+    return tuple(float(f32(v)) for v in tuple(p) + tuple(q))
+
+
+def hinged_pendulum(limit=None, spring=None, motor=None, gravity=(0.0, -10.0, 0.0), spin=(0.0, 0.0, 0.0)) -> Scene:
+    """A 0.4 m box (density 1000) hinged 0.5 m from its centre to a static body at
+    (0, 3, 0), starting level (hinged_pendulum, revolute_joint_tests.rs:103-131). The hinge
+    axis is z. `limit` = (lower, upper), `spring` = (hertz, damping ratio, target angle),
+    `motor` = (speed, max torque); `spin` is the box's initial angular velocity."""
+    sc = Scene(name="hinged_pendulum", enable_sleep=False, gravity=tuple(gravity))
+    a = sc.add_body(BodySpec(STATIC, (0.0, 3.0, 0.0)))
+    b = sc.add_body(BodySpec(DYNAMIC, (1.5, 3.0, 0.0), angular_velocity=tuple(float(f32(v)) for v in spin)))
+    sc.add_shape(ShapeSpec(b, BOX, (0.2, 0.2, 0.2), density=1000.0))
+    j = JointSpec(a, b, frame_a=_frame((1.0, 0.0, 0.0)), frame_b=_frame((-0.5, 0.0, 0.0)))
+    if limit is not None:
+        j.enable_limit, j.lower_angle, j.upper_angle = True, float(f32(limit[0])), float(f32(limit[1]))
+    if spring is not None:
+        j.enable_spring, j.hertz, j.damping_ratio, j.target_angle = True, *(float(f32(v)) for v in spring)
+    if motor is not None:
+        j.enable_motor, j.motor_speed, j.max_motor_torque = True, float(f32(motor[0])), float(f32(motor[1]))
+    sc.add_joint(j)
+    return sc
+
+
+def touching_boxes(collide_connected: bool = False) -> Scene:
+    """Two overlapping unit boxes, gravity off (create_joint_clears_contacts /
+    create_joint_keeps_contacts, joint_inspection_tests.rs:460-567). The joint between
+    them is added by the test after the first step."""
+    sc = Scene(name="touching_boxes", gravity=(0.0, 0.0, 0.0))
+    for x in (0.0, 0.5):
+        b = sc.add_body(BodySpec(DYNAMIC, (x, 0.0, 0.0)))
+        sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), density=1.0))
+    return sc
+
+
+# A chain link: capsule along x, segment 0.4 m, radius 0.1 m; neighbours are hinged 0.5 m
+# apart at the link ends, so adjacent capsules overlap by 0.1 m and only the joint's
+# collide_connected = false keeps them from colliding.
+LINK_HALF = 0.2
+LINK_RADIUS = 0.1
+LINK_PITCH = 0.5
+# alternate hinges turn about z and about y: frame rotated 90 degrees about x
+_QX90 = (float(f32(np.sin(np.pi / 4))), 0.0, 0.0, float(f32(np.cos(np.pi / 4))))
+
+
+def add_chain(sc: Scene, start: tuple, links: int, anchor: int | None = None, alternate_axes: bool = True,
+              density: float = 1000.0, friction: float = 0.6) -> list:
+    """Append a chain of `links` capsule links along +x from `start` (the first link's
+    centre), joined by revolute joints at the link ends (collide_connected false). With
+    `anchor` the first link is also hinged to that body at the chain's start point."""
+    ids = []
+    x0, y0, z0 = start
+    for i in range(links):
+        b = sc.add_body(BodySpec(DYNAMIC, (float(f32(x0 + LINK_PITCH * i)), float(f32(y0)), float(f32(z0))), enable_sleep=False))
+        sc.add_shape(ShapeSpec(b, CAPSULE, (-LINK_HALF, 0.0, 0.0, LINK_HALF, 0.0, 0.0, LINK_RADIUS), density=density,
+                               friction=friction))
+        ids.append(b)
+    half = 0.5 * LINK_PITCH
+    for i in range(links - 1):
+        q = _QX90 if (alternate_axes and i % 2 == 1) else (0.0, 0.0, 0.0, 1.0)
+        sc.add_joint(JointSpec(ids[i], ids[i + 1], frame_a=_frame((half, 0.0, 0.0), q), frame_b=_frame((-half, 0.0, 0.0), q)))
+    if anchor is not None:
+        ax = sc.bodies[anchor].position
+        p = (float(f32(x0 - half - ax[0])), float(f32(y0 - ax[1])), float(f32(z0 - ax[2])))
+        sc.add_joint(JointSpec(anchor, ids[0], frame_a=_frame(p), frame_b=_frame((-half, 0.0, 0.0))))
+    return ids
+
+
+def hanging_chain(links: int = 8) -> Scene:
+    """A chain of capsule links hinged (about z) to a static body, starting level, falling
+    under gravity onto nothing: it swings down and hangs."""
+    sc = Scene(name=f"hanging_chain_{links}", enable_sleep=False)
+    anchor = sc.add_body(BodySpec(STATIC, (0.0, 10.0, 0.0)))
+    add_chain(sc, (0.5 * LINK_PITCH, 10.0, 0.0), links, anchor=anchor, alternate_axes=False)
+    return sc
+
+
+def chain_pile(count: int = 400, chains: int = 40, links: int = 16, seed: int = 0xC4A1, size_x: int = 12,
+               size_y: int = 12, size_z: int = 12) -> Scene:
+    """A mixed pile (`pile`'s generator) with chains of capsule links laid in rows above
+    it, all falling into the container together."""
+    per_row, rows = _chain_rows(chains, links, size_x)
+    layers = -(-rows // max(1, size_z - 1))
+    y0 = 1.0 + size_y + 1.0
+    sc = pile(count, seed, size_x, size_y, size_z, wall_height=y0 + layers + 2.0)
+    sc.name = f"chain_pile_{count}_{chains}x{links}"
+    _add_chain_rows(sc, chains, links, size_x, size_z, y0)
+    return sc
+
+
+def _chain_rows(chains: int, links: int, size_x: int) -> tuple:
+    """Chains per row along x, and rows."""
+    per_row = max(1, int(size_x // (LINK_PITCH * links + 0.5)))
+    return per_row, -(-chains // per_row)
+
+
+def _add_chain_rows(sc: Scene, chains: int, links: int, size_x: int, size_z: int, y0: float) -> None:
+    """Chains along x, `per_row` to a row, rows 1 m apart in z, layers 1 m apart in y."""
+    per_row, rows = _chain_rows(chains, links, size_x)
+    rows_per_layer = max(1, size_z - 1)
+    length = LINK_PITCH * links
+    k = 0
+    for r in range(rows):
+        layer, rz = divmod(r, rows_per_layer)
+        for c in range(per_row):
+            if k == chains:
+                return
+            x = -size_x / 2.0 + 0.5 + c * (length + 0.5) + 0.5 * LINK_PITCH
+            z = -size_z / 2.0 + 1.0 + rz
+            add_chain(sc, (x, y0 + layer, z), links)
+            k += 1
+
+
+def large_world(bodies: int = 100_000, chains: int = 2000, links: int = 16, seed: int = 0x5EED5) -> Scene:
+    """100k mixed bodies (the pile generator over a 50 x 60 x 50 m region) plus 2,000
+    chains of 16 capsule links (15 revolute joints each, collide_connected false) in layers
+    above the pile; sleep and continuous collision off."""
+    sc = chain_pile(bodies, chains, links, seed, 50, 60, 50)
+    sc.name = f"large_world_{bodies}_{chains}x{links}"
     return sc

@@ -7,6 +7,7 @@ bench.py's CPU-baseline leg use it.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import os
 import subprocess
 
@@ -57,6 +58,8 @@ def lib():
         L.nslo_get_shape_bounds.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint8)]
         L.nslo_get_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.nslo_max_threads.restype = ctypes.c_int
+        L.nslo_create_revolute_joint.argtypes = [P, I, I, fp, fp, F, F, I, I, F, F, F, I, F, F, I, F, F]
+        L.nslo_get_joint.argtypes = [P, I, fp]
         _LIB = L
     return _LIB
 
@@ -107,11 +110,29 @@ class OracleWorld:
         return self.L.nslo_create_capsule(self.h, body, (ctypes.c_float * 3)(*c1), (ctypes.c_float * 3)(*c2), radius, density,
                                           friction, restitution)
 
+    def create_revolute_joint(self, j) -> int:
+        """j: natsel_amd.scenes.JointSpec."""
+        fa = (ctypes.c_float * 7)(*j.frame_a)
+        fb = (ctypes.c_float * 7)(*j.frame_b)
+        return self.L.nslo_create_revolute_joint(self.h, j.body_a, j.body_b, fa, fb, j.constraint_hertz,
+                                                 j.constraint_damping_ratio, int(j.collide_connected), int(j.enable_spring),
+                                                 j.hertz, j.damping_ratio, j.target_angle, int(j.enable_limit), j.lower_angle,
+                                                 j.upper_angle, int(j.enable_motor), j.max_motor_torque, j.motor_speed)
+
+    def joint(self, joint_id: int):
+        """(impulses f32[9]: linear xyz, perpendicular xy, spring, motor, lower, upper; graph colour)."""
+        out = np.zeros(9, np.float32)
+        color = self.L.nslo_get_joint(self.h, joint_id, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return out, int(color)
+
     def step(self, dt=1.0 / 60.0, substeps=4):
         self.L.nslo_step(self.h, dt, substeps)
 
     def world_state_hash(self) -> int:
         return self.L.nslo_state_hash(self.h)
+
+    def contact_count(self) -> int:
+        return int(self.L.nslo_contact_count(self.h))
 
     def body_arrays(self):
         n = self.L.nslo_body_capacity(self.h)
@@ -172,4 +193,6 @@ def load_scene(scene) -> OracleWorld:
             w.create_sphere(ids[s.body], s.params[:3], s.params[3], s.density, s.friction, s.restitution)
         else:
             w.create_capsule(ids[s.body], s.params[:3], s.params[3:6], s.params[6], s.density, s.friction, s.restitution)
+    for j in scene.joints:
+        w.create_revolute_joint(dataclasses.replace(j, body_a=ids[j.body_a], body_b=ids[j.body_b]))
     return w

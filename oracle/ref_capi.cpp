@@ -69,6 +69,43 @@ int nslo_create_capsule(void* wp, int body, const float* c1, const float* c2, fl
                                                        Capsule{{c1[0], c1[1], c1[2]}, {c2[0], c2[1], c2[2]}, r});
 }
 
+// Revolute joint (types/joint_defs.rs:30-58, :154-167): frames are (px, py, pz, qx, qy, qz, qs).
+int nslo_create_revolute_joint(void* wp, int body_a, int body_b, const float* fa, const float* fb, float constraint_hertz,
+                               float constraint_damping_ratio, int collide_connected, int enable_spring, float hertz,
+                               float damping_ratio, float target_angle, int enable_limit, float lower_angle,
+                               float upper_angle, int enable_motor, float max_motor_torque, float motor_speed) {
+  RevoluteJointDef d;
+  d.body_a = body_a;
+  d.body_b = body_b;
+  d.local_frame_a = Xf{{fa[0], fa[1], fa[2]}, {{fa[3], fa[4], fa[5]}, fa[6]}};
+  d.local_frame_b = Xf{{fb[0], fb[1], fb[2]}, {{fb[3], fb[4], fb[5]}, fb[6]}};
+  d.constraint_hertz = constraint_hertz;
+  d.constraint_damping_ratio = constraint_damping_ratio;
+  d.collide_connected = collide_connected != 0;
+  d.enable_spring = enable_spring != 0;
+  d.hertz = hertz;
+  d.damping_ratio = damping_ratio;
+  d.target_angle = target_angle;
+  d.enable_limit = enable_limit != 0;
+  d.lower_angle = lower_angle;
+  d.upper_angle = upper_angle;
+  d.enable_motor = enable_motor != 0;
+  d.max_motor_torque = max_motor_torque;
+  d.motor_speed = motor_speed;
+  return static_cast<World*>(wp)->create_revolute_joint(d);
+}
+
+// Accumulated impulses of a joint: linear (3), perpendicular (2), spring, motor, lower,
+// upper; returns its graph colour (-1 when it is not in the graph).
+int nslo_get_joint(void* wp, int joint_id, float* out) {
+  World* w = static_cast<World*>(wp);
+  const JointSim& j = w->joint_sim(joint_id);
+  const float vals[9] = {j.linear_impulse.x, j.linear_impulse.y, j.linear_impulse.z, j.perp_impulse.x, j.perp_impulse.y,
+                         j.spring_impulse, j.motor_impulse, j.lower_impulse, j.upper_impulse};
+  std::memcpy(out, vals, sizeof(vals));
+  return w->joints[joint_id].color_index;
+}
+
 void nslo_step(void* wp, float dt, int substeps) { static_cast<World*>(wp)->step(dt, substeps); }
 uint64_t nslo_state_hash(void* wp) { return static_cast<World*>(wp)->state_hash(); }
 int nslo_body_capacity(void* wp) { return (int)static_cast<World*>(wp)->bodies.size(); }

@@ -888,21 +888,41 @@ void World::store_overflow() {
 // One parallel region: every thread walks the schedule; each stage's loop over one
 // colour (or over bodies) is shared out with an implicit barrier at its end, and the
 // overflow list runs on one thread.
+// solver_stages.rs:541-589: per substep, overflow joints then overflow contacts, then each
+// colour's joints and contacts (disjoint bodies within a colour).
 void World::solve_stages(const StepContext& ctx) {
 #pragma omp parallel
   {
     for (int sub_i = 0; sub_i < ctx.sub_step_count; ++sub_i) {
       integrate_velocities(ctx.h);
 #pragma omp single
-      warm_start_overflow();
-      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) warm_start_color(ci);
+      {
+        warm_start_overflow_joints();
+        warm_start_overflow();
+      }
+      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) {
+        warm_start_joints(ci);
+        warm_start_color(ci);
+      }
 #pragma omp single
-      push_overflow(ctx);
-      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) push_color(ci, ctx);
+      {
+        solve_overflow_joints(ctx, true);
+        push_overflow(ctx);
+      }
+      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) {
+        solve_joints(ci, ctx, true);
+        push_color(ci, ctx);
+      }
       integrate_positions(ctx.h, ctx.inv_dt);
 #pragma omp single
-      relax_overflow(ctx);
-      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) relax_color(ci, ctx);
+      {
+        solve_overflow_joints(ctx, false);
+        relax_overflow(ctx);
+      }
+      for (int ci = 0; ci < kGraphColorCount - 1; ++ci) {
+        solve_joints(ci, ctx, false);
+        relax_color(ci, ctx);
+      }
     }
     if (solve_any_restitution && restitution_iterations > 0) {
       for (int it = 0; it < restitution_iterations; ++it) {

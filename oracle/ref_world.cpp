@@ -410,9 +410,8 @@ bool World::should_create_pair(int32_t ia, int32_t ib) const {
   } else if (!((a.filter.mask & b.filter.category) != 0 && (a.filter.category & b.filter.mask) != 0)) {
     return false;
   }
-  // contact_creation.rs:107-137 (no joints here)
-  if (bodies[a.body_id].type != kDynamic && bodies[b.body_id].type != kDynamic) return false;
-  return true;
+  // contact_creation.rs:110-143: a joint without collide_connected blocks the pair
+  return should_bodies_collide(bodies[a.body_id], bodies[b.body_id]);
 }
 
 void World::update_broad_phase_pairs() {
@@ -514,6 +513,7 @@ void World::create_contact(int32_t ia, int32_t ib, int32_t child) {
   float ra = sa.type == kSphere ? sa.sphere.radius : (sa.type == kCapsule ? sa.capsule.radius : 0.0f);
   float rb = sb.type == kSphere ? sb.sphere.radius : (sb.type == kCapsule ? sb.capsule.radius : 0.0f);
   c.rolling_resistance = maxf(sa.rolling_resistance, sb.rolling_resistance) * maxf(ra, rb);
+  c.created = ++contact_serial;
   contacts[id] = c;
 }
 
@@ -927,6 +927,7 @@ void World::step(float time_step, int sub_step_count) {
   stats.ms_collide += 1e3 * (t2 - t1);
   if (time_step > 0.0f) {
     if (awake_sims.empty()) return;
+    prepare_joints(ctx);
     prepare_contacts(ctx);
     double t3 = omp_get_wtime();
     solve_stages(ctx);

@@ -11,8 +11,9 @@
 //   SATCache/FeaturePair       src/types/collision_types.rs:97-121
 //   ConstraintGraph            src/simulation/constraint_graph.rs
 //   World::step                src/simulation/solver_integration.rs:807-1015
-// Islands, sleeping, CCD, joints, sensors, events and meshes are not restated: the
-// pinned scenes run with them inert (see the oracle's header in tests/).
+//   Joint/JointSim (revolute)  src/simulation/joint.rs:164-375, revolute_joint.rs:27-57
+// Islands, sleeping, CCD, sensors, events, meshes and joint types other than the
+// revolute joint are not restated: the pinned scenes run with them inert.
 #pragma once
 #include <cstdint>
 #include <memory>
@@ -77,6 +78,7 @@ struct Body {
   int32_t set_index = kNullIndex, local_index = kNullIndex;
   int32_t head_contact_key = kNullIndex, contact_count = 0;
   int32_t head_shape_id = kNullIndex, shape_count = 0;
+  int32_t head_joint_key = kNullIndex, joint_count = 0;
   float sleep_threshold = 0, sleep_time = 0, sleep_velocity = 0, safety_factor = 0, mass = 0;
   int32_t id = kNullIndex;
   uint32_t flags = 0;
@@ -176,6 +178,7 @@ struct Contact {
   SimplexCache simplex_cache{};
   float restitution = 0, rolling_resistance = 0;
   Vec3 tangent_velocity{};
+  uint64_t created = 0;  // creation order: the order of a body's head-inserted contact list
 };
 
 // src/containers/id_pool.rs: LIFO free list
@@ -199,15 +202,62 @@ struct BitSet {
   void clear(uint32_t i) { uint32_t w = i >> 6; if (w < words.size()) words[w] &= ~(1ull << (i & 63)); }
 };
 
+struct Softness { float bias_rate, mass_scale, impulse_scale; };
+Softness make_soft(float hertz, float zeta, float h);
+
+// Joint bookkeeping (joint.rs:164-375): doubly linked edge lists on both bodies, keyed
+// joint_id << 1 | edge.
+struct JointEdge { int32_t body_id = kNullIndex, prev_key = kNullIndex, next_key = kNullIndex; };
+struct Joint {
+  int32_t joint_id = kNullIndex, set_index = kNullIndex, color_index = kNullIndex, local_index = kNullIndex;
+  JointEdge edges[2];
+  bool collide_connected = false;
+};
+
+// JointDef + RevoluteJointDef (types/joint_defs.rs:30-58, :154-167)
+struct RevoluteJointDef {
+  int32_t body_a = kNullIndex, body_b = kNullIndex;
+  Xf local_frame_a{kZero3, kQuatIdentity}, local_frame_b{kZero3, kQuatIdentity};
+  float constraint_hertz = 60.0f, constraint_damping_ratio = 2.0f;
+  bool collide_connected = false;
+  bool enable_spring = false;
+  float hertz = 0.0f, damping_ratio = 0.0f, target_angle = 0.0f;
+  bool enable_limit = false;
+  float lower_angle = 0.0f, upper_angle = 0.0f;
+  bool enable_motor = false;
+  float max_motor_torque = 0.0f, motor_speed = 0.0f;
+};
+
+// JointSim with the revolute payload (world_runtime.rs JointSim; revolute_joint.rs:27-57)
+struct JointSim {
+  int32_t joint_id = kNullIndex, body_id_a = kNullIndex, body_id_b = kNullIndex;
+  Xf local_frame_a{kZero3, kQuatIdentity}, local_frame_b{kZero3, kQuatIdentity};
+  float inv_mass_a = 0, inv_mass_b = 0;
+  Mat3 inv_ia{}, inv_ib{};
+  float constraint_hertz = 60.0f, constraint_damping_ratio = 2.0f;
+  Softness constraint_softness{0.0f, 1.0f, 0.0f};
+  bool fixed_rotation = false;
+  // revolute payload
+  Vec3 linear_impulse = kZero3;
+  Vec2 perp_impulse{0, 0};
+  float spring_impulse = 0, motor_impulse = 0, lower_impulse = 0, upper_impulse = 0;
+  float hertz = 0, damping_ratio = 0, max_motor_torque = 0, motor_speed = 0;
+  float target_angle = 0, lower_angle = 0, upper_angle = 0;
+  int32_t index_a = kNullIndex, index_b = kNullIndex;
+  Xf frame_a{kZero3, kQuatIdentity}, frame_b{kZero3, kQuatIdentity};
+  Vec3 rotation_axis_z = kZero3, perp_axis_x = kZero3, perp_axis_y = kZero3, delta_center = kZero3;
+  float axial_mass = 0;
+  Softness spring_softness{0, 0, 0};
+  bool enable_spring = false, enable_motor = false, enable_limit = false;
+};
+
 struct ContactSpec { int32_t contact_id; int32_t manifold_start; uint16_t manifold_count; };
 struct GraphColor {
   BitSet body_set;
+  std::vector<JointSim> joint_sims;
   std::vector<int32_t> convex_contacts;
   std::vector<ContactSpec> contacts;  // overflow (and mesh) contacts
 };
-
-struct Softness { float bias_rate, mass_scale, impulse_scale; };
-Softness make_soft(float hertz, float zeta, float h);
 
 // src/simulation/contact_solver.rs:19-115
 struct SymMat3 { float cxx, cxy, cxz, cyy, cyz, czz; };
@@ -301,6 +351,8 @@ class World {
   int32_t create_hull_shape(int32_t body, const ShapeDef& def, const Hull& hull);
   int32_t create_sphere_shape(int32_t body, const ShapeDef& def, Sphere s);
   int32_t create_capsule_shape(int32_t body, const ShapeDef& def, Capsule c);
+  int32_t create_revolute_joint(const RevoluteJointDef& def);
+  const JointSim& joint_sim(int32_t joint_id) const;
   void step(float dt, int sub_steps);
   uint64_t state_hash() const;
 
@@ -343,6 +395,10 @@ class World {
   std::vector<ManifoldConstraint> overflow_manifolds;
   bool solve_any_restitution = false;
   std::vector<uint64_t> last_pair_keys;  // sorted keys created by the last pair update
+  std::vector<Joint> joints;
+  IDPool joint_pool;
+  uint64_t contact_serial = 0;
+  std::vector<JointSim> static_joint_sims;  // joints with no dynamic body: never solved
 
  private:
   void update_body_mass_data(int32_t body_id);
@@ -363,12 +419,23 @@ class World {
   void remove_contact_from_graph(int32_t body_a, int32_t body_b, int32_t color, int32_t local, bool mesh);
   void add_non_touching_contact(int32_t contact_id);
   void remove_non_touching_contact(int32_t local_index);
+  bool should_bodies_collide(const Body& a, const Body& b) const;
+  void destroy_contacts_between_bodies(int32_t body_a, int32_t body_b);
+  int32_t assign_joint_color(int32_t body_a, int32_t body_b, BodyType type_a, BodyType type_b);
 
   void collide();
   void collide_one_contact(int32_t contact_id, std::vector<uint8_t>& state_bits);
   bool update_contact(int32_t contact_id, Vec3 lc_a, WXf xf_a, Vec3 lc_b, WXf xf_b);
   bool compute_convex_manifold(Contact& c, const Shape& sa, WXf xf_a, const Shape& sb, WXf xf_b);
 
+  void prepare_joints(const StepContext& ctx);
+  void prepare_revolute_joint(JointSim& j, const StepContext& ctx);
+  void warm_start_joint(JointSim& j);
+  void solve_joint(JointSim& j, const StepContext& ctx, bool use_bias);
+  void warm_start_joints(int ci);
+  void solve_joints(int ci, const StepContext& ctx, bool use_bias);
+  void warm_start_overflow_joints();
+  void solve_overflow_joints(const StepContext& ctx, bool use_bias);
   void prepare_contacts(const StepContext& ctx);
   void prepare_overflow_contacts(Softness soft, Softness static_soft);
   void prepare_contact_constraint(int32_t contact_id, ContactConstraint& out, int32_t manifold_start, Softness soft, Softness static_soft, float warm_start_scale);

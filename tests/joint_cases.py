@@ -1,0 +1,82 @@
+"""Revolute-joint measurements over any stepper (the oracle, or the HIP world twinned with it).
+
+The reference's own revolute tests (src/tests/revolute_joint_tests.rs) only check that a run
+replays identically, so these scenes are chosen for what mechanics fixes instead: a hinged
+box is a compound pendulum whose period is an elliptic integral; the hinge keeps its pivot
+and its axis; a motor drives the relative angular speed to its target; a limit stops the
+angle; a soft spring oscillates at its own frequency; a hanging chain keeps its length.
+"""
+import math
+
+import numpy as np
+from scipy.special import ellipk
+
+from natsel_amd import scenes
+
+TIME_STEP = 1.0 / 60.0
+SUB_STEPS = 4
+PIVOT = np.array([1.0, 3.0, 0.0])  # hinged_pendulum: static body at (0, 3, 0), frame A at (1, 0, 0)
+
+
+def rotate(q, v):
+    """Rotate v by the unit quaternion q = (x, y, z, s)."""
+    qv, s = np.asarray(q[:3], np.float64), float(q[3])
+    t = 2.0 * np.cross(qv, v)
+    return v + s * t + np.cross(qv, t)
+
+
+def pendulum_period(theta0: float) -> float:
+    """Period of hinged_pendulum's compound pendulum released from rest at theta0: a cube
+    of side 0.4 hinged 0.5 m from its centre, g = 10 (T = 4 sqrt(I_p / (m g d)) K(sin(theta0/2)))."""
+    side, d, g = 0.4, 0.5, 10.0
+    i_over_m = side * side / 6.0 + d * d
+    return 4.0 * math.sqrt(i_over_m / (g * d)) * float(ellipk(math.sin(theta0 / 2.0) ** 2))
+
+
+def run_pendulum(load, steps, scene=None):
+    """Step hinged_pendulum; per step the box's pose, velocities and the pivot error (distance
+    between the two bodies' anchor points)."""
+    w = load(scene or scenes.hinged_pendulum())
+    out = {"pos": [], "rot": [], "v": [], "w": [], "pivot_err": []}
+    for _ in range(steps):
+        w.step(TIME_STEP, SUB_STEPS)
+        pos, rot, v, av = w.body_arrays()
+        anchor_b = pos[1] + rotate(rot[1], np.array([-0.5, 0.0, 0.0]))
+        out["pos"].append(pos[1].copy())
+        out["rot"].append(rot[1].copy())
+        out["v"].append(v[1].copy())
+        out["w"].append(av[1].copy())
+        out["pivot_err"].append(float(np.linalg.norm(anchor_b - PIVOT)))
+    return w, {k: np.array(v) for k, v in out.items()}
+
+
+def crossing_period(t, x):
+    """Mean interval between successive upward zero crossings of x(t), linearly interpolated."""
+    ups = [t[i] - x[i] * (t[i + 1] - t[i]) / (x[i + 1] - x[i]) for i in range(len(x) - 1) if x[i] < 0.0 <= x[i + 1]]
+    return float(np.mean(np.diff(ups))) if len(ups) >= 2 else float("nan")
+
+
+def twist_angle(rot):
+    """Rotation angle about z of quaternions (x, y, z, s) whose axis is z."""
+    return 2.0 * np.arctan2(rot[:, 2], rot[:, 3])
+
+
+def chain_stretch(load, links=8, steps=300):
+    """hanging_chain: per step the largest distance between the two anchor points of any
+    joint (anchor to first link included) and the contact count; finally the last link's
+    centre."""
+    sc = scenes.hanging_chain(links)
+    w = load(sc)
+    half = 0.5 * scenes.LINK_PITCH
+    worst, contacts = [], []
+    for _ in range(steps):
+        w.step(TIME_STEP, SUB_STEPS)
+        pos, rot, _, _ = w.body_arrays()
+        err = float(np.linalg.norm(pos[1] + rotate(rot[1], np.array([-half, 0.0, 0.0])) - pos[0]))
+        for i in range(1, links):
+            a = pos[i] + rotate(rot[i], np.array([half, 0.0, 0.0]))
+            b = pos[i + 1] + rotate(rot[i + 1], np.array([-half, 0.0, 0.0]))
+            err = max(err, float(np.linalg.norm(a - b)))
+        worst.append(err)
+        contacts.append(w.contact_count())
+    return np.array(worst), np.array(contacts), w.body_arrays()[0][links].copy()
