@@ -107,8 +107,26 @@ typedef struct nsl_kernel_stats {
   double host_ms[4];     /* host wall time: pair update, collide (incl. wait), state pass, solve+finalize (incl. wait) */
 } nsl_kernel_stats;
 
+/* A revolute joint (RevoluteJointDef, src/types/joint_defs.rs:30-58, :154-167). Frames
+   are (px, py, pz, qx, qy, qz, qs) in each body's frame; the hinge axis is frame A's z. */
+typedef struct nsl_revolute_joint_def {
+  int32_t body_a, body_b;
+  float local_frame_a[7];
+  float local_frame_b[7];
+  float constraint_hertz;            /* 60 */
+  float constraint_damping_ratio;    /* 2 */
+  int32_t collide_connected;         /* 0: the bodies' contacts are destroyed and no new one forms */
+  int32_t enable_spring;             /* 0 */
+  float hertz, damping_ratio, target_angle;
+  int32_t enable_limit;              /* 0 */
+  float lower_angle, upper_angle;
+  int32_t enable_motor;              /* 0 */
+  float max_motor_torque, motor_speed;
+} nsl_revolute_joint_def;
+
 const char* nsl_last_error(void);
 void nsl_default_options(nsl_options* out);
+void nsl_default_revolute_joint_def(nsl_revolute_joint_def* out);
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out);
 int nsl_world_destroy(nsl_world* world);
@@ -120,6 +138,15 @@ int nsl_create_sphere(nsl_world* world, int32_t body, const float center[3], flo
                       int32_t* out_shape);
 int nsl_create_capsule(nsl_world* world, int32_t body, const float c1[3], const float c2[3], float radius,
                        const nsl_material* m, int32_t* out_shape);
+
+/* World::create_revolute_joint (src/simulation/revolute_joint.rs:149-174; joint.rs:164-375). */
+int nsl_create_revolute_joint(nsl_world* world, const nsl_revolute_joint_def* def, int32_t* out_joint);
+/* Accumulated impulses of a joint: linear xyz, perpendicular xy, spring, motor, lower,
+   upper (RevoluteJointSim, revolute_joint.rs:27-57) and its graph colour (-1: not in the
+   graph). */
+int nsl_world_get_joint(nsl_world* world, int32_t joint, float impulses[9], int32_t* color);
+int nsl_world_joint_count(nsl_world* world, int32_t* out);
+int nsl_world_contact_count(nsl_world* world, int32_t* out);
 
 int nsl_world_step(nsl_world* world, float dt, int32_t sub_steps);
 int nsl_world_step_n(nsl_world* world, float dt, int32_t sub_steps, int32_t steps);

@@ -53,6 +53,28 @@ class Material(ctypes.Structure):
     ]
 
 
+class RevoluteJointDef(ctypes.Structure):
+    _fields_ = [
+        ("body_a", ctypes.c_int32),
+        ("body_b", ctypes.c_int32),
+        ("local_frame_a", ctypes.c_float * 7),
+        ("local_frame_b", ctypes.c_float * 7),
+        ("constraint_hertz", ctypes.c_float),
+        ("constraint_damping_ratio", ctypes.c_float),
+        ("collide_connected", ctypes.c_int32),
+        ("enable_spring", ctypes.c_int32),
+        ("hertz", ctypes.c_float),
+        ("damping_ratio", ctypes.c_float),
+        ("target_angle", ctypes.c_float),
+        ("enable_limit", ctypes.c_int32),
+        ("lower_angle", ctypes.c_float),
+        ("upper_angle", ctypes.c_float),
+        ("enable_motor", ctypes.c_int32),
+        ("max_motor_torque", ctypes.c_float),
+        ("motor_speed", ctypes.c_float),
+    ]
+
+
 class Profile(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("pairs_ms", "collide_ms", "state_pass_ms", "solve_ms", "finalize_ms", "step_ms")] + [
         (n, ctypes.c_int32)
@@ -111,6 +133,11 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), _F, ctypes.POINTER(Material), ctypes.POINTER(_I)],
     ),
+    "nsl_default_revolute_joint_def": (None, [ctypes.POINTER(RevoluteJointDef)]),
+    "nsl_create_revolute_joint": (ctypes.c_int, [_P, ctypes.POINTER(RevoluteJointDef), ctypes.POINTER(_I)]),
+    "nsl_world_get_joint": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_I)]),
+    "nsl_world_joint_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "nsl_world_contact_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "nsl_world_step": (ctypes.c_int, [_P, _F, _I]),
     "nsl_world_step_n": (ctypes.c_int, [_P, _F, _I, _I]),
     "nsl_world_body_capacity": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),

@@ -54,7 +54,7 @@ __global__ void k_bp_count(int n, const int32_t* ids, const uint32_t* keys, cons
 
 // One candidate (p, q) per index t < offsets[n]; p is the last position with offsets[p] <= t.
 __global__ void k_bp_test(int n, const int64_t* offsets, const int32_t* ids, const Aabb* fat, const uint8_t* moved,
-                          const Proxy* proxies, PairSet set, uint64_t* out, int64_t cap, int64_t* found) {
+                          const Proxy* proxies, PairSet set, PairSet blocked, uint64_t* out, int64_t cap, int64_t* found) {
   const int64_t total = offsets[n];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t tests = 0;
@@ -79,6 +79,8 @@ __global__ void k_bp_test(int n, const int64_t* offsets, const int32_t* ids, con
     } else if (!((pa.mask & pb.category) != 0 && (pa.category & pb.mask) != 0)) {
       continue;
     }
+    // contact_creation.rs:110-143: a joint without collide_connected between the bodies
+    if (blocked.slots != nullptr && pairset_contains(blocked, pair_key(pa.body, pb.body))) continue;
     const uint64_t key = pair_key(a, b);
     if (pairset_contains(set, key)) continue;
     const int64_t slot = (int64_t)atomicAdd((unsigned long long*)&found[0], 1ull);
@@ -148,7 +150,7 @@ void PairFinder::find(const DeviceWorld& d, const Proxy* proxies, int n, std::ve
   for (int attempt = 0; attempt < 2; ++attempt) {
     BP_CHECK(hipMemsetAsync(found_, 0, 2 * sizeof(int64_t), stream_));
     hipLaunchKernelGGL(k_bp_test, dim3(kTestBlocks), dim3(kBlock), 0, stream_, n, offsets_, ids_out_, d.fat_aabbs,
-                       d.shape_moved, proxies, d.pairs, pairs_, (int64_t)cap_pairs_, found_);
+                       d.shape_moved, proxies, d.pairs, d.joint_pairs, pairs_, (int64_t)cap_pairs_, found_);
     BP_CHECK(hipMemcpyAsync(host_, found_, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
     BP_CHECK(hipStreamSynchronize(stream_));
     if (host_[0] <= (int64_t)cap_pairs_) break;

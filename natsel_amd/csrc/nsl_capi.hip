@@ -1,6 +1,7 @@
 // extern "C" boundary of the GPU stepper (declared in include/natsel_gpu.h).
 #include <exception>
 #include <new>
+#include <cstring>
 #include <string>
 
 #include <stdexcept>
@@ -169,6 +170,62 @@ int nsl_create_capsule(nsl_world* world, int32_t body, const float c1[3], const 
 int nsl_world_step(nsl_world* world, float dt, int32_t sub_steps) {
   if (!world || !(dt >= 0.0f)) return fail(NSL_ERR_INVALID, "null world or negative time step");
   return guarded([&] { world->impl->step(dt, sub_steps); });
+}
+
+void nsl_default_revolute_joint_def(nsl_revolute_joint_def* out) {
+  if (!out) return;
+  nsl::JointDesc d;
+  std::memset(out, 0, sizeof(*out));
+  out->body_a = d.body_a;
+  out->body_b = d.body_b;
+  out->local_frame_a[6] = 1.0f;
+  out->local_frame_b[6] = 1.0f;
+  out->constraint_hertz = d.constraint_hertz;
+  out->constraint_damping_ratio = d.constraint_damping_ratio;
+}
+
+int nsl_create_revolute_joint(nsl_world* world, const nsl_revolute_joint_def* def, int32_t* out_joint) {
+  if (!world || !def || !out_joint) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] {
+    nsl::JointDesc d;
+    d.body_a = def->body_a;
+    d.body_b = def->body_b;
+    const float* fa = def->local_frame_a;
+    const float* fb = def->local_frame_b;
+    d.local_frame_a = nsl::Xf{{fa[0], fa[1], fa[2]}, {{fa[3], fa[4], fa[5]}, fa[6]}};
+    d.local_frame_b = nsl::Xf{{fb[0], fb[1], fb[2]}, {{fb[3], fb[4], fb[5]}, fb[6]}};
+    d.constraint_hertz = def->constraint_hertz;
+    d.constraint_damping_ratio = def->constraint_damping_ratio;
+    d.collide_connected = def->collide_connected != 0;
+    d.enable_spring = def->enable_spring != 0;
+    d.hertz = def->hertz;
+    d.damping_ratio = def->damping_ratio;
+    d.target_angle = def->target_angle;
+    d.enable_limit = def->enable_limit != 0;
+    d.lower_angle = def->lower_angle;
+    d.upper_angle = def->upper_angle;
+    d.enable_motor = def->enable_motor != 0;
+    d.max_motor_torque = def->max_motor_torque;
+    d.motor_speed = def->motor_speed;
+    *out_joint = world->impl->create_revolute_joint(d);
+  });
+}
+
+int nsl_world_get_joint(nsl_world* world, int32_t joint, float impulses[9], int32_t* color) {
+  if (!world || !impulses || !color) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { world->impl->get_joint(joint, impulses, color); });
+}
+
+int nsl_world_joint_count(nsl_world* world, int32_t* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  *out = world->impl->joint_count();
+  return NSL_OK;
+}
+
+int nsl_world_contact_count(nsl_world* world, int32_t* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  *out = world->impl->contact_count();
+  return NSL_OK;
 }
 
 int nsl_world_step_n(nsl_world* world, float dt, int32_t sub_steps, int32_t steps) {

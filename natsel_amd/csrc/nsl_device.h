@@ -27,6 +27,9 @@ struct DeviceWorld {
                             // [4] manifold points prepared (coloured), [5] overflow points,
                             // [6] proxies moved (fat AABB enlarged) since the last pair update
   PairSet pairs;            // keys of the shape pairs that have a contact
+  JointSim* joints;         // by joint id
+  int32_t* joint_ids;       // joints in the graph, colour-major; the overflow colour's last
+  PairSet joint_pairs;      // body-pair keys of joints without collide_connected (no contact allowed)
   int32_t n_awake;
 };
 
@@ -76,6 +79,8 @@ struct ColorRanges {
 // Everything the step kernel needs beyond the world pointers.
 struct SolvePlan {
   ColorRanges cr;                 // coloured slots per colour (colour-major)
+  ColorRanges jr;                 // joint_ids per colour, overflow colour included
+  int32_t n_joints;               // joints in the graph (prepared each step)
   int32_t n_constraints;          // coloured slots in total
   int32_t n_overflow;             // overflow constraints, solved in list order by one lane
   int32_t any_restitution;
@@ -90,6 +95,7 @@ constexpr int kMaxPhaseLog = 1024;
 void launch_init_contacts(const DeviceWorld& d, const int32_t* ids, const Contact* records, int n, hipStream_t s);
 void launch_collide(const DeviceWorld& d, int n, hipStream_t s, KernelTimer* timer);
 void launch_pairset_insert(PairSet set, const uint64_t* keys, int n, hipStream_t s);
+void launch_pairset_erase(PairSet set, const uint64_t* keys, int n, hipStream_t s);
 // One launch (all workgroups resident): prepare, the substep schedule, restitution, store
 // and finalize.
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,

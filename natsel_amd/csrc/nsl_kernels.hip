@@ -8,6 +8,7 @@
 #include <string>
 
 #include "nsl_device.h"
+#include "nsl_joint.h"
 #include "nsl_pairset.h"
 #include "nsl_stages.h"
 
@@ -31,6 +32,11 @@ __global__ void k_init_contacts(DeviceWorld d, const int32_t* ids, const Contact
 __global__ void k_pairset_insert(PairSet set, const uint64_t* keys, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) pairset_insert(set, keys[t]);
+}
+
+__global__ void k_pairset_erase(PairSet set, const uint64_t* keys, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) pairset_erase(set, keys[t]);
 }
 
 __global__ void k_collide(DeviceWorld d, int n) {
@@ -111,36 +117,58 @@ struct PhaseClock {
   }
 };
 
-// The non-empty colours in order with their slot ranges, kept in LDS (a table indexed at
-// run time in private memory would cost a memory round trip per phase).
+// The non-empty colours in order with their contact-slot and joint ranges, kept in LDS (a
+// table indexed at run time in private memory would cost a memory round trip per phase).
 struct ColourTable {
   int n;
   int begin[kOverflowColor];
   int end[kOverflowColor];
+  int jbegin[kOverflowColor];
+  int jend[kOverflowColor];
 };
 
-// Body indices of a lane's first slot in active colour `a` ({-1, -1} past its end).
-__device__ __forceinline__ int2 first_indices(const DeviceWorld& d, const ColourTable& t, int a, int tid) {
-  const int slot = t.begin[a] + tid;
-  if (slot >= t.end[a]) return make_int2(kNullIndex, kNullIndex);
+// A colour's items are its joints (in joint_ids order) and then its contact slots: the
+// reference's unit order within a colour (solver_stages.rs:303-345). The two touch
+// disjoint bodies, so lanes may take them in any mix.
+__device__ __forceinline__ int joints_in(const ColourTable& t, int a, bool with_joints) {
+  return with_joints ? t.jend[a] - t.jbegin[a] : 0;
+}
+
+// Body indices of a lane's first contact slot in active colour `a` ({-1, -1} when its
+// first item is a joint or lies past the end).
+__device__ __forceinline__ int2 first_indices(const DeviceWorld& d, const ColourTable& t, int a, int tid, bool with_joints) {
+  const int nj = joints_in(t, a, with_joints);
+  const int slot = t.begin[a] + tid - nj;
+  if (tid < nj || slot >= t.end[a]) return make_int2(kNullIndex, kNullIndex);
   return make_int2(d.constraints[slot].index_a, d.constraints[slot].index_b);
 }
 
 // One stage over the active colours in order, a grid barrier after each. A lane's body
-// indices for its first slot in the next colour are read before the barrier, so each
-// phase opens with its body-state loads.
-template <class Item>
+// indices for its first contact slot in the next colour are read before the barrier, so
+// each phase opens with its body-state loads. `joint` runs the colour's joints when
+// `with_joints` (warm start, push and relax; restitution has no joint rows).
+template <class JointItem, class Item>
 __device__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid, int nthreads, GridBarrier* bar, PhaseClock& clock,
-                            int cls, Item item) {
+                            int cls, bool with_joints, JointItem joint, Item item) {
   const int n_active = t.n;
   if (n_active == 0) return true;
-  int2 pf = first_indices(d, t, 0, tid);
+  int2 pf = first_indices(d, t, 0, tid, with_joints);
   for (int a = 0; a < n_active; ++a) {
-    const int end = t.end[a];
-    int slot = t.begin[a] + tid;
-    if (slot < end) item(slot, pf.x, pf.y);
-    for (slot += nthreads; slot < end; slot += nthreads) item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
-    if (a + 1 < n_active) pf = first_indices(d, t, a + 1, tid);
+    const int nj = joints_in(t, a, with_joints);
+    const int jb = t.jbegin[a], cb = t.begin[a] - nj;
+    const int n_items = nj + (t.end[a] - t.begin[a]);
+    int k = tid;
+    if (k < nj) joint(d.joint_ids[jb + k]);
+    else if (k < n_items) item(cb + k, pf.x, pf.y);
+    for (k += nthreads; k < n_items; k += nthreads) {
+      if (k < nj) {
+        joint(d.joint_ids[jb + k]);
+      } else {
+        const int slot = cb + k;
+        item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
+      }
+    }
+    if (a + 1 < n_active) pf = first_indices(d, t, a + 1, tid, with_joints);
     if (!grid_sync(bar)) return false;
     clock.mark(cls);
   }
@@ -165,17 +193,24 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   if (threadIdx.x == 0) {
     int n = 0;
     for (int ci = 0; ci < kOverflowColor; ++ci)
-      if (plan.cr.end[ci] > plan.cr.begin[ci]) {
+      if (plan.cr.end[ci] > plan.cr.begin[ci] || plan.jr.end[ci] > plan.jr.begin[ci]) {
         colours.begin[n] = plan.cr.begin[ci];
         colours.end[n] = plan.cr.end[ci];
+        colours.jbegin[n] = plan.jr.begin[ci];
+        colours.jend[n] = plan.jr.end[ci];
         ++n;
       }
     colours.n = n;
   }
   __syncthreads();
+  // overflow-colour joints, solved by one lane ahead of the overflow contacts
+  const int ovf_jb = plan.jr.begin[kOverflowColor], ovf_je = plan.jr.end[kOverflowColor];
+  const bool any_ovf = n_ovf > 0 || ovf_je > ovf_jb;
+  auto no_joint = [](int32_t) {};
 
   {
     int points = 0, ovf_points = 0;
+    for (int k = tid; k < plan.n_joints; k += nthreads) prepare_joint_item(d, d.joint_ids[k], p);
     for (int slot = tid; slot < plan.n_constraints; slot += nthreads) points += prepare_item(d, slot, p);
     for (int k = tid; k < n_ovf; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);
     block_add(&d.counters[4], points);
@@ -189,21 +224,29 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     if (!grid_sync(bar)) return;
     clock.mark(kPIntegrateVelocities);
 
-    if (n_ovf > 0) {
-      if (lane0) overflow_warm_all(d, n_ovf);
+    if (any_ovf) {
+      if (lane0) {
+        for (int k = ovf_jb; k < ovf_je; ++k) warm_start_joint_item(d, d.joint_ids[k]);
+        overflow_warm_all(d, n_ovf);
+      }
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
-    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPWarmStart,
+    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPWarmStart, true,
+                     [&](int32_t jid) { warm_start_joint_item(d, jid); },
                      [&](int slot, int32_t ia, int32_t ib) { warm_start_item(d, slot, ia, ib); }))
       return;
 
-    if (n_ovf > 0) {
-      if (lane0) overflow_solve_all(d, n_ovf, p, 0);
+    if (any_ovf) {
+      if (lane0) {
+        for (int k = ovf_jb; k < ovf_je; ++k) solve_joint_item(d, d.joint_ids[k], p, true);
+        overflow_solve_all(d, n_ovf, p, 0);
+      }
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
-    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPPush,
+    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPPush, true,
+                     [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
                      [&](int slot, int32_t ia, int32_t ib) { push_item(d, slot, p, ia, ib); }))
       return;
 
@@ -211,12 +254,16 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     if (!grid_sync(bar)) return;
     clock.mark(kPIntegratePositions);
 
-    if (n_ovf > 0) {
-      if (lane0) overflow_solve_all(d, n_ovf, p, 1);
+    if (any_ovf) {
+      if (lane0) {
+        for (int k = ovf_jb; k < ovf_je; ++k) solve_joint_item(d, d.joint_ids[k], p, false);
+        overflow_solve_all(d, n_ovf, p, 1);
+      }
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
-    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRelax,
+    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRelax, true,
+                     [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
                      [&](int slot, int32_t ia, int32_t ib) { relax_item(d, slot, p, ia, ib); }))
       return;
   }
@@ -228,7 +275,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         if (!grid_sync(bar)) return;
         clock.mark(kPOverflow);
       }
-      if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRestitution,
+      if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRestitution, false, no_joint,
                        [&](int slot, int32_t ia, int32_t ib) { restitution_item(d, slot, p, ia, ib); }))
         return;
     }
@@ -260,6 +307,10 @@ void launch_init_contacts(const DeviceWorld& d, const int32_t* ids, const Contac
 
 void launch_pairset_insert(PairSet set, const uint64_t* keys, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_pairset_insert, dim3(blocks_for(n)), dim3(kBlock), 0, s, set, keys, n);
+}
+
+void launch_pairset_erase(PairSet set, const uint64_t* keys, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_pairset_erase, dim3(blocks_for(n)), dim3(kBlock), 0, s, set, keys, n);
 }
 
 void launch_collide(const DeviceWorld& d, int n, hipStream_t s, KernelTimer* t) {

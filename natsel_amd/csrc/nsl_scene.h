@@ -28,6 +28,21 @@ struct Material {
   int32_t group_index = 0;
 };
 
+// RevoluteJointDef (types/joint_defs.rs:30-58, :154-167). Frames are in the bodies' own
+// frames; the hinge axis is frame A's z axis.
+struct JointDesc {
+  int32_t body_a = -1, body_b = -1;
+  Xf local_frame_a{{0, 0, 0}, {{0, 0, 0}, 1}}, local_frame_b{{0, 0, 0}, {{0, 0, 0}, 1}};
+  float constraint_hertz = 60.0f, constraint_damping_ratio = 2.0f;
+  bool collide_connected = false;
+  bool enable_spring = false;
+  float hertz = 0.0f, damping_ratio = 0.0f, target_angle = 0.0f;
+  bool enable_limit = false;
+  float lower_angle = 0.0f, upper_angle = 0.0f;
+  bool enable_motor = false;
+  float max_motor_torque = 0.0f, motor_speed = 0.0f;
+};
+
 // World settings (world_runtime.rs:1243-1304), shared by the steppers.
 struct WorldOptions {
   Vec3 gravity{0.0f, -10.0f, 0.0f};

@@ -161,6 +161,35 @@ struct OverflowConstraint {
   ConstraintPoint points[4];
 };
 
+// A revolute joint (JointSim + RevoluteJointSim, world_runtime.rs / revolute_joint.rs:27-57),
+// indexed by joint id. The definition fields are written once by the host; prepare fills
+// the per-step fields; the impulses persist across steps for warm starting.
+namespace jflag {
+constexpr uint32_t kSpring = 0x1, kMotor = 0x2, kLimit = 0x4;
+}  // namespace jflag
+
+struct JointSim {
+  // definition (types/joint_defs.rs:30-58, :154-167)
+  int32_t enc_a, enc_b;  // encoded body sim index: awake >= 0, static -(index + 2)
+  Xf local_frame_a, local_frame_b;
+  float constraint_hertz, constraint_damping_ratio;
+  float hertz, damping_ratio, target_angle, lower_angle, upper_angle, max_motor_torque, motor_speed;
+  uint32_t flags;  // jflag
+  // prepared each step
+  Softness constraint_softness, spring_softness;
+  float inv_mass_a, inv_mass_b;
+  Mat3 inv_ia, inv_ib;
+  int32_t index_a, index_b;
+  int32_t fixed_rotation;
+  float axial_mass;
+  Xf frame_a, frame_b;
+  Vec3 rotation_axis_z, perp_axis_x, perp_axis_y, delta_center;
+  // accumulated impulses
+  Vec3 linear_impulse;
+  Vec2 perp_impulse;
+  float spring_impulse, motor_impulse, lower_impulse, upper_impulse;
+};
+
 struct StepParams {
   float dt, inv_dt, h, inv_h;
   int32_t sub_steps;

@@ -36,6 +36,14 @@ class GpuWorld : public HostScene {
 
   void step(float dt, int32_t sub_steps);
 
+  // A revolute joint between two bodies (joint.rs:164-375); returns its id. Without
+  // collide_connected the bodies' contacts are destroyed and no new one forms.
+  int32_t create_revolute_joint(const JointDesc& desc);
+  int32_t joint_count() const { return (int32_t)hjoints_.size(); }
+  // Accumulated impulses (linear xyz, perpendicular xy, spring, motor, lower, upper) and
+  // graph colour of a joint (-1: not in the graph, both bodies non-dynamic).
+  void get_joint(int32_t id, float* impulses, int32_t* color);
+
   int32_t body_capacity() const { return (int32_t)bodies_.size(); }
   int32_t contact_count() const { return contact_pool_next_ - (int32_t)contact_free_.size(); }
   // Per body id: origin position, rotation (x, y, z, s), linear and angular velocity.
@@ -67,11 +75,16 @@ class GpuWorld : public HostScene {
     int32_t shape_a = -1, shape_b = -1, color = -1, local = -1, set = -1;
     int32_t alive = 0;
     float restitution = 0;
+    uint64_t created = 0;  // creation order (a body's contact list is newest first)
+  };
+  struct HJoint {
+    int32_t body_a = -1, body_b = -1, color = -1;
   };
   struct ColorSet {
     std::vector<uint64_t> bits;
     std::vector<int32_t> convex;
     std::vector<int32_t> overflow;
+    std::vector<int32_t> joints;  // joint ids in creation order
     bool get(int32_t b) const { uint32_t w = (uint32_t)b >> 6; return w < bits.size() && ((bits[w] >> (b & 63)) & 1); }
     void set(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w >= bits.size()) bits.resize(w + 1, 0); bits[w] |= 1ull << (b & 63); }
     void clear(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w < bits.size()) bits[w] &= ~(1ull << (b & 63)); }
@@ -90,6 +103,8 @@ class GpuWorld : public HostScene {
   void add_to_graph(int32_t id);
   void remove_from_graph(int32_t body_a, int32_t body_b, int32_t color, int32_t local);
   void state_pass();
+  int32_t assign_joint_color(int32_t a, int32_t b);
+  void upload_joint_pairs();
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -137,6 +152,15 @@ class GpuWorld : public HostScene {
   int64_t pair_set_tombs_ = 0;
   bool pairs_pending_ = true;   // proxies marked moved on the host since the last pair update
   StepProfile profile_{};
+
+  std::vector<HJoint> hjoints_;
+  uint64_t contact_serial_ = 0;
+  size_t cap_joints_ = 0, cap_joint_ids_ = 0, cap_erase_keys_ = 0;
+  uint64_t* d_erase_keys_ = nullptr;
+  std::vector<uint64_t> blocked_keys_;  // body pairs of joints without collide_connected
+  std::vector<int32_t> joint_ids_host_;
+  ColorRanges joint_ranges_{};
+  bool joints_dirty_ = false;
 };
 
 }  // namespace nsl

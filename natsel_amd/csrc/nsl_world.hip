@@ -50,7 +50,7 @@ GpuWorld::~GpuWorld() {
   void* ptrs[] = {d_.sims, d_.states, d_.static_sims, d_.shapes, d_.fat_aabbs, d_.shape_moved, d_.hulls, d_.contacts,
                   d_.contact_change, d_.constraints, d_.overflow, d_.collide_ids, d_.solve_ids, d_.overflow_ids,
                   d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
-                  d_rebuild_keys_, d_.pairs.slots};
+                  d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -244,6 +244,7 @@ void GpuWorld::create_contact(int32_t a, int32_t b) {
   c.rolling_resistance = maxf(sa.rolling_resistance, sb.rolling_resistance) * maxf(sa.radius, sb.radius);
   c.cached_rot_a = quat_identity();
   c.cached_rot_b = quat_identity();
+  hc.created = ++contact_serial_;
   init_ids_.push_back(id);
   init_records_.push_back(c);
 }
@@ -348,6 +349,132 @@ void GpuWorld::state_pass() {
 }
 
 // ---------------------------------------------------------------------------------
+// joints: joint.rs:130-375, constraint_graph.rs:256-323
+
+// constraint_graph.rs:256-298: the contact colouring's rules on the same body bits.
+int32_t GpuWorld::assign_joint_color(int32_t a, int32_t b) {
+  const int32_t ta = bodies_[a].type, tb = bodies_[b].type;
+  if (ta == kBodyDynamic && tb == kBodyDynamic) {
+    for (int i = 0; i < kDynamicColorCount; ++i) {
+      if (colors_[i].get(a) || colors_[i].get(b)) continue;
+      colors_[i].set(a);
+      colors_[i].set(b);
+      return i;
+    }
+  } else if (ta == kBodyDynamic || tb == kBodyDynamic) {
+    const int32_t dyn = ta == kBodyDynamic ? a : b;
+    for (int i = kOverflowColor - 1; i >= 1; --i) {
+      if (!colors_[i].get(dyn)) {
+        colors_[i].set(dyn);
+        return i;
+      }
+    }
+  }
+  return kOverflowColor;
+}
+
+// The device set of body pairs that may not touch, rebuilt whole (joints are few and
+// created rarely): power-of-two table at most a quarter full.
+void GpuWorld::upload_joint_pairs() {
+  if (blocked_keys_.empty()) return;
+  uint32_t want = 64;
+  while ((size_t)want < 4 * blocked_keys_.size()) want <<= 1;
+  if (!d_.joint_pairs.slots || d_.joint_pairs.mask + 1 != want) {
+    if (d_.joint_pairs.slots) {
+      NSL_CHECK(hipStreamSynchronize(stream_));
+      NSL_CHECK(hipFree(d_.joint_pairs.slots));
+    }
+    NSL_CHECK(hipMalloc(&d_.joint_pairs.slots, (size_t)want * sizeof(uint64_t)));
+    d_.joint_pairs.mask = want - 1;
+  }
+  NSL_CHECK(hipMemsetAsync(d_.joint_pairs.slots, 0xFF, (size_t)want * sizeof(uint64_t), stream_));
+  ensure(d_erase_keys_, cap_erase_keys_, blocked_keys_.size(), false);
+  NSL_CHECK(hipMemcpyAsync(d_erase_keys_, blocked_keys_.data(), blocked_keys_.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                           stream_));
+  launch_pairset_insert(d_.joint_pairs, d_erase_keys_, (int)blocked_keys_.size(), stream_);
+  NSL_CHECK(hipStreamSynchronize(stream_));
+}
+
+// joint.rs:164-375 + revolute_joint.rs:149-174 (no sleeping sets: a joint with a dynamic
+// body goes into the graph, one between non-dynamic bodies is kept but never solved).
+int32_t GpuWorld::create_revolute_joint(const JointDesc& desc) {
+  const int32_t a = desc.body_a, b = desc.body_b;
+  if (a == b || a < 0 || b < 0 || a >= (int32_t)bodies_.size() || b >= (int32_t)bodies_.size())
+    throw std::invalid_argument("joint bodies must be two distinct existing bodies");
+  NSL_CHECK(hipSetDevice(device_));
+
+  if (!desc.collide_connected) {
+    // joint.rs:130-161: the contacts between the bodies, newest first, are destroyed now
+    std::vector<std::pair<uint64_t, int32_t>> between;
+    for (size_t id = 0; id < hcontacts_.size(); ++id) {
+      const HContact& c = hcontacts_[id];
+      if (!c.alive) continue;
+      const int32_t ba = hshapes_[c.shape_a].body, bb = hshapes_[c.shape_b].body;
+      if ((ba == a && bb == b) || (ba == b && bb == a)) between.push_back({c.created, (int32_t)id});
+    }
+    std::sort(between.begin(), between.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<uint64_t> erase;
+    for (const auto& e : between) {
+      const HContact& c = hcontacts_[e.second];
+      erase.push_back(pair_key(c.shape_a, c.shape_b));
+      destroy_contact(e.second);
+    }
+    if (!erase.empty()) {
+      ensure(d_erase_keys_, cap_erase_keys_, erase.size(), false);
+      NSL_CHECK(hipMemcpyAsync(d_erase_keys_, erase.data(), erase.size() * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+      launch_pairset_erase(d_.pairs, d_erase_keys_, (int)erase.size(), stream_);
+      NSL_CHECK(hipStreamSynchronize(stream_));
+    }
+    blocked_keys_.push_back(pair_key(a, b));
+    upload_joint_pairs();
+  }
+
+  const int32_t id = (int32_t)hjoints_.size();
+  HJoint hj;
+  hj.body_a = a;
+  hj.body_b = b;
+  hj.color = bodies_[a].type != kBodyDynamic && bodies_[b].type != kBodyDynamic ? kNullIndex : assign_joint_color(a, b);
+  if (hj.color != kNullIndex) colors_[hj.color].joints.push_back(id);
+  hjoints_.push_back(hj);
+
+  JointSim j{};
+  j.enc_a = encode(bodies_[a]);
+  j.enc_b = encode(bodies_[b]);
+  j.local_frame_a = desc.local_frame_a;
+  j.local_frame_b = desc.local_frame_b;
+  j.constraint_hertz = desc.constraint_hertz;
+  j.constraint_damping_ratio = desc.constraint_damping_ratio;
+  j.constraint_softness = Softness{0.0f, 1.0f, 0.0f};
+  j.hertz = desc.hertz;
+  j.damping_ratio = desc.damping_ratio;
+  j.target_angle = desc.target_angle;
+  j.lower_angle = desc.lower_angle;
+  j.upper_angle = desc.upper_angle;
+  j.max_motor_torque = desc.max_motor_torque;
+  j.motor_speed = desc.motor_speed;
+  j.flags = (desc.enable_spring ? jflag::kSpring : 0u) | (desc.enable_motor ? jflag::kMotor : 0u) |
+            (desc.enable_limit ? jflag::kLimit : 0u);
+  j.index_a = j.index_b = kNullIndex;
+  ensure(d_.joints, cap_joints_, hjoints_.size(), true);
+  NSL_CHECK(hipMemcpyAsync(&d_.joints[id], &j, sizeof(JointSim), hipMemcpyHostToDevice, stream_));
+  NSL_CHECK(hipStreamSynchronize(stream_));
+  joints_dirty_ = true;
+  return id;
+}
+
+void GpuWorld::get_joint(int32_t id, float* impulses, int32_t* color) {
+  if (id < 0 || id >= (int32_t)hjoints_.size()) throw std::invalid_argument("no such joint");
+  NSL_CHECK(hipSetDevice(device_));
+  JointSim j{};
+  NSL_CHECK(hipMemcpyAsync(&j, &d_.joints[id], sizeof(JointSim), hipMemcpyDeviceToHost, stream_));
+  NSL_CHECK(hipStreamSynchronize(stream_));
+  const float vals[9] = {j.linear_impulse.x, j.linear_impulse.y, j.linear_impulse.z, j.perp_impulse.x, j.perp_impulse.y,
+                         j.spring_impulse, j.motor_impulse, j.lower_impulse, j.upper_impulse};
+  if (impulses) std::memcpy(impulses, vals, sizeof(vals));
+  if (color) *color = hjoints_[id].color;
+}
+
+// ---------------------------------------------------------------------------------
 // one step: solver_integration.rs:807-1015
 
 void GpuWorld::step(float dt, int32_t sub_steps) {
@@ -443,9 +570,26 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     ensure(d_.overflow, cap_overflow_, ovf.size(), false);
     NSL_CHECK(hipMemcpyAsync(d_.overflow_ids, ovf.data(), ovf.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
   }
+  // joint lists: colour-major ids, the overflow colour's last (solver_prepare.rs:178-212)
+  if (joints_dirty_) {
+    joint_ids_host_.clear();
+    for (int ci = 0; ci < kColorCount; ++ci) {
+      joint_ranges_.begin[ci] = (int32_t)joint_ids_host_.size();
+      joint_ids_host_.insert(joint_ids_host_.end(), colors_[ci].joints.begin(), colors_[ci].joints.end());
+      joint_ranges_.end[ci] = (int32_t)joint_ids_host_.size();
+    }
+    if (!joint_ids_host_.empty()) {
+      ensure(d_.joint_ids, cap_joint_ids_, joint_ids_host_.size(), false);
+      NSL_CHECK(hipMemcpyAsync(d_.joint_ids, joint_ids_host_.data(), joint_ids_host_.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, stream_));
+    }
+    joints_dirty_ = false;
+  }
   KernelTimer* tm = timing_ ? &timer_ : nullptr;
   SolvePlan plan{};
   plan.cr = cr;
+  plan.jr = joint_ranges_;
+  plan.n_joints = (int32_t)joint_ids_host_.size();
   plan.n_constraints = (int32_t)solve.size();
   plan.n_overflow = (int32_t)ovf.size();
   plan.any_restitution = any_restitution ? 1 : 0;
@@ -453,7 +597,8 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   plan.barrier = d_barrier_;
   plan.phase_log = timing_ ? d_phase_log_ : nullptr;
   int widest = (d_.n_awake + 3) / 4;
-  for (int ci = 0; ci < kOverflowColor; ++ci) widest = std::max(widest, cr.end[ci] - cr.begin[ci]);
+  for (int ci = 0; ci < kOverflowColor; ++ci)
+    widest = std::max(widest, (cr.end[ci] - cr.begin[ci]) + (joint_ranges_.end[ci] - joint_ranges_.begin[ci]));
   int blocks = options.solver_blocks > 0 ? options.solver_blocks : (widest + kSolveBlock - 1) / kSolveBlock;
   blocks = std::max(1, std::min(blocks, std::min(max_blocks_, 128)));
   launch_solve_step(d_, plan, p, blocks, stream_, tm);

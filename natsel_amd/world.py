@@ -10,6 +10,7 @@ computes physics.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -165,6 +166,37 @@ class World:
         capi.check(self._lib.nsl_create_capsule(self._h, body, c1, c2, c.radius, ctypes.byref(_material(d)), ctypes.byref(out)))
         return out.value
 
+    def create_revolute_joint(self, d) -> int:
+        """World::create_revolute_joint (revolute_joint.rs:149-174). `d` has the fields of
+        RevoluteJointDef (natsel_amd.scenes.JointSpec): body_a, body_b, frame_a/frame_b as
+        (px, py, pz, qx, qy, qz, qs), constraint_hertz, constraint_damping_ratio,
+        collide_connected, and the spring, limit and motor settings."""
+        c = capi.RevoluteJointDef()
+        c.body_a, c.body_b = int(d.body_a), int(d.body_b)
+        c.local_frame_a[:] = d.frame_a
+        c.local_frame_b[:] = d.frame_b
+        c.constraint_hertz, c.constraint_damping_ratio = d.constraint_hertz, d.constraint_damping_ratio
+        c.collide_connected = int(d.collide_connected)
+        c.enable_spring, c.hertz, c.damping_ratio, c.target_angle = int(d.enable_spring), d.hertz, d.damping_ratio, d.target_angle
+        c.enable_limit, c.lower_angle, c.upper_angle = int(d.enable_limit), d.lower_angle, d.upper_angle
+        c.enable_motor, c.max_motor_torque, c.motor_speed = int(d.enable_motor), d.max_motor_torque, d.motor_speed
+        out = ctypes.c_int32()
+        capi.check(self._lib.nsl_create_revolute_joint(self._h, ctypes.byref(c), ctypes.byref(out)))
+        return out.value
+
+    def joint(self, joint_id: int):
+        """(impulses f32[9]: linear xyz, perpendicular xy, spring, motor, lower, upper; graph colour)."""
+        out = np.zeros(9, np.float32)
+        color = ctypes.c_int32()
+        capi.check(self._lib.nsl_world_get_joint(self._h, joint_id, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                 ctypes.byref(color)))
+        return out, color.value
+
+    def contact_count(self) -> int:
+        out = ctypes.c_int32()
+        capi.check(self._lib.nsl_world_contact_count(self._h, ctypes.byref(out)))
+        return out.value
+
     # --- stepping and queries ----------------------------------------------------------
     def step(self, time_step: float, sub_step_count: int) -> None:
         capi.check(self._lib.nsl_world_step(self._h, time_step, sub_step_count))
@@ -240,4 +272,6 @@ def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0) -> World:
             w.create_sphere_shape(ids[s.body], d, Sphere(tuple(s.params[:3]), s.params[3]))
         else:
             w.create_capsule_shape(ids[s.body], d, Capsule(tuple(s.params[:3]), tuple(s.params[3:6]), s.params[6]))
+    for j in scene.joints:
+        w.create_revolute_joint(dataclasses.replace(j, body_a=ids[j.body_a], body_b=ids[j.body_b]))
     return w
