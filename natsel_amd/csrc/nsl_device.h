@@ -81,6 +81,14 @@ struct SolvePlan {
   ColorRanges cr;                 // coloured slots per colour (colour-major)
   ColorRanges jr;                 // joint_ids per colour, overflow colour included
   int32_t n_joints;               // joints in the graph (prepared each step)
+  // The overflow colour's joints and contacts in dependency levels: an item's level is one
+  // more than the highest level of the earlier items (joints first, then contacts, in list
+  // order) that share a dynamic body with it. Items of a level touch disjoint bodies, so
+  // running the levels in order with the items of each in parallel reproduces the
+  // sequential walk exactly. Items: contact k >= 0 (overflow[k]), joint id -(id + 1).
+  const int32_t* ovf_items;
+  const int32_t* ovf_level_begin; // [n_ovf_levels + 1]
+  int32_t n_ovf_levels;
   int32_t n_constraints;          // coloured slots in total
   int32_t n_overflow;             // overflow constraints, solved in list order by one lane
   int32_t any_restitution;

@@ -175,6 +175,22 @@ __device__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid,
   return true;
 }
 
+// The overflow colour's levels, run by workgroup 0 (one workgroup barrier per level; the
+// workgroup shares one L1, so its stores are visible to its loads after the barrier).
+template <class JointItem, class Item>
+__device__ void run_overflow_levels(const SolvePlan& plan, JointItem joint, Item item) {
+  if (blockIdx.x != 0) return;
+  for (int l = 0; l < plan.n_ovf_levels; ++l) {
+    const int b = plan.ovf_level_begin[l], e = plan.ovf_level_begin[l + 1];
+    for (int i = b + (int)threadIdx.x; i < e; i += (int)blockDim.x) {
+      const int32_t it = plan.ovf_items[i];
+      if (it >= 0) item(it);
+      else joint(-it - 1);
+    }
+    __syncthreads();
+  }
+}
+
 // Stage schedule of one step (solver_stages.rs:543-589, solver_integration.rs:807-1015):
 // prepare; per substep: integrate velocities, warm start, push, integrate positions,
 // relax, each stage running the overflow list first and then colours 0..22 in order;
@@ -184,7 +200,6 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
   const int n_ovf = plan.n_overflow;
-  const bool lane0 = tid == 0;
   GridBarrier* bar = plan.barrier;
   PhaseClock clock{plan.phase_log, 0};
   clock.mark(-1);
@@ -203,9 +218,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     colours.n = n;
   }
   __syncthreads();
-  // overflow-colour joints, solved by one lane ahead of the overflow contacts
-  const int ovf_jb = plan.jr.begin[kOverflowColor], ovf_je = plan.jr.end[kOverflowColor];
-  const bool any_ovf = n_ovf > 0 || ovf_je > ovf_jb;
+  const bool any_ovf = plan.n_ovf_levels > 0;
   auto no_joint = [](int32_t) {};
 
   {
@@ -225,10 +238,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     clock.mark(kPIntegrateVelocities);
 
     if (any_ovf) {
-      if (lane0) {
-        for (int k = ovf_jb; k < ovf_je; ++k) warm_start_joint_item(d, d.joint_ids[k]);
-        overflow_warm_all(d, n_ovf);
-      }
+      run_overflow_levels(plan, [&](int32_t jid) { warm_start_joint_item(d, jid); }, [&](int k) { overflow_warm_item(d, k); });
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
@@ -238,10 +248,8 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       return;
 
     if (any_ovf) {
-      if (lane0) {
-        for (int k = ovf_jb; k < ovf_je; ++k) solve_joint_item(d, d.joint_ids[k], p, true);
-        overflow_solve_all(d, n_ovf, p, 0);
-      }
+      run_overflow_levels(plan, [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
+                          [&](int k) { overflow_solve_item(d, k, p, 0); });
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
@@ -255,10 +263,8 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     clock.mark(kPIntegratePositions);
 
     if (any_ovf) {
-      if (lane0) {
-        for (int k = ovf_jb; k < ovf_je; ++k) solve_joint_item(d, d.joint_ids[k], p, false);
-        overflow_solve_all(d, n_ovf, p, 1);
-      }
+      run_overflow_levels(plan, [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
+                          [&](int k) { overflow_solve_item(d, k, p, 1); });
       if (!grid_sync(bar)) return;
       clock.mark(kPOverflow);
     }
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   if (plan.any_restitution) {
     for (int it = 0; it < plan.restitution_iterations; ++it) {
       if (n_ovf > 0) {
-        if (lane0) overflow_solve_all(d, n_ovf, p, 2);
+        run_overflow_levels(plan, no_joint, [&](int k) { overflow_solve_item(d, k, p, 2); });
         if (!grid_sync(bar)) return;
         clock.mark(kPOverflow);
       }

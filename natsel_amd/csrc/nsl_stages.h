@@ -742,9 +742,9 @@ __device__ __forceinline__ void overflow_store_velocity(const DeviceWorld& d, in
   }
 }
 
-// One lane walks the whole list.
-__device__ void overflow_warm_all(const DeviceWorld& d, int n) {
-  for (int k = 0; k < n; ++k) {
+// Warm start of overflow constraint k.
+__device__ inline void overflow_warm_item(const DeviceWorld& d, int k) {
+  {
     const OverflowConstraint& c = d.overflow[k];
     Vec3 linear = zero3(), ang_a = zero3(), ang_b = zero3();
     if (c.manifold_count > 0) {
@@ -778,12 +778,12 @@ __device__ void overflow_warm_all(const DeviceWorld& d, int n) {
   }
 }
 
-// mode 0: push (biased), 1: relax, 2: restitution
-__device__ void overflow_solve_all(const DeviceWorld& d, int n, const StepParams& p, int mode) {
+// Overflow constraint k; mode 0: push (biased), 1: relax, 2: restitution.
+__device__ inline void overflow_solve_item(const DeviceWorld& d, int k, const StepParams& p, int mode) {
   const float inv_h = p.inv_h;
-  for (int k = 0; k < n; ++k) {
+  {
     OverflowConstraint& c = d.overflow[k];
-    if (mode == 2 && p.enable_restitution_propagation == 0 && c.restitution == 0.0f) continue;
+    if (mode == 2 && p.enable_restitution_propagation == 0 && c.restitution == 0.0f) return;
     float mA = c.inv_mass_a, mB = c.inv_mass_b;
     Mat3 i_a = c.inv_ia, i_b = c.inv_ib;
     BodyState sa = load_state(d, c.index_a), sb = load_state(d, c.index_b);
@@ -919,6 +919,15 @@ __device__ void overflow_solve_all(const DeviceWorld& d, int n, const StepParams
     overflow_store_velocity(d, c.index_a, sa.flags, v_a, w_a);
     overflow_store_velocity(d, c.index_b, sb.flags, v_b, w_b);
   }
+}
+
+// The overflow list in order by one lane (overflow_contact_solver.rs: sequential).
+__device__ void overflow_warm_all(const DeviceWorld& d, int n) {
+  for (int k = 0; k < n; ++k) overflow_warm_item(d, k);
+}
+
+__device__ void overflow_solve_all(const DeviceWorld& d, int n, const StepParams& p, int mode) {
+  for (int k = 0; k < n; ++k) overflow_solve_item(d, k, p, mode);
 }
 
 __device__ __forceinline__ void overflow_store_item(const DeviceWorld& d, int k) {

@@ -161,6 +161,12 @@ class GpuWorld : public HostScene {
   std::vector<int32_t> joint_ids_host_;
   ColorRanges joint_ranges_{};
   bool joints_dirty_ = false;
+  // overflow dependency levels (rebuilt each step)
+  std::vector<int32_t> ovf_items_host_, ovf_level_host_, body_level_;
+  int32_t* d_ovf_items_ = nullptr;
+  int32_t* d_ovf_levels_ = nullptr;
+  size_t cap_ovf_items_ = 0, cap_ovf_levels_ = 0;
+  int32_t build_overflow_levels();
 };
 
 }  // namespace nsl

@@ -50,7 +50,8 @@ GpuWorld::~GpuWorld() {
   void* ptrs[] = {d_.sims, d_.states, d_.static_sims, d_.shapes, d_.fat_aabbs, d_.shape_moved, d_.hulls, d_.contacts,
                   d_.contact_change, d_.constraints, d_.overflow, d_.collide_ids, d_.solve_ids, d_.overflow_ids,
                   d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
-                  d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_};
+                  d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
+                  d_ovf_items_, d_ovf_levels_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -474,6 +475,51 @@ void GpuWorld::get_joint(int32_t id, float* impulses, int32_t* color) {
   if (color) *color = hjoints_[id].color;
 }
 
+// The overflow colour's joints (first) and contacts in list order, grouped into dependency
+// levels through their dynamic bodies; uploads items and level offsets, returns the count
+// of levels.
+int32_t GpuWorld::build_overflow_levels() {
+  const std::vector<int32_t>& ovf_joints = colors_[kOverflowColor].joints;
+  const std::vector<int32_t>& ovf = colors_[kOverflowColor].overflow;
+  const size_t n = ovf_joints.size() + ovf.size();
+  if (n == 0) return 0;
+  if (body_level_.size() < bodies_.size()) body_level_.resize(bodies_.size(), 0);
+  std::vector<int32_t> level(n);
+  std::vector<int32_t> touched;
+  int32_t n_levels = 0;
+  for (size_t i = 0; i < n; ++i) {
+    int32_t a, b;
+    if (i < ovf_joints.size()) {
+      a = hjoints_[ovf_joints[i]].body_a;
+      b = hjoints_[ovf_joints[i]].body_b;
+    } else {
+      const HContact& c = hcontacts_[ovf[i - ovf_joints.size()]];
+      a = hshapes_[c.shape_a].body;
+      b = hshapes_[c.shape_b].body;
+    }
+    const bool da = bodies_[a].type == kBodyDynamic, db = bodies_[b].type == kBodyDynamic;
+    const int32_t l = std::max(da ? body_level_[a] : 0, db ? body_level_[b] : 0);  // body_level_: last level + 1
+    level[i] = l;
+    n_levels = std::max(n_levels, l + 1);
+    if (da) { body_level_[a] = l + 1; touched.push_back(a); }
+    if (db) { body_level_[b] = l + 1; touched.push_back(b); }
+  }
+  for (int32_t t : touched) body_level_[t] = 0;
+  ovf_level_host_.assign(n_levels + 1, 0);
+  for (size_t i = 0; i < n; ++i) ovf_level_host_[level[i] + 1] += 1;
+  for (int32_t l = 0; l < n_levels; ++l) ovf_level_host_[l + 1] += ovf_level_host_[l];
+  ovf_items_host_.assign(n, 0);
+  std::vector<int32_t> fill(ovf_level_host_.begin(), ovf_level_host_.end() - 1);
+  for (size_t i = 0; i < n; ++i)
+    ovf_items_host_[fill[level[i]]++] = i < ovf_joints.size() ? -(ovf_joints[i] + 1) : (int32_t)(i - ovf_joints.size());
+  ensure(d_ovf_items_, cap_ovf_items_, n, false);
+  ensure(d_ovf_levels_, cap_ovf_levels_, ovf_level_host_.size(), false);
+  NSL_CHECK(hipMemcpyAsync(d_ovf_items_, ovf_items_host_.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+  NSL_CHECK(hipMemcpyAsync(d_ovf_levels_, ovf_level_host_.data(), ovf_level_host_.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, stream_));
+  return n_levels;
+}
+
 // ---------------------------------------------------------------------------------
 // one step: solver_integration.rs:807-1015
 
@@ -590,6 +636,9 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   plan.cr = cr;
   plan.jr = joint_ranges_;
   plan.n_joints = (int32_t)joint_ids_host_.size();
+  plan.n_ovf_levels = build_overflow_levels();
+  plan.ovf_items = d_ovf_items_;
+  plan.ovf_level_begin = d_ovf_levels_;
   plan.n_constraints = (int32_t)solve.size();
   plan.n_overflow = (int32_t)ovf.size();
   plan.any_restitution = any_restitution ? 1 : 0;

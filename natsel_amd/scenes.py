@@ -592,3 +592,28 @@ def large_world(bodies: int = 100_000, chains: int = 2000, links: int = 16, seed
     sc = chain_pile(bodies, chains, links, seed, 50, 60, 50)
     sc.name = f"large_world_{bodies}_{chains}x{links}"
     return sc
+
+
+def joint_hub(satellites: int = 24) -> Scene:
+    """A dynamic hub box with `satellites` small boxes hinged around it on a ring and two
+    boxes dropped on top, over the ground: the hub's first 20 joints take colours 0..19 and
+    the rest overflow (constraint_graph.rs:256-298), and so do the contacts of the boxes on
+    top with the hub, so the overflow colour holds joints and contacts that share a body."""
+    sc = Scene(name=f"joint_hub_{satellites}", enable_sleep=False)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -1.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (20.0, 1.0, 20.0)))
+    hub = sc.add_body(BodySpec(DYNAMIC, (0.0, 0.5, 0.0)))
+    sc.add_shape(ShapeSpec(hub, BOX, (0.5, 0.5, 0.5)))
+    for x in (-0.25, 0.25):
+        b = sc.add_body(BodySpec(DYNAMIC, (x, 1.3, 0.0)))
+        sc.add_shape(ShapeSpec(b, BOX, (0.2, 0.2, 0.2)))
+    for i in range(satellites):
+        c, s = compute_cos_sin(f32(2.0 * np.pi * i / satellites))
+        p = (float(f32(1.5) * c), 0.3, float(f32(1.5) * s))
+        b = sc.add_body(BodySpec(DYNAMIC, p))
+        sc.add_shape(ShapeSpec(b, BOX, (0.2, 0.2, 0.2)))
+        # hinge point on the hub's rim, 0.2 m below its centre, as seen from each body
+        rim = (float(f32(0.6) * c), -0.2, float(f32(0.6) * s))
+        from_b = (float(f32(rim[0]) - f32(p[0])), 0.0, float(f32(rim[2]) - f32(p[2])))
+        sc.add_joint(JointSpec(hub, b, frame_a=_frame(rim), frame_b=_frame(from_b)))
+    return sc

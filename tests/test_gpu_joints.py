@@ -136,3 +136,12 @@ def test_chain_pile_workgroup_count_does_not_change_results(gpu_available, block
     for _ in range(120):
         tw.step(jc.TIME_STEP, jc.SUB_STEPS)
     tw.check_joints()
+
+
+def test_joint_hub_overflow_every_step(gpu_available):
+    """Overflow joints and overflow contacts sharing the hub body: the dependency levels
+    must reproduce the sequential overflow walk bit for bit."""
+    tw = Twin(scenes.joint_hub())
+    for _ in range(120):
+        tw.step(jc.TIME_STEP, jc.SUB_STEPS)
+    tw.check_joints()

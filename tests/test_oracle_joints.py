@@ -103,3 +103,14 @@ def test_chain_joint_colours_alternate():
     w = _load(scenes.hanging_chain(6))
     colours = [w.joint(j)[1] for j in range(6)]
     assert colours == [0, 1, 0, 1, 0, 22]
+
+
+def test_joint_hub_overflows_joints_and_contacts():
+    """A hub jointed to 24 bodies: joints 0..19 take colours 0..19, the rest overflow; the
+    two boxes resting on the hub overflow too (constraint_graph.rs:113-197, :256-298)."""
+    w = _load(scenes.joint_hub())
+    assert [w.joint(j)[1] for j in range(24)] == list(range(20)) + [23] * 4
+    for _ in range(60):
+        w.step(jc.TIME_STEP, jc.SUB_STEPS)
+    color, _, alive = w.contact_colors()
+    assert int(((color == 23) & (alive == 1)).sum()) == 2
