@@ -31,13 +31,18 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Algorithmic bytes per item: distinct fields the reference formula reads plus writes,
 # f32 = 4 B, each body-state gather counted per constraint touch. Coloured constraint
 # stages: per constraint (one manifold) + per manifold point.
+# Revolute joints (per joint, same rule): prepare reads two bodies' pose, centre, mass and
+# world inertia (2 x 92) and the definition (104) and writes the prepared rows (224); warm
+# start reads masses, inertias, anchors, axes and impulses (184) and both states (128) and
+# writes two velocities (48); push and relax read the prepared rows and impulses (224) and
+# both states and write the impulses and axes (60) and two velocities.
 BYTE_MODEL = {
-    "prepare": {"per_constraint": 268 + 452, "per_point": 0},
+    "prepare": {"per_constraint": 268 + 452, "per_point": 0, "per_joint": 184 + 104 + 224},
     "integrate_velocities": {"per_body": 100 + 24},
-    "warm_start": {"per_constraint": 64 + 72 + 56 + 48, "per_point": 28},
-    "push": {"per_constraint": 68 + 12 + 112 + 48, "per_point": 40},
+    "warm_start": {"per_constraint": 64 + 72 + 56 + 48, "per_point": 28, "per_joint": 184 + 128 + 48},
+    "push": {"per_constraint": 68 + 12 + 112 + 48, "per_point": 40, "per_joint": 224 + 60 + 128 + 48},
     "integrate_positions": {"per_body": 56 + 56},
-    "relax": {"per_constraint": 68 + 108 + 112 + 48, "per_point": 52},
+    "relax": {"per_constraint": 68 + 108 + 112 + 48, "per_point": 52, "per_joint": 224 + 60 + 128 + 48},
     "store": {"per_constraint": 76, "per_point": 16},
     "finalize": {"per_body": 400},
     # contact record read and written (~300 B each way), both bodies' poses (2 x 64 B),
@@ -55,7 +60,8 @@ def stage_bytes(stage: str, ks: dict, substeps: int) -> float:
         return reps * m["per_body"] * ks["bodies"]
     if "per_contact" in m:
         return m["per_contact"] * ks["collided"]
-    return reps * (m["per_constraint"] * ks["constraints"] + m["per_point"] * ks["points"])
+    return reps * (m["per_constraint"] * ks["constraints"] + m["per_point"] * ks["points"] +
+                   m.get("per_joint", 0) * ks.get("joints", 0))
 
 
 def step_bytes(counts: dict, substeps: int) -> float:
@@ -161,6 +167,128 @@ def batched_measurement(worlds: int, steps: int) -> dict:
     }
 
 
+def large_world_measurement(steps: int) -> dict:
+    """The large single world: 100,000 mixed bodies in a seeded pile plus 2,000 chains of 16
+    capsule links (30,000 revolute joints), steps 1..`steps` from the scene's start. One run
+    for the rate, a second with the step kernel's phase clock on for the per-stage
+    algorithmic bytes / time (HBM GB/s by the byte model)."""
+    import torch
+
+    from natsel_amd import scenes
+    from natsel_amd.world import load_scene
+
+    sc = scenes.large_world()
+    n = sc.dynamic_count
+    runs = []
+    for timing in (False, True):
+        world = load_scene(sc)
+        world.step(sc.dt, sc.substeps)  # uploads the world
+        world.reset_kernel_stats()
+        world.set_kernel_timing(timing)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            world.step(sc.dt, sc.substeps)
+        torch.cuda.synchronize()
+        runs.append((time.perf_counter() - t0, world.kernel_stats()))
+        world.close()
+    elapsed, _ = runs[0]
+    ks = runs[1][1]
+    stage_ms = {
+        "prepare": ks["phase_ms"]["prepare"],
+        "integrate_velocities": ks["phase_ms"]["integrate_velocities"],
+        "warm_start": ks["phase_ms"]["warm_start"],
+        "push": ks["phase_ms"]["push"],
+        "integrate_positions": ks["phase_ms"]["integrate_positions"],
+        "relax": ks["phase_ms"]["relax"],
+        "collide": ks["ms"]["collide"],
+    }
+    per_stage = {}
+    for st, ms in stage_ms.items():
+        b = stage_bytes(st, ks, sc.substeps)
+        per_stage[st] = {"ms_per_step": ms / steps, "gb_per_step": b / steps / 1e9, "gbs": b / (ms / 1e3) / 1e9 if ms > 0 else None}
+    sf_bytes = stage_bytes("store", ks, sc.substeps) + stage_bytes("finalize", ks, sc.substeps)
+    sf_ms = ks["phase_ms"]["store_finalize"]
+    per_stage["store_finalize"] = {"ms_per_step": sf_ms / steps, "gb_per_step": sf_bytes / steps / 1e9,
+                                   "gbs": sf_bytes / (sf_ms / 1e3) / 1e9 if sf_ms > 0 else None}
+    return {
+        "metric": "body-substeps/s (large world)",
+        "value": n * sc.substeps * steps / elapsed,
+        "unit": "body-substeps/s",
+        "ms_per_step": 1e3 * elapsed / steps,
+        "steps": steps,
+        "config": {"workload": f"{n} dynamic bodies: a 100,000-body pile (boxes/spheres/capsules, 50 x 60 x 50 m) and "
+                               f"{len(sc.joints) // 15} chains of 16 capsule links ({len(sc.joints)} revolute joints), dt 1/60, "
+                               "4 substeps, sleep and continuous collision off", "bodies": n, "joints": len(sc.joints)},
+        "per_stage_hbm": per_stage,
+        "overflow_ms_per_step": ks["phase_ms"]["overflow"] / steps,
+        "host_ms_per_step": {k: v / steps for k, v in ks["host_ms"].items()},
+        "points_per_step": ks["points"] / max(ks["steps"], 1),
+        "constraints_per_step": ks["constraints"] / max(ks["steps"], 1),
+        "note": "per-stage figures from the run with the phase clock on; value from the run without it",
+    }
+
+
+def adjoint_measurement(worlds: int, steps: int, fd_worlds: int = 16) -> dict:
+    """Differentiable rollout: `worlds` worlds of 32 cubes pushed across the ground
+    (natsel_amd.scenes.pushed_boxes), `steps` steps; loss |com(T) - target|^2; gradients
+    w.r.t. every body's initial velocity, angular velocity and friction. Forward = the
+    batched stepper's device time for the steps; forward+adjoint = recording the same steps
+    plus the adjoint. Gradient check: central differences (epsilon 1e-5) of the same model in
+    double precision on `fd_worlds` seeded worlds, with the recorded active set (frozen) and
+    with the model deciding its own branches (free)."""
+    import numpy as np
+
+    from natsel_amd import scenes
+    from natsel_amd.batch import load_batch
+
+    sc = [scenes.pushed_boxes(1000 + k) for k in range(worlds)]
+    targets = scenes.pushed_box_targets(worlds)
+    dt, sub = sc[0].dt, sc[0].substeps
+    fwd = load_batch(sc)
+    fwd.step(dt, sub, steps)
+    fwd_ms = fwd.stats()["last_step_ms"]
+    fwd.close()
+    rb = load_batch(sc)
+    rb.record(dt, sub, steps)
+    r = rb.adjoint(targets)
+    rec_ms, adj_ms = rb.rollout_times()
+    offs = np.cumsum([0] + [s.dynamic_count for s in sc])
+    sample = np.sort(np.random.default_rng(5).choice(worlds, min(fd_worlds, worlds), replace=False))
+    wl = np.repeat(sample, 32 * 7)
+    pl = np.tile(np.arange(32 * 7), len(sample))
+    g = np.concatenate([r["grad_v"], r["grad_w"], r["grad_friction"][:, None]], axis=1)
+    adj = np.array([g[offs[w] + p // 7, p % 7] for w, p in zip(wl, pl)])
+    eps = 1e-5
+    check = {"epsilon": eps, "worlds_sampled": len(sample), "parameters_per_world": 32 * 7}
+    for mode in ("frozen", "free"):
+        lp = rb.model_losses(np.concatenate([wl, wl]), np.concatenate([pl, pl]),
+                             np.concatenate([np.full(len(wl), eps), np.full(len(wl), -eps)]), targets, frozen=mode == "frozen")
+        fd = (lp[:len(wl)] - lp[len(wl):]) / (2 * eps)
+        errs, within = [], []
+        for w in sample:
+            m = wl == w
+            scale = np.abs(fd[m]).max()
+            errs.append(float(np.abs(adj[m] - fd[m]).max() / scale))
+            within.append(np.abs(adj[m] - fd[m]) <= 1e-3 * scale)
+        check[mode] = {"max_rel_error": max(errs), "median_rel_error": float(np.median(errs)),
+                       "components_within_1e-3": float(np.concatenate(within).mean())}
+    rb.close()
+    fs, fas = fwd_ms / 1e3, (rec_ms + adj_ms) / 1e3
+    return {
+        "metric": "world-steps/s (differentiable rollout: forward vs forward+adjoint)",
+        "forward": worlds * steps / fs,
+        "forward_adjoint": worlds * steps / fas,
+        "unit": "world-steps/s",
+        "fwd_adjoint_over_fwd": fas / fs,
+        "ms": {"forward": fwd_ms, "record": rec_ms, "adjoint": adj_ms},
+        "config": {"workload": f"{worlds} worlds x 32 cubes pushed across the ground, {steps} steps, dt 1/60, 4 substeps; "
+                               "loss |com - target|^2; gradients w.r.t. v0, w0, friction", "worlds": worlds, "steps": steps},
+        "status_nonzero_worlds": int((r["status"] != 0).sum()),
+        "gradient_check": check,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +300,9 @@ def main() -> int:
     ap.add_argument("--blocks", type=int, default=0, help="workgroups of the step kernel (0: sized from the largest phase)")
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched measurement (0: skip it)")
     ap.add_argument("--batched-steps", type=int, default=240)
+    ap.add_argument("--large-world-steps", type=int, default=500, help="steps of the large-world measurement (0: skip it)")
+    ap.add_argument("--adjoint-worlds", type=int, default=1024, help="worlds of the differentiable rollout (0: skip it)")
+    ap.add_argument("--adjoint-steps", type=int, default=64)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (reference kernel for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
@@ -271,6 +402,10 @@ def main() -> int:
         out["cpu_baseline"] = cb
     if args.batched_worlds > 0:
         out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps)
+    if args.adjoint_worlds > 0:
+        out["adjoint"] = adjoint_measurement(args.adjoint_worlds, args.adjoint_steps)
+    if args.large_world_steps > 0:
+        out["large_world"] = large_world_measurement(args.large_world_steps)
     print(json.dumps(out))
     return 0
 

@@ -105,6 +105,7 @@ typedef struct nsl_kernel_stats {
   int64_t solver_blocks; /* workgroups of the last step kernel launch */
   int64_t pair_tests;    /* candidate shape pairs tested by the device pair finder */
   double host_ms[4];     /* host wall time: pair update, collide (incl. wait), state pass, solve+finalize (incl. wait) */
+  int64_t joints;        /* joints solved, summed over steps */
 } nsl_kernel_stats;
 
 /* A revolute joint (RevoluteJointDef, src/types/joint_defs.rs:30-58, :154-167). Frames
@@ -216,6 +217,37 @@ int nsl_batch_get_contact_shapes(nsl_batch* batch, int32_t world, int32_t* shape
  * pair update. */
 int nsl_batch_get_shape_bounds(nsl_batch* batch, int32_t world, float* fat, uint8_t* moved);
 int nsl_batch_get_stats(nsl_batch* batch, nsl_batch_stats* out);
+
+/* Differentiable rollout of a batch. The reference has no differentiable path; these
+ * entry points are for gradient-based callers (trajectory optimisation, system
+ * identification) of the batched stepper.
+ *
+ * nsl_batch_record steps like nsl_batch_step while recording a tape of every step (at most
+ * max_constraints <= 64 contact constraints and 64 dynamic bodies per world).
+ * nsl_batch_adjoint then returns, per world, the loss |com - target|^2 of the dynamic bodies'
+ * centre of mass after the recorded steps (targets: [worlds][3]) and, per dynamic body
+ * (arrays in the batch's body order, world after world), the gradient of its world's loss
+ * with respect to the body's linear and angular velocity at the first recorded step
+ * ([bodies][3] each) and its shape's friction ([bodies]). The differentiated model holds the
+ * recorded active set, colour order and contact geometry fixed, lets each point's separation
+ * follow the body centres and carries warm-start impulses between steps as the stepper does.
+ * status per world: 0 ok; 1 a recorded step lies outside the model (overflow colour,
+ * restitution, rolling resistance, capacity); 2 the adjoint's recompute departed from the
+ * recording.
+ * nsl_batch_model_losses runs the same model in double precision from the recorded first
+ * state for n cases, each with one parameter of one world perturbed by deltas[i]
+ * (params[i] = awake body index in its world * 7 + {0..2: v, 3..5: w, 6: friction}), and
+ * returns the loss of each case: central finite differences of the adjoint's model. With
+ * frozen != 0 every clamp, cone and separation branch is taken from the branch record of the
+ * last nsl_batch_adjoint (the model with the recorded active set, which the adjoint
+ * differentiates); with frozen = 0 the model decides its own branches. */
+int nsl_batch_record(nsl_batch* batch, float dt, int32_t sub_steps, int32_t steps, int32_t max_constraints);
+int nsl_batch_adjoint(nsl_batch* batch, const float* targets, float* loss, float* grad_v, float* grad_w, float* grad_friction,
+                      int32_t* status);
+int nsl_batch_model_losses(nsl_batch* batch, const int32_t* worlds, const int32_t* params, const double* deltas, int32_t n,
+                           const float* targets, int32_t frozen, double* out);
+/* Device time of the last nsl_batch_record and nsl_batch_adjoint calls. */
+int nsl_batch_rollout_times(nsl_batch* batch, double* record_ms, double* adjoint_ms);
 
 #ifdef __cplusplus
 }

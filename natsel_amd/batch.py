@@ -136,6 +136,48 @@ class Batch:
                                                         moved.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return fat, moved
 
+    # --- differentiable rollout ---------------------------------------------------------
+    def record(self, time_step: float, sub_step_count: int, steps: int, max_constraints: int = 64) -> None:
+        """Step `steps` steps like step() while recording the tape for adjoint()."""
+        capi.check(self._lib.nsl_batch_record(self._h, time_step, sub_step_count, steps, max_constraints))
+
+    def adjoint(self, targets: np.ndarray) -> dict:
+        """Loss |com - target|^2 per world after the recorded steps and its gradients per
+        dynamic body (rows in the batch's body order: each world's dynamic bodies in creation
+        order, world after world) w.r.t. the first step's velocity, angular velocity and the
+        body's friction."""
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        n = int(self.stats()["bodies"])
+        loss = np.zeros(len(targets), np.float32)
+        gv, gw = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        gf = np.zeros(n, np.float32)
+        status = np.zeros(len(targets), np.int32)
+        f = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+        capi.check(self._lib.nsl_batch_adjoint(self._h, f(targets), f(loss), f(gv), f(gw), f(gf),
+                                               status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return {"loss": loss, "grad_v": gv, "grad_w": gw, "grad_friction": gf, "status": status}
+
+    def model_losses(self, worlds, params, deltas, targets: np.ndarray, frozen: bool = False) -> np.ndarray:
+        """The adjoint's model in double precision, one perturbed parameter per case
+        (param = body * 7 + {0..2: v, 3..5: w, 6: friction}); `frozen` takes every branch
+        from the last adjoint()'s record."""
+        worlds = np.ascontiguousarray(worlds, np.int32)
+        params = np.ascontiguousarray(params, np.int32)
+        deltas = np.ascontiguousarray(deltas, np.float64)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        out = np.zeros(len(worlds), np.float64)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+        capi.check(self._lib.nsl_batch_model_losses(self._h, ip(worlds), ip(params), dp(deltas), len(worlds),
+                                                    targets.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(frozen), dp(out)))
+        return out
+
+    def rollout_times(self) -> tuple:
+        """Device milliseconds of the last record() and adjoint() calls."""
+        r, a = ctypes.c_double(), ctypes.c_double()
+        capi.check(self._lib.nsl_batch_rollout_times(self._h, ctypes.byref(r), ctypes.byref(a)))
+        return r.value, a.value
+
     def stats(self) -> dict:
         s = capi.BatchStats()
         capi.check(self._lib.nsl_batch_get_stats(self._h, ctypes.byref(s)))

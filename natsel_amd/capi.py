@@ -98,7 +98,7 @@ class KernelStats(ctypes.Structure):
         (n, ctypes.c_int64)
         for n in ("steps", "bodies", "collided", "constraints", "points", "overflow_points", "recycled", "sat_calls",
                   "sat_hits", "fast_bodies", "barriers", "solver_blocks", "pair_tests")
-    ] + [("host_ms", ctypes.c_double * len(HOST_PHASES))]
+    ] + [("host_ms", ctypes.c_double * len(HOST_PHASES)), ("joints", ctypes.c_int64)]
 
 
 class BatchStats(ctypes.Structure):
@@ -166,6 +166,12 @@ SIGNATURES = {
         [_P, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), _F, ctypes.POINTER(Material), ctypes.POINTER(_I)],
     ),
     "nsl_batch_step": (ctypes.c_int, [_P, _F, _I, _I]),
+    "nsl_batch_record": (ctypes.c_int, [_P, _F, _I, _I, _I]),
+    "nsl_batch_adjoint": (ctypes.c_int, [_P, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F),
+                                         ctypes.POINTER(_F), ctypes.POINTER(_I)]),
+    "nsl_batch_model_losses": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double), _I,
+                                              ctypes.POINTER(_F), _I, ctypes.POINTER(ctypes.c_double)]),
+    "nsl_batch_rollout_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "nsl_batch_world_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "nsl_batch_body_count": (ctypes.c_int, [_P, _I, ctypes.POINTER(_I)]),
     "nsl_batch_state_hashes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),

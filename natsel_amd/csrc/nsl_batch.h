@@ -67,6 +67,27 @@ struct ContactMeta {  // per contact id: shapes, graph colour and index, livenes
   float restitution;
 };
 
+// Recording of a batched rollout for its adjoint (nsl_adjoint.hip): per step and world
+// the prepared constraints in solve order (warm-start impulses included), their points'
+// feature ids and shape pair, the links to the previous step's constraints that carry
+// impulses, the colour ranges, and the body states and centres at the step's start.
+constexpr int kTapeColourWords = 1 + 2 * kOverflowColor;
+constexpr int kTapeBranchBytes = 10;  // push points 0..3, relax points 0..3, twist, cone
+constexpr uint8_t kTapeNoPoint = 0xFF;
+struct BatchTape {
+  Constraint* cons;     // [steps][worlds][max_con]
+  uint32_t* features;   // [steps][worlds][max_con][4]
+  uint32_t* pairs;      // [steps][worlds][max_con]: (shape_a << 16) | shape_b
+  int32_t* links;       // [steps][worlds][max_con]: same contact's slot in the previous step, or -1
+  uint8_t* pmap;        // [steps][worlds][max_con][4]: the matching previous point per point (kTapeNoPoint: none)
+  int32_t* count;       // [steps][worlds]: constraints; -1: the step is outside the model
+  int32_t* colours;     // [steps][worlds][kTapeColourWords]: n, begin[n], end[n] of the non-empty colours
+  BodyState* states;    // [steps + 1][worlds][max_bodies]
+  Pos* centers;         // [steps + 1][worlds][max_bodies]
+  uint8_t* branches;    // [steps][worlds][sub_steps][max_con][kTapeBranchBytes]: the adjoint's recompute's branch record
+  int32_t max_con, max_bodies, steps, n_worlds, sub_steps;
+};
+
 // Device arrays of a batch, passed by value to the kernel.
 struct BatchDevice {
   BodySim* sims;
@@ -99,6 +120,8 @@ struct BatchDevice {
   int32_t n_worlds;
   int32_t enable_speculative;
   int32_t restitution_iterations;
+  BatchTape tape;
+  int32_t tape_step;  // step of the tape to record this launch (-1: not recording)
 };
 
 void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hipStream_t s);
@@ -142,10 +165,29 @@ class BatchStepper {
   // overflow constraints.
   void work_counts(int64_t out[4]);
 
+  // Differentiable rollout (nsl_adjoint.hip). record() steps like step() while writing the
+  // tape (at most max_constraints constraints and kBatchLanes dynamic bodies per world).
+  // adjoint() returns per world the loss |com - target|^2 of the dynamic bodies' centre of
+  // mass after the recorded steps, and its gradient with respect to each dynamic body's
+  // velocity and angular velocity at the first recorded step and its shape's friction; a
+  // world's status is 0, or 1 when a recorded step lies outside the differentiable model,
+  // 2 when the recompute departed from the recording. model_losses() runs the model in
+  // double precision from the tape with one parameter perturbed per case (parameter
+  // index = awake body * 7 + {0..2: v, 3..5: w, 6: friction}), for finite differences.
+  void record(float dt, int32_t sub_steps, int32_t steps, int32_t max_constraints);
+  void adjoint(const float* targets, float* loss, float* grad_v, float* grad_w, float* grad_friction, int32_t* status);
+  void model_losses(const int32_t* worlds, const int32_t* params, const double* deltas, int32_t n, const float* targets,
+                    double* out, bool frozen);
+  float last_record_ms() const { return record_ms_; }
+  float last_adjoint_ms() const { return adjoint_ms_; }
+  int32_t tape_steps() const { return tape_.steps; }
+
  private:
   void upload();
   void pull();
   template <class T> T* alloc(size_t n, bool zero);
+  StepParams step_params(float dt, int32_t sub_steps) const;
+  void free_tape();
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -160,6 +202,12 @@ class BatchStepper {
   int64_t body_steps_ = 0;
   int64_t total_awake_ = 0;
   int64_t total_contact_cap_ = 0;
+  BatchTape tape_{};
+  StepParams tape_params_{};
+  std::vector<void*> tape_allocs_;
+  int32_t tape_max_colours_ = 0;
+  bool tape_branches_ = false;  // the branch record is filled (by adjoint())
+  float record_ms_ = 0.0f, adjoint_ms_ = 0.0f;
 };
 
 }  // namespace nsl

@@ -84,6 +84,9 @@ __device__ DeviceWorld world_view(const BatchDevice& b, const BatchWorldDesc& w,
   d.overflow_ids = b.overflow_ids + w.overflow_off;
   d.counters = b.counters;
   d.pairs = PairSet{nullptr, 0};
+  d.joints = nullptr;
+  d.joint_ids = nullptr;
+  d.joint_pairs = PairSet{nullptr, 0};
   d.n_awake = w.n_awake;
   return d;
 }
@@ -387,6 +390,94 @@ __global__ __launch_bounds__(kBatchCollideBlock) void k_batch_collide(BatchDevic
 // Contact transitions in id order (one lane), the solve order, then prepare, the
 // substeps with the world's body states in LDS, restitution, impulse storage and
 // finalisation (solver_stages.rs:543-589, solver_integration.rs:587-799).
+// Tape of one world's step, written after preparation (one wavefront): the prepared
+// constraints, their points' feature ids and shape pair, the links to the previous step's
+// constraints (same contact, points matched by feature id in order, as the impulse
+// carry-over matches them), the colour ranges, and the body states and centres at the
+// step's start. A step outside the differentiable model (overflow constraints,
+// restitution, rolling resistance, tangent velocity, gyroscopic bodies, or more
+// constraints or bodies than the tape holds) is marked with count -1.
+__device__ void tape_record(const BatchDevice& b, int world, const BatchWorldDesc& wd, const DeviceWorld& dl, const WorldRefs& r,
+                            int n_constraints, int n_ovf, bool any_restitution, const Colours& colours,
+                            const BodyState* s_states) {
+  const BatchTape& t = b.tape;
+  const int lane = threadIdx.x;
+  const int k = b.tape_step;
+  const size_t ws = (size_t)k * t.n_worlds + world;
+  const int n = mini(n_constraints, t.max_con);
+  bool bad = false;
+  for (int slot = lane; slot < n; slot += kBatchLanes) {
+    const Constraint& c = dl.constraints[slot];
+    const Contact& ct = dl.contacts[c.contact_id];
+    t.cons[ws * t.max_con + slot] = c;
+    for (int i = 0; i < 4; ++i) t.features[(ws * t.max_con + slot) * 4 + i] = i < c.point_count ? ct.points[i].feature_id : 0xffffffffu;
+    const ContactMeta& m = r.meta[c.contact_id];
+    t.pairs[ws * t.max_con + slot] = ((uint32_t)m.shape_a << 16) | (uint32_t)m.shape_b;
+    bad = bad || c.rolling_resistance > 0.0f || c.tangent_velocity1 != 0.0f || c.tangent_velocity2 != 0.0f;
+  }
+  const int nb = mini(wd.n_awake, t.max_bodies);
+  for (int i = lane; i < nb; i += kBatchLanes) {
+    t.states[ws * t.max_bodies + i] = s_states[i];
+    t.centers[ws * t.max_bodies + i] = dl.sims[i].center;
+    bad = bad || (dl.sims[i].flags & bflag::kGyroscopic) != 0;
+  }
+  const bool ok = wave_ballot(bad) == 0 && n_ovf == 0 && !any_restitution && n_constraints <= t.max_con &&
+                  wd.n_awake <= t.max_bodies;
+  if (lane == 0) {
+    int32_t* col = t.colours + ws * kTapeColourWords;
+    col[0] = colours.n;
+    for (int a = 0; a < colours.n; ++a) {
+      col[1 + a] = colours.begin[a];
+      col[1 + kOverflowColor + a] = colours.end[a];
+    }
+  }
+  // links: the previous step's slot of the same contact (same id and shapes)
+  const size_t wp = k > 0 ? (size_t)(k - 1) * t.n_worlds + world : 0;
+  const int n_prev = k > 0 ? t.count[wp] : -1;
+  for (int slot = lane; slot < n; slot += kBatchLanes) {
+    int32_t link = -1;
+    uint8_t pm[4] = {kTapeNoPoint, kTapeNoPoint, kTapeNoPoint, kTapeNoPoint};
+    const Constraint& c = t.cons[ws * t.max_con + slot];
+    const uint32_t pair = t.pairs[ws * t.max_con + slot];
+    for (int j = 0; j < n_prev; ++j) {
+      if (t.cons[wp * t.max_con + j].contact_id == c.contact_id && t.pairs[wp * t.max_con + j] == pair) {
+        link = j;
+        break;
+      }
+    }
+    if (link >= 0) {
+      uint32_t old[4];
+      const int old_count = t.cons[wp * t.max_con + link].point_count;
+      for (int i = 0; i < 4; ++i) old[i] = i < old_count ? t.features[(wp * t.max_con + link) * 4 + i] : 0xffffffffu;
+      for (int i = 0; i < c.point_count; ++i) {
+        const uint32_t f = t.features[(ws * t.max_con + slot) * 4 + i];
+        for (int j = 0; j < old_count; ++j)
+          if (old[j] == f) {
+            pm[i] = (uint8_t)j;
+            old[j] = 0xffffffffu;
+            break;
+          }
+      }
+    }
+    t.links[ws * t.max_con + slot] = link;
+    for (int i = 0; i < 4; ++i) t.pmap[(ws * t.max_con + slot) * 4 + i] = pm[i];
+  }
+  if (lane == 0) t.count[ws] = ok ? n_constraints : -1;
+}
+
+// States and centres after the last recorded step.
+__global__ __launch_bounds__(kBatchLanes) void k_tape_final(BatchDevice b, int step) {
+  const int world = blockIdx.x;
+  const BatchWorldDesc wd = b.worlds[world];
+  const BatchTape& t = b.tape;
+  const size_t ws = (size_t)step * t.n_worlds + world;
+  const int nb = mini(wd.n_awake, t.max_bodies);
+  for (int i = threadIdx.x; i < nb; i += kBatchLanes) {
+    t.states[ws * t.max_bodies + i] = b.states[wd.awake_off + i];
+    t.centers[ws * t.max_bodies + i] = b.sims[wd.awake_off + i].center;
+  }
+}
+
 __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, StepParams p) {
   __shared__ BodyState s_states[kBatchMaxAwake];
   __shared__ Colours colours;
@@ -471,6 +562,10 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
     }
   }
   wave_sync();
+  if (b.tape_step >= 0) {
+    tape_record(b, world, wd, dl, r, n_constraints, n_ovf, any_restitution, colours, s_states);
+    wave_sync();
+  }
   for (int sub = 0; sub < p.sub_steps; ++sub) {
     for (int i = lane; i < wd.n_awake; i += kBatchLanes) integrate_velocities_item(dl, i, p);
     wave_sync();
@@ -529,10 +624,13 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
 void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hipStream_t s) {
   if (b.n_worlds <= 0) return;
   for (int k = 0; k < steps; ++k) {
-    hipLaunchKernelGGL(k_batch_pairs, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, b);
-    hipLaunchKernelGGL(k_batch_collide, dim3(b.n_worlds), dim3(kBatchCollideBlock), 0, s, b);
-    hipLaunchKernelGGL(k_batch_solve, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, b, p);
+    BatchDevice bk = b;
+    bk.tape_step = b.tape_step >= 0 ? b.tape_step + k : -1;
+    hipLaunchKernelGGL(k_batch_pairs, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, bk);
+    hipLaunchKernelGGL(k_batch_collide, dim3(b.n_worlds), dim3(kBatchCollideBlock), 0, s, bk);
+    hipLaunchKernelGGL(k_batch_solve, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, bk, p);
   }
+  if (b.tape_step >= 0) hipLaunchKernelGGL(k_tape_final, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, b, b.tape_step + steps);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) throw std::runtime_error(std::string("batch step launch: ") + hipGetErrorString(err));
 }
@@ -550,6 +648,7 @@ BatchStepper::BatchStepper(int device) : device_(device) {
 BatchStepper::~BatchStepper() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  free_tape();
   for (void* p : allocations_) (void)hipFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -650,6 +749,7 @@ void BatchStepper::upload() {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return desc_[a].n_awake > desc_[b].n_awake; });
 
   BatchDevice& b = d_;
+  b.tape_step = -1;
   b.n_worlds = nw;
   b.enable_speculative = options.enable_speculative ? 1 : 0;
   b.restitution_iterations = options.restitution_iterations;
@@ -686,11 +786,7 @@ void BatchStepper::upload() {
   uploaded_ = true;
 }
 
-void BatchStepper::step(float dt, int32_t sub_steps, int32_t steps) {
-  if (!(dt > 0.0f)) throw std::invalid_argument("the batched stepper needs a positive time step");
-  if (steps <= 0) return;
-  BATCH_CHECK(hipSetDevice(device_));
-  if (!uploaded_) upload();
+StepParams BatchStepper::step_params(float dt, int32_t sub_steps) const {
   StepParams p{};
   p.dt = dt;
   p.sub_steps = std::max(1, sub_steps);
@@ -708,6 +804,15 @@ void BatchStepper::step(float dt, int32_t sub_steps, int32_t steps) {
   p.enable_sleep = options.enable_sleep;
   p.enable_continuous = options.enable_continuous;
   p.enable_restitution_propagation = options.enable_restitution_propagation;
+  return p;
+}
+
+void BatchStepper::step(float dt, int32_t sub_steps, int32_t steps) {
+  if (!(dt > 0.0f)) throw std::invalid_argument("the batched stepper needs a positive time step");
+  if (steps <= 0) return;
+  BATCH_CHECK(hipSetDevice(device_));
+  if (!uploaded_) upload();
+  const StepParams p = step_params(dt, sub_steps);
   BATCH_CHECK(hipEventRecord(ev0_, stream_));
   launch_batch_step(d_, p, steps, stream_);
   BATCH_CHECK(hipEventRecord(ev1_, stream_));

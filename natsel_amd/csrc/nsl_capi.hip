@@ -320,6 +320,7 @@ int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out) {
   out->solver_blocks = k.solver_blocks;
   out->pair_tests = k.pair_tests;
   for (int i = 0; i < 4; ++i) out->host_ms[i] = k.host_ms[i];
+  out->joints = k.joints;
   return NSL_OK;
 }
 
@@ -434,6 +435,30 @@ int nsl_batch_get_contact_shapes(nsl_batch* batch, int32_t world, int32_t* shape
 int nsl_batch_get_shape_bounds(nsl_batch* batch, int32_t world, float* fat, uint8_t* moved) {
   if (!batch || !fat || !moved || world < 0 || world >= batch->impl->world_count()) return fail(NSL_ERR_INVALID, "bad argument");
   return guarded([&] { batch->impl->get_shape_bounds(world, fat, moved); });
+}
+
+int nsl_batch_record(nsl_batch* batch, float dt, int32_t sub_steps, int32_t steps, int32_t max_constraints) {
+  if (!batch || !(dt > 0.0f) || steps <= 0) return fail(NSL_ERR_INVALID, "null batch, non-positive time step or step count");
+  return guarded([&] { batch->impl->record(dt, sub_steps, steps, max_constraints); });
+}
+
+int nsl_batch_adjoint(nsl_batch* batch, const float* targets, float* loss, float* grad_v, float* grad_w, float* grad_friction,
+                      int32_t* status) {
+  if (!batch || !targets) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { batch->impl->adjoint(targets, loss, grad_v, grad_w, grad_friction, status); });
+}
+
+int nsl_batch_model_losses(nsl_batch* batch, const int32_t* worlds, const int32_t* params, const double* deltas, int32_t n,
+                           const float* targets, int32_t frozen, double* out) {
+  if (!batch || n < 0 || (n > 0 && (!worlds || !params || !deltas || !targets || !out))) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { batch->impl->model_losses(worlds, params, deltas, n, targets, out, frozen != 0); });
+}
+
+int nsl_batch_rollout_times(nsl_batch* batch, double* record_ms, double* adjoint_ms) {
+  if (!batch || !record_ms || !adjoint_ms) return fail(NSL_ERR_INVALID, "null argument");
+  *record_ms = batch->impl->last_record_ms();
+  *adjoint_ms = batch->impl->last_adjoint_ms();
+  return NSL_OK;
 }
 
 int nsl_batch_get_stats(nsl_batch* batch, nsl_batch_stats* out) {

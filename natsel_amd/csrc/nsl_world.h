@@ -65,6 +65,7 @@ class GpuWorld : public HostScene {
     int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
     int64_t barriers, solver_blocks, pair_tests;
     double host_ms[4];  // host wall time: pair update, collide (with its wait), state pass, solve+finalize (with its wait)
+    int64_t joints;
   };
   void set_kernel_timing(bool on);
   const KernelStats& kernel_stats() const { return kstats_; }

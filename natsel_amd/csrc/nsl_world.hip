@@ -688,6 +688,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   kstats_.bodies += (int64_t)d_.n_awake;
   kstats_.collided += (int64_t)ids.size();
   kstats_.constraints += (int64_t)solve.size();
+  kstats_.joints += (int64_t)joint_ids_host_.size();
   kstats_.fast_bodies = counters_host_[0] - counters_base_[0];
   kstats_.sat_calls = counters_host_[1] - counters_base_[1];
   kstats_.sat_hits = counters_host_[2] - counters_base_[2];

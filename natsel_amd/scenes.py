@@ -617,3 +617,37 @@ def joint_hub(satellites: int = 24) -> Scene:
         from_b = (float(f32(rim[0]) - f32(p[0])), 0.0, float(f32(rim[2]) - f32(p[2])))
         sc.add_joint(JointSpec(hub, b, frame_a=_frame(rim), frame_b=_frame(from_b)))
     return sc
+
+
+def pushed_boxes(seed: int, boxes: int = 32) -> Scene:
+    """A differentiable-rollout world: `boxes` cubes (half size U[0.2, 0.35] m, density 1000,
+    friction U[0.3, 0.8]) resting on a ground box, 3 m apart on an 8-wide grid, pushed with a
+    seeded horizontal velocity U[-2, 2] m/s per axis and a spin about the vertical U[-2, 2]
+    rad/s. They slide to rest within the rollout without reaching each other or tipping, so
+    every contact is a flat box-ground contact. Sleep and continuous collision are off."""
+    rng = XorShift32(seed)
+    cols = 8
+    rows = -(-boxes // cols)
+    sc = Scene(name=f"pushed_boxes_{seed}", enable_sleep=False, enable_continuous=False)
+    _ground(sc, 1.5 * cols, 1.5 * rows)
+    for i in range(boxes):
+        h = rng.uniform(0.2, 0.35)
+        x = -1.5 * cols + 1.5 + 3.0 * (i % cols)
+        z = -1.5 * rows + 1.5 + 3.0 * (i // cols)
+        v = (float(rng.uniform(-2.0, 2.0)), 0.0, float(rng.uniform(-2.0, 2.0)))
+        w = (0.0, float(rng.uniform(-2.0, 2.0)), 0.0)
+        friction = float(rng.uniform(0.3, 0.8))
+        b = sc.add_body(BodySpec(DYNAMIC, (float(f32(x)), float(h), float(f32(z))), linear_velocity=v,
+                                 angular_velocity=w, enable_sleep=False))
+        sc.add_shape(ShapeSpec(b, BOX, (float(h), float(h), float(h)), density=1000.0, friction=friction))
+    return sc
+
+
+def pushed_box_targets(count: int, seed: int = 0x7A46) -> np.ndarray:
+    """Seeded loss targets for the differentiable rollout: one centre-of-mass target per
+    world, U[-1, 1] m in x and z about the origin, at height 0.25 m."""
+    rng = XorShift32(seed)
+    t = np.zeros((count, 3), np.float32)
+    for k in range(count):
+        t[k] = (rng.uniform(-1.0, 1.0), 0.25, rng.uniform(-1.0, 1.0))
+    return t

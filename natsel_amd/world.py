@@ -239,7 +239,7 @@ class World:
     def kernel_stats(self) -> dict:
         k = capi.KernelStats()
         capi.check(self._lib.nsl_world_get_kernel_stats(self._h, ctypes.byref(k)))
-        out = {n: getattr(k, n) for n, _ in capi.KernelStats._fields_[4:-1]}
+        out = {n: getattr(k, n) for n, _ in capi.KernelStats._fields_ if n not in ("ms", "launches", "phase_ms", "phase_count", "host_ms")}
         out["ms"] = dict(zip(capi.KERNEL_CLASSES, list(k.ms)))
         out["launches"] = dict(zip(capi.KERNEL_CLASSES, list(k.launches)))
         out["phase_ms"] = dict(zip(capi.PHASE_CLASSES, list(k.phase_ms)))

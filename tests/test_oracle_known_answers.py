@@ -85,3 +85,18 @@ def test_pyramid_settles_and_rests():
     assert max(abs(h - settled) for h in heights[240:]) < 0.01
     assert max(speeds[240:]) < 0.05
     assert 0.02 < 19.5 - settled < 0.03
+
+
+def test_pushed_box_worlds_only_touch_the_ground():
+    """The differentiable-rollout worlds: 32 boxes each, every contact a box-ground contact
+    in colour 22 (no box reaches another within 64 steps), all at rest at the end."""
+    for seed in (3, 77):
+        sc = scenes.pushed_boxes(seed)
+        assert sc.dynamic_count == 32
+        w = _oracle(sc)
+        for _ in range(64):
+            w.step(1.0 / 60.0, 4)
+        color, _, alive = w.contact_colors()
+        assert sorted(set(color[alive == 1].tolist())) == [22] and int(alive.sum()) == 32
+    t = scenes.pushed_box_targets(16)
+    assert t.shape == (16, 3) and np.array_equal(t, scenes.pushed_box_targets(16))
