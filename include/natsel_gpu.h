@@ -149,6 +149,15 @@ int nsl_world_get_joint(nsl_world* world, int32_t joint, float impulses[9], int3
 int nsl_world_joint_count(nsl_world* world, int32_t* out);
 int nsl_world_contact_count(nsl_world* world, int32_t* out);
 
+/* World::save_state / World::restore_state (src/simulation/world_state.rs:340, :439): a
+ * snapshot holds everything that decides later steps (bodies, contacts and their caches,
+ * the pair set, joints, ids, colours). Restoring needs the bodies, shapes and joints the
+ * world held when the snapshot was taken. */
+typedef struct nsl_snapshot nsl_snapshot;
+int nsl_world_save_state(nsl_world* world, nsl_snapshot** out);
+int nsl_world_restore_state(nsl_world* world, const nsl_snapshot* snapshot);
+int nsl_snapshot_destroy(nsl_snapshot* snapshot);
+
 int nsl_world_step(nsl_world* world, float dt, int32_t sub_steps);
 int nsl_world_step_n(nsl_world* world, float dt, int32_t sub_steps, int32_t steps);
 

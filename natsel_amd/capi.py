@@ -138,6 +138,9 @@ SIGNATURES = {
     "nsl_world_get_joint": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_I)]),
     "nsl_world_joint_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "nsl_world_contact_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "nsl_world_save_state": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "nsl_world_restore_state": (ctypes.c_int, [_P, _P]),
+    "nsl_snapshot_destroy": (ctypes.c_int, [_P]),
     "nsl_world_step": (ctypes.c_int, [_P, _F, _I]),
     "nsl_world_step_n": (ctypes.c_int, [_P, _F, _I, _I]),
     "nsl_world_body_capacity": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),

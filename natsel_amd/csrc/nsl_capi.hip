@@ -222,6 +222,30 @@ int nsl_world_joint_count(nsl_world* world, int32_t* out) {
   return NSL_OK;
 }
 
+struct nsl_snapshot {
+  nsl::GpuWorld::Snapshot* impl = nullptr;
+  ~nsl_snapshot() { nsl::GpuWorld::destroy_snapshot(impl); }
+};
+
+int nsl_world_save_state(nsl_world* world, nsl_snapshot** out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] {
+    auto s = std::make_unique<nsl_snapshot>();
+    s->impl = world->impl->save_state();
+    *out = s.release();
+  });
+}
+
+int nsl_world_restore_state(nsl_world* world, const nsl_snapshot* snapshot) {
+  if (!world || !snapshot) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { world->impl->restore_state(*snapshot->impl); });
+}
+
+int nsl_snapshot_destroy(nsl_snapshot* snapshot) {
+  delete snapshot;
+  return NSL_OK;
+}
+
 int nsl_world_contact_count(nsl_world* world, int32_t* out) {
   if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
   *out = world->impl->contact_count();

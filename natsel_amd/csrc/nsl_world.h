@@ -71,6 +71,17 @@ class GpuWorld : public HostScene {
   const KernelStats& kernel_stats() const { return kstats_; }
   void reset_kernel_stats();
 
+  // Everything that decides later steps (world_state.rs:340-438 save_state): body states
+  // and sims, fat AABBs and moved flags, contacts with their manifolds and caches, the
+  // pair-key set, joints, the id pool, the non-touching list, the colour lists and body
+  // bits, the work counters. Device arrays are copied device to device.
+  struct Snapshot;
+  Snapshot* save_state();  // owned by the caller; free with destroy_snapshot
+  static void destroy_snapshot(Snapshot* s);
+  // world_state.rs:439-560 restore_state. The world must hold the bodies, shapes and joints
+  // it held when the snapshot was taken.
+  void restore_state(const Snapshot& s);
+
  private:
   struct HContact {
     int32_t shape_a = -1, shape_b = -1, color = -1, local = -1, set = -1;

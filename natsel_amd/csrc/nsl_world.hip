@@ -521,6 +521,135 @@ int32_t GpuWorld::build_overflow_levels() {
 }
 
 // ---------------------------------------------------------------------------------
+// snapshots: world_state.rs:340-560
+
+struct GpuWorld::Snapshot {
+  int device = 0;
+  size_t n_bodies = 0, n_shapes = 0, n_joints = 0, n_awake = 0, n_device_shapes = 0, n_contacts = 0;
+  std::vector<void*> buffers;
+  BodySim* sims = nullptr;
+  BodyState* states = nullptr;
+  Aabb* fat = nullptr;
+  uint8_t* moved = nullptr;
+  Contact* contacts = nullptr;
+  uint64_t* pair_slots = nullptr;
+  uint32_t pair_mask = 0;
+  JointSim* joints = nullptr;
+  int32_t counters[8] = {};
+  std::vector<HBody> bodies;
+  std::vector<BodySim> awake_host;
+  std::vector<BodyState> states_host;
+  std::vector<HShape> hshapes;
+  std::vector<HContact> hcontacts;
+  std::vector<int32_t> contact_free, non_touching;
+  int32_t contact_pool_next = 0;
+  ColorSet colors[kColorCount];
+  int32_t counters_host[8] = {}, counters_base[8] = {};
+  int64_t pair_set_tombs = 0;
+  bool pairs_pending = false;
+  uint64_t contact_serial = 0;
+  ~Snapshot() {
+    if (!buffers.empty()) (void)hipSetDevice(device);
+    for (void* p : buffers) (void)hipFree(p);
+  }
+  template <class T> T* copy_of(const T* src, size_t n, hipStream_t stream) {
+    if (n == 0 || src == nullptr) return nullptr;
+    T* p = nullptr;
+    NSL_CHECK(hipMalloc(&p, n * sizeof(T)));
+    buffers.push_back(p);
+    NSL_CHECK(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    return p;
+  }
+};
+
+void GpuWorld::destroy_snapshot(Snapshot* s) { delete s; }
+
+GpuWorld::Snapshot* GpuWorld::save_state() {
+  NSL_CHECK(hipSetDevice(device_));
+  if (bodies_dirty_ || shapes_dirty_) upload_bodies();
+  if (device_newer_) pull_bodies();
+  auto s = std::make_unique<Snapshot>();
+  s->device = device_;
+  s->n_bodies = bodies_.size();
+  s->n_shapes = hshapes_.size();
+  s->n_joints = hjoints_.size();
+  s->n_awake = awake_host_.size();
+  s->n_device_shapes = n_device_shapes_;
+  s->n_contacts = hcontacts_.size();
+  s->sims = s->copy_of(d_.sims, s->n_awake, stream_);
+  s->states = s->copy_of(d_.states, s->n_awake, stream_);
+  s->fat = s->copy_of(d_.fat_aabbs, n_device_shapes_, stream_);
+  s->moved = s->copy_of(d_.shape_moved, n_device_shapes_, stream_);
+  s->contacts = s->copy_of(d_.contacts, s->n_contacts, stream_);
+  if (d_.pairs.slots) {
+    s->pair_mask = d_.pairs.mask;
+    s->pair_slots = s->copy_of(d_.pairs.slots, (size_t)d_.pairs.mask + 1, stream_);
+  }
+  s->joints = s->copy_of(d_.joints, hjoints_.size(), stream_);
+  NSL_CHECK(hipMemcpyAsync(s->counters, d_.counters, sizeof(s->counters), hipMemcpyDeviceToHost, stream_));
+  NSL_CHECK(hipStreamSynchronize(stream_));
+  s->bodies = bodies_;
+  s->awake_host = awake_host_;
+  s->states_host = states_host_;
+  s->hshapes = hshapes_;
+  s->hcontacts = hcontacts_;
+  s->contact_free = contact_free_;
+  s->non_touching = non_touching_;
+  s->contact_pool_next = contact_pool_next_;
+  for (int c = 0; c < kColorCount; ++c) s->colors[c] = colors_[c];
+  std::memcpy(s->counters_host, counters_host_, sizeof(counters_host_));
+  std::memcpy(s->counters_base, counters_base_, sizeof(counters_base_));
+  s->pair_set_tombs = pair_set_tombs_;
+  s->pairs_pending = pairs_pending_;
+  s->contact_serial = contact_serial_;
+  return s.release();
+}
+
+void GpuWorld::restore_state(const Snapshot& s) {
+  if (s.n_bodies != bodies_.size() || s.n_shapes != hshapes_.size() || s.n_joints != hjoints_.size())
+    throw std::invalid_argument("restore_state: the world's bodies, shapes or joints changed since the snapshot");
+  NSL_CHECK(hipSetDevice(device_));
+  if (bodies_dirty_ || shapes_dirty_) upload_bodies();
+  NSL_CHECK(hipStreamSynchronize(stream_));
+  auto back = [&](auto* dst, const auto* src, size_t n) {
+    if (n) NSL_CHECK(hipMemcpyAsync(dst, src, n * sizeof(*src), hipMemcpyDeviceToDevice, stream_));
+  };
+  back(d_.sims, s.sims, s.n_awake);
+  back(d_.states, s.states, s.n_awake);
+  back(d_.fat_aabbs, s.fat, s.n_device_shapes);
+  back(d_.shape_moved, s.moved, s.n_device_shapes);
+  ensure(d_.contacts, cap_contacts_, std::max<size_t>(s.n_contacts, 1), false);
+  back(d_.contacts, s.contacts, s.n_contacts);
+  if (s.pair_slots) {
+    if (!d_.pairs.slots || d_.pairs.mask != s.pair_mask) {
+      if (d_.pairs.slots) NSL_CHECK(hipFree(d_.pairs.slots));
+      NSL_CHECK(hipMalloc(&d_.pairs.slots, ((size_t)s.pair_mask + 1) * sizeof(uint64_t)));
+      d_.pairs.mask = s.pair_mask;
+    }
+    back(d_.pairs.slots, s.pair_slots, (size_t)s.pair_mask + 1);
+  }
+  back(d_.joints, s.joints, s.n_joints);
+  NSL_CHECK(hipMemcpyAsync(d_.counters, s.counters, sizeof(s.counters), hipMemcpyHostToDevice, stream_));
+  NSL_CHECK(hipStreamSynchronize(stream_));
+  bodies_ = s.bodies;
+  awake_host_ = s.awake_host;
+  states_host_ = s.states_host;
+  hshapes_ = s.hshapes;
+  hcontacts_ = s.hcontacts;
+  contact_free_ = s.contact_free;
+  non_touching_ = s.non_touching;
+  contact_pool_next_ = s.contact_pool_next;
+  for (int c = 0; c < kColorCount; ++c) colors_[c] = s.colors[c];
+  std::memcpy(counters_host_, s.counters_host, sizeof(counters_host_));
+  std::memcpy(counters_base_, s.counters_base, sizeof(counters_base_));
+  pair_set_tombs_ = s.pair_set_tombs;
+  pairs_pending_ = s.pairs_pending;
+  contact_serial_ = s.contact_serial;
+  device_newer_ = false;
+  joints_dirty_ = true;  // the colour lists of joints are rebuilt from colors_
+}
+
+// ---------------------------------------------------------------------------------
 // one step: solver_integration.rs:807-1015
 
 void GpuWorld::step(float dt, int32_t sub_steps) {

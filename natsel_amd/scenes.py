@@ -651,3 +651,23 @@ def pushed_box_targets(count: int, seed: int = 0x7A46) -> np.ndarray:
     for k in range(count):
         t[k] = (rng.uniform(-1.0, 1.0), 0.25, rng.uniform(-1.0, 1.0))
     return t
+
+
+def run_rollback(world, steps: int = 200, dt: float = 1.0 / 60.0, substeps: int = 4):
+    """The reference's rollback benchmark driver (bin/benchmarks/feature_scenes.rs:702-734 and
+    the harness loop bin/benchmarks.rs:334-360): before every step the state is saved (the
+    last 8 kept); at every 30th step the oldest is restored and 7 steps replayed. With an
+    exact save and restore the run ends on the plain run's state. `world` is any stepper
+    with save_state, restore_state and step (the GPU world or the oracle)."""
+    states = []
+    for i in range(steps):
+        states.append(world.save_state())
+        if len(states) > 8:
+            states.pop(0)
+        if i % 30 == 29 and states:
+            world.restore_state(states[0])
+            for _ in range(7):
+                world.step(dt, substeps)
+            states.clear()
+        world.step(dt, substeps)
+    return world

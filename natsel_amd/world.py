@@ -100,6 +100,21 @@ def c_options(o: WorldOptions) -> capi.Options:
     return c
 
 
+class Snapshot:
+    """A saved world state (device and host copies); freed with the object."""
+
+    def __init__(self, lib, handle):
+        self._lib, self._h = lib, handle
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.nsl_snapshot_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 class World:
     def __init__(self, device: int = 0, options: WorldOptions | None = None):
         self._lib = capi.load_library()
@@ -191,6 +206,16 @@ class World:
         capi.check(self._lib.nsl_world_get_joint(self._h, joint_id, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                                  ctypes.byref(color)))
         return out, color.value
+
+    def save_state(self) -> "Snapshot":
+        """World::save_state (world_state.rs:340)."""
+        h = ctypes.c_void_p()
+        capi.check(self._lib.nsl_world_save_state(self._h, ctypes.byref(h)))
+        return Snapshot(self._lib, h)
+
+    def restore_state(self, s: "Snapshot") -> None:
+        """World::restore_state (world_state.rs:439)."""
+        capi.check(self._lib.nsl_world_restore_state(self._h, s._h))
 
     def contact_count(self) -> int:
         out = ctypes.c_int32()

@@ -60,6 +60,9 @@ def lib():
         L.nslo_max_threads.restype = ctypes.c_int
         L.nslo_create_revolute_joint.argtypes = [P, I, I, fp, fp, F, F, I, I, F, F, F, I, F, F, I, F, F]
         L.nslo_get_joint.argtypes = [P, I, fp]
+        L.nslo_save_state.argtypes = [P]
+        L.nslo_save_state.restype = P
+        L.nslo_restore_state.argtypes = [P, P]
         _LIB = L
     return _LIB
 
@@ -71,6 +74,19 @@ def set_threads(n: int) -> None:
 
 def max_threads() -> int:
     return int(lib().nslo_max_threads())
+
+
+class _OracleSnapshot:
+    def __init__(self, L, h):
+        self.L, self.h = L, h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.nslo_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 class OracleWorld:
@@ -130,6 +146,13 @@ class OracleWorld:
 
     def world_state_hash(self) -> int:
         return self.L.nslo_state_hash(self.h)
+
+    def save_state(self):
+        """A copy of the world (freed with the returned object)."""
+        return _OracleSnapshot(self.L, self.L.nslo_save_state(self.h))
+
+    def restore_state(self, s) -> None:
+        self.L.nslo_restore_state(self.h, s.h)
 
     def contact_count(self) -> int:
         return int(self.L.nslo_contact_count(self.h))

@@ -106,6 +106,11 @@ int nslo_get_joint(void* wp, int joint_id, float* out) {
   return w->joints[joint_id].color_index;
 }
 
+// World::save_state / restore_state (world_state.rs:340, :439): the oracle's World is plain
+// values, so a snapshot is a copy of it.
+void* nslo_save_state(void* wp) { return new World(*static_cast<World*>(wp)); }
+void nslo_restore_state(void* wp, void* sp) { *static_cast<World*>(wp) = *static_cast<World*>(sp); }
+
 void nslo_step(void* wp, float dt, int substeps) { static_cast<World*>(wp)->step(dt, substeps); }
 uint64_t nslo_state_hash(void* wp) { return static_cast<World*>(wp)->state_hash(); }
 int nslo_body_capacity(void* wp) { return (int)static_cast<World*>(wp)->bodies.size(); }

@@ -84,6 +84,41 @@ def test_golden_hash_on_gpu(gpu_available, name, builder):
     assert w.world_state_hash() == int(g["state_hash"], 16)
 
 
+def test_rollback_golden_hash_on_gpu(gpu_available):
+    """rollback (reference-hashes.json:90-94) on the GPU world: snapshots saved every step,
+    restored and replayed every 30 steps; the end hash is the large pyramid's."""
+    from natsel_amd.world import load_scene
+
+    g = GOLDEN["scenes"]["rollback"]
+    w = scenes.run_rollback(load_scene(scenes.pyramid(100)), g["steps"])
+    assert w.world_state_hash() == int(g["state_hash"], 16)
+
+
+def test_snapshot_restores_a_pile_with_joints(gpu_available):
+    """A restored snapshot replays bit for bit: the chain pile (contacts, overflow, joints)
+    saved at step 60, stepped 40 more, restored and stepped 40 again ends on the same hash,
+    which the oracle reaches too."""
+    from natsel_amd.world import load_scene
+    from oracle import pyoracle
+
+    sc = scenes.chain_pile(count=200, chains=20)
+    w = load_scene(sc)
+    o = pyoracle.load_scene(sc)
+    for _ in range(60):
+        w.step(1.0 / 60.0, 4)
+        o.step(1.0 / 60.0, 4)
+    s = w.save_state()
+    for _ in range(40):
+        w.step(1.0 / 60.0, 4)
+        o.step(1.0 / 60.0, 4)
+    first = w.world_state_hash()
+    assert first == o.world_state_hash()
+    w.restore_state(s)
+    for _ in range(40):
+        w.step(1.0 / 60.0, 4)
+    assert w.world_state_hash() == first
+
+
 def test_small_mixed_pile_every_step_matches_oracle(gpu_available):
     """Boxes, spheres and capsules dropped into a container: exercises every pair type
     (hull-hull SAT, GJK hull-sphere and hull-capsule, capsule-capsule clipping,

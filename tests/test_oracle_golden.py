@@ -41,6 +41,14 @@ def test_many_pyramids_golden_hash():
     assert w.world_state_hash() == int(g["state_hash"], 16)
 
 
+def test_rollback_golden_hash():
+    """rollback (reference-hashes.json:90-94): the large pyramid saved every step, every 30
+    steps rolled back 8 states and replayed, ends on the large pyramid's hash."""
+    g = GOLDEN["scenes"]["rollback"]
+    w = scenes.run_rollback(pyoracle.load_scene(scenes.pyramid(100)), g["steps"])
+    assert w.world_state_hash() == int(g["state_hash"], 16)
+
+
 def test_planted_mistake_changes_hash():
     """A one-ulp change of one body's start position must change the end hash: the
     hash check is sensitive to single-bit differences, not just coarse motion."""
