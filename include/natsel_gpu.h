@@ -16,6 +16,8 @@
  *                                                               src/simulation/body_api.rs:186-262
  *   nsl_world_state_hash           World::world_state_hash      src/simulation/body_queries.rs:432
  *   nsl_world_get_contact_colors   constraint-graph colour of each contact (graph_tests.rs:152-172)
+ *   nsl_world_get_contact_shapes   the shape pair of each contact id (world_runtime.rs:517-555)
+ *   nsl_time_of_impact             time_of_impact               src/collision/time_of_impact.rs:406-598
  *   nsl_world_get_profile          World::profile               src/types/world_def.rs:158-184
  *   nsl_world_*kernel_*            per-kernel device timing (no counterpart: the reference times phases
  *                                  on the host, solver_stages.rs:551-586)
@@ -72,6 +74,9 @@ typedef struct nsl_body_def {
   float angular_velocity[3];
   float linear_damping, angular_damping, gravity_scale;
   int32_t enable_sleep;
+  int32_t is_bullet;                 /* swept against every body type, after the other fast bodies */
+  float safety_factor;               /* 0.5: fast (swept) when a step moves it more than this times
+                                        its smallest extent (body_def.rs:68, continuous.rs:121-151) */
 } nsl_body_def;
 
 typedef struct nsl_material {
@@ -85,7 +90,7 @@ typedef struct nsl_profile {
 } nsl_profile;
 
 /* Launch classes of nsl_kernel_stats.ms / launches (HIP events around each launch). */
-enum { NSL_K_INIT = 0, NSL_K_COLLIDE, NSL_K_SOLVE, NSL_K_CLASS_COUNT };
+enum { NSL_K_INIT = 0, NSL_K_COLLIDE, NSL_K_SOLVE, NSL_K_CONTINUOUS, NSL_K_CLASS_COUNT };
 
 /* Phases of the step kernel (NSL_K_SOLVE), timed with the device wall clock at its grid
  * barriers; each phase's time includes the barrier that ends it. */
@@ -106,7 +111,29 @@ typedef struct nsl_kernel_stats {
   int64_t pair_tests;    /* candidate shape pairs tested by the device pair finder */
   double host_ms[4];     /* host wall time: pair update, collide (incl. wait), state pass, solve+finalize (incl. wait) */
   int64_t joints;        /* joints solved, summed over steps */
+  int64_t toi_calls;     /* continuous collision: time-of-impact queries */
+  int64_t toi_hits;      /* continuous collision: fast bodies moved back to an impact */
 } nsl_kernel_stats;
+
+/* One time-of-impact query (TOIInput, src/types/queries.rs:201-207): two proxies of 1 to 8
+   support points with a radius, and their sweeps: local centre, c1, c2, q1 (x, y, z, s), q2. */
+typedef struct nsl_toi_input {
+  float points_a[8][3], points_b[8][3];
+  int32_t count_a, count_b;
+  float radius_a, radius_b;
+  float sweep_a[17], sweep_b[17];
+  float max_fraction;
+} nsl_toi_input;
+
+enum { NSL_TOI_UNKNOWN = 0, NSL_TOI_FAILED, NSL_TOI_OVERLAPPED, NSL_TOI_HIT, NSL_TOI_SEPARATED };
+
+/* TOIOutput (src/types/queries.rs:224-234). */
+typedef struct nsl_toi_output {
+  int32_t state;
+  float point[3], normal[3];
+  float fraction, distance;
+  int32_t distance_iterations, push_back_iterations, root_iterations;
+} nsl_toi_output;
 
 /* A revolute joint (RevoluteJointDef, src/types/joint_defs.rs:30-58, :154-167). Frames
    are (px, py, pz, qx, qy, qz, qs) in each body's frame; the hinge axis is frame A's z. */
@@ -128,6 +155,12 @@ typedef struct nsl_revolute_joint_def {
 const char* nsl_last_error(void);
 void nsl_default_options(nsl_options* out);
 void nsl_default_revolute_joint_def(nsl_revolute_joint_def* out);
+/* BodyDef::default (src/types/body_def.rs:80-104): a static body at the origin. */
+void nsl_default_body_def(nsl_body_def* out);
+
+/* time_of_impact (src/collision/time_of_impact.rs:406-598) for n queries on the device, one
+   lane each; host arrays in and out. */
+int nsl_time_of_impact(int device, const nsl_toi_input* in, nsl_toi_output* out, int32_t n);
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out);
 int nsl_world_destroy(nsl_world* world);
@@ -168,6 +201,9 @@ int nsl_world_state_hash(nsl_world* world, uint64_t* out);
 int nsl_world_contact_capacity(nsl_world* world, int32_t* out);
 /* Arrays sized by nsl_world_contact_capacity. */
 int nsl_world_get_contact_colors(nsl_world* world, int32_t* color, int32_t* local, int32_t* alive);
+/* Per contact id: its two shape ids (Contact::shape_id_a/_b, world_runtime.rs:517-555), -1 when
+   the id is free. */
+int nsl_world_get_contact_shapes(nsl_world* world, int32_t* shape_a, int32_t* shape_b);
 int nsl_world_get_profile(nsl_world* world, nsl_profile* out);
 /* Timing adds an event pair per launch and a wall-clock read per phase; results do not change. */
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on);

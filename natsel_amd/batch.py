@@ -33,6 +33,8 @@ class BatchWorld:
         c.angular_velocity[:] = d.angular_velocity
         c.linear_damping, c.angular_damping, c.gravity_scale = d.linear_damping, d.angular_damping, d.gravity_scale
         c.enable_sleep = int(d.enable_sleep)
+        c.is_bullet = int(d.is_bullet)
+        c.safety_factor = d.safety_factor
         out = ctypes.c_int32()
         capi.check(self._b._lib.nsl_batch_create_body(self._b._h, self.index, ctypes.byref(c), ctypes.byref(out)))
         return out.value
@@ -201,7 +203,8 @@ def load_batch(scene_list: list[Scene], device: int = 0) -> Batch:
     for scene in scene_list:
         w = b.add_world()
         ids = [w.create_body(BodyDef(s.body_type, s.position, s.rotation, s.linear_velocity, s.angular_velocity, s.linear_damping,
-                                     s.angular_damping, s.gravity_scale, s.enable_sleep)) for s in scene.bodies]
+                                     s.angular_damping, s.gravity_scale, s.enable_sleep, s.is_bullet, s.safety_factor))
+               for s in scene.bodies]
         for s in scene.shapes:
             d = ShapeDef(s.density, s.friction, s.restitution)
             if s.kind == BOX:

@@ -41,6 +41,8 @@ class BodyDef(ctypes.Structure):
         ("angular_damping", ctypes.c_float),
         ("gravity_scale", ctypes.c_float),
         ("enable_sleep", ctypes.c_int32),
+        ("is_bullet", ctypes.c_int32),
+        ("safety_factor", ctypes.c_float),
     ]
 
 
@@ -82,7 +84,7 @@ class Profile(ctypes.Structure):
     ]
 
 
-KERNEL_CLASSES = ["init", "collide", "solve"]
+KERNEL_CLASSES = ["init", "collide", "solve", "continuous"]
 PHASE_CLASSES = ["prepare", "integrate_velocities", "warm_start", "push", "integrate_positions", "relax", "restitution",
                  "overflow", "store_finalize"]
 HOST_PHASES = ["pairs", "collide", "state_pass", "solve_finalize"]
@@ -98,7 +100,35 @@ class KernelStats(ctypes.Structure):
         (n, ctypes.c_int64)
         for n in ("steps", "bodies", "collided", "constraints", "points", "overflow_points", "recycled", "sat_calls",
                   "sat_hits", "fast_bodies", "barriers", "solver_blocks", "pair_tests")
-    ] + [("host_ms", ctypes.c_double * len(HOST_PHASES)), ("joints", ctypes.c_int64)]
+    ] + [("host_ms", ctypes.c_double * len(HOST_PHASES)), ("joints", ctypes.c_int64), ("toi_calls", ctypes.c_int64),
+         ("toi_hits", ctypes.c_int64)]
+
+
+class ToiInput(ctypes.Structure):
+    _fields_ = [
+        ("points_a", (ctypes.c_float * 3) * 8),
+        ("points_b", (ctypes.c_float * 3) * 8),
+        ("count_a", ctypes.c_int32),
+        ("count_b", ctypes.c_int32),
+        ("radius_a", ctypes.c_float),
+        ("radius_b", ctypes.c_float),
+        ("sweep_a", ctypes.c_float * 17),
+        ("sweep_b", ctypes.c_float * 17),
+        ("max_fraction", ctypes.c_float),
+    ]
+
+
+class ToiOutput(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_int32),
+        ("point", ctypes.c_float * 3),
+        ("normal", ctypes.c_float * 3),
+        ("fraction", ctypes.c_float),
+        ("distance", ctypes.c_float),
+        ("distance_iterations", ctypes.c_int32),
+        ("push_back_iterations", ctypes.c_int32),
+        ("root_iterations", ctypes.c_int32),
+    ]
 
 
 class BatchStats(ctypes.Structure):
@@ -134,6 +164,8 @@ SIGNATURES = {
         [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), _F, ctypes.POINTER(Material), ctypes.POINTER(_I)],
     ),
     "nsl_default_revolute_joint_def": (None, [ctypes.POINTER(RevoluteJointDef)]),
+    "nsl_default_body_def": (None, [ctypes.POINTER(BodyDef)]),
+    "nsl_time_of_impact": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ToiInput), ctypes.POINTER(ToiOutput), _I]),
     "nsl_create_revolute_joint": (ctypes.c_int, [_P, ctypes.POINTER(RevoluteJointDef), ctypes.POINTER(_I)]),
     "nsl_world_get_joint": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_I)]),
     "nsl_world_joint_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
@@ -151,6 +183,7 @@ SIGNATURES = {
     "nsl_world_state_hash": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "nsl_world_contact_capacity": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "nsl_world_get_contact_colors": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "nsl_world_get_contact_shapes": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "nsl_world_get_profile": (ctypes.c_int, [_P, ctypes.POINTER(Profile)]),
     "nsl_world_set_kernel_timing": (ctypes.c_int, [_P, _I]),
     "nsl_world_reset_kernel_stats": (ctypes.c_int, [_P]),

@@ -41,6 +41,7 @@ struct BatchWorldDesc {
   int32_t color_off, color_cap;        // lists of colours 0..22: [23][color_cap] contact ids
   int32_t bits_off, bits_words;        // body bits of colours 0..22: [23][bits_words] u32
   int32_t pair_off;                    // pair-exists bits: n_shapes * n_shapes bits (u32 words)
+  int32_t n_ccd_static, n_ccd_kinematic, n_ccd_dynamic;  // continuous-collision candidates at shape_off
 };
 
 // The integer bookkeeping of one world (what the single-world stepper keeps on the host).
@@ -96,6 +97,10 @@ struct BatchDevice {
   BodyMeta* body_meta;
   Shape* shapes;
   Aabb* fat_aabbs;
+  Aabb* shape_aabbs;   // per shape: bounds from its body's last finalize or sweep
+  int32_t* candidates; // per world at shape_off: continuous-collision candidates (static, kinematic, dynamic)
+  int32_t* fast_ids;   // this step's fast bodies (awake index into the batch arrays), counters[kCntFastList] long
+  int32_t* awake_world;  // per awake body of the batch: its world
   uint8_t* shape_moved;
   Proxy* proxies;  // pad holds the shape's enable_speculative flag
   BoxHull* hulls;
@@ -120,6 +125,8 @@ struct BatchDevice {
   int32_t n_worlds;
   int32_t enable_speculative;
   int32_t restitution_iterations;
+  int32_t enable_continuous;
+  int32_t any_bullets;  // a dynamic bullet body in some world: the bullet pass runs
   BatchTape tape;
   int32_t tape_step;  // step of the tape to record this launch (-1: not recording)
 };

@@ -7,6 +7,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "nsl_continuous.h"
 #include "nsl_stages.h"
 
 namespace nsl {
@@ -87,6 +88,9 @@ __device__ DeviceWorld world_view(const BatchDevice& b, const BatchWorldDesc& w,
   d.joints = nullptr;
   d.joint_ids = nullptr;
   d.joint_pairs = PairSet{nullptr, 0};
+  d.shape_aabbs = b.shape_aabbs + w.shape_off;
+  d.fast_ids = b.fast_ids;
+  d.fast_base = w.awake_off;
   d.n_awake = w.n_awake;
   return d;
 }
@@ -316,6 +320,7 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_pairs(BatchDevice b) {
   __shared__ uint32_t s_keys[kBatchMaxNewPairs];
   __shared__ int s_count;
   const int lane = threadIdx.x;
+  if (blockIdx.x == 0 && lane == 0) b.counters[kCntFastList] = 0;  // the solve launch lists this step's fast bodies
   const int world = b.order[blockIdx.x];
   const BatchWorldDesc wd = b.worlds[world];
   const WorldRefs r = world_refs(b, wd, world);
@@ -619,6 +624,37 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
   for (int i = lane; i < wd.n_awake; i += kBatchLanes) dg.states[i] = s_states[i];
 }
 
+// Continuous collision of the fast bodies the solve launch listed, each in its own world:
+// without `bullets` the fast bodies against their world's static shapes, each followed by
+// its world inverse inertia (solver_integration.rs:538-566); with `bullets` the bullets
+// against every shape type of their world (solver_integration.rs:948-995). A body reads its
+// world's static shapes (and, as a bullet, its other bodies) and writes only itself.
+__global__ __launch_bounds__(kBatchLanes) void k_batch_continuous(BatchDevice b, StepParams p, int bullets) {
+  const int n = b.counters[kCntFastList];
+  int hits = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int g = b.fast_ids[k];
+    const int world = b.awake_world[g];
+    const BatchWorldDesc wd = b.worlds[world];
+    const DeviceWorld d = world_view(b, wd, b.states + wd.awake_off);
+    const int i = g - wd.awake_off;
+    if (((d.sims[i].flags & bflag::kIsBullet) != 0) != (bullets != 0)) continue;
+    const ContinuousArgs ca{b.candidates + wd.shape_off, wd.n_ccd_static, wd.n_ccd_kinematic, wd.n_ccd_dynamic,
+                            b.proxies + wd.shape_off};
+    const bool hit = solve_continuous_body(d, ca, i, p);
+    if (!bullets) {
+      BodySim& sim = d.sims[i];
+      Mat3 rm = matrix_from_quat(sim.transform.q);
+      sim.inv_inertia_world = mul_mm(mul_mm(rm, sim.inv_inertia_local), transpose(rm));
+    }
+    if (!hit) continue;
+    hits += 1;
+    // a body moved back to an impact leaves the differentiable rollout's model
+    if (b.tape_step >= 0) b.tape.count[(size_t)b.tape_step * b.tape.n_worlds + world] = -1;
+  }
+  if (hits) atomicAdd(&b.counters[kCntToiHits], hits);
+}
+
 }  // namespace
 
 void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hipStream_t s) {
@@ -629,6 +665,11 @@ void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hip
     hipLaunchKernelGGL(k_batch_pairs, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, bk);
     hipLaunchKernelGGL(k_batch_collide, dim3(b.n_worlds), dim3(kBatchCollideBlock), 0, s, bk);
     hipLaunchKernelGGL(k_batch_solve, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, bk, p);
+    if (b.enable_continuous) {
+      const int blocks = std::max(1, std::min(b.n_worlds, 1024));
+      hipLaunchKernelGGL(k_batch_continuous, dim3(blocks), dim3(kBatchLanes), 0, s, bk, p, 0);
+      if (b.any_bullets) hipLaunchKernelGGL(k_batch_continuous, dim3(blocks), dim3(kBatchLanes), 0, s, bk, p, 1);
+    }
   }
   if (b.tape_step >= 0) hipLaunchKernelGGL(k_tape_final, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, b, b.tape_step + steps);
   const hipError_t err = hipGetLastError();
@@ -694,7 +735,9 @@ void BatchStepper::upload() {
   std::vector<BodyState> states;
   std::vector<BodyMeta> bmeta;
   std::vector<Shape> shapes;
-  std::vector<Aabb> fat;
+  std::vector<Aabb> fat, aabbs;
+  std::vector<int32_t> candidates, awake_world;
+  int32_t any_bullets = 0;
   std::vector<uint8_t> moved;
   std::vector<Proxy> proxies;
   std::vector<BoxHull> hulls;
@@ -716,12 +759,25 @@ void BatchStepper::upload() {
     d.shape_off = (int32_t)shapes.size();
     d.hull_off = (int32_t)hulls.size();
     sims.insert(sims.end(), sc.awake_host_.begin(), sc.awake_host_.end());
+    awake_world.insert(awake_world.end(), sc.awake_host_.size(), w);
+    for (const BodySim& s : sc.awake_host_)
+      if (s.body_type == kBodyDynamic && (s.flags & bflag::kIsBullet)) any_bullets = 1;
     states.insert(states.end(), sc.states_host_.begin(), sc.states_host_.end());
     statics.insert(statics.end(), sc.static_host_.begin(), sc.static_host_.end());
     for (const HostScene::HBody& hb : sc.bodies_) bmeta.push_back(BodyMeta{hb.type, sc.encode(hb), hb.flags});
     shapes.insert(shapes.end(), sc.shapes_.begin(), sc.shapes_.end());
+    for (int type = kBodyStatic; type <= kBodyDynamic; ++type) {
+      int32_t n = 0;
+      for (const HostScene::HShape& hs : sc.hshapes_)
+        if (sc.bodies_[hs.body].type == type) {
+          candidates.push_back(hs.id);
+          n += 1;
+        }
+      (type == kBodyStatic ? d.n_ccd_static : type == kBodyKinematic ? d.n_ccd_kinematic : d.n_ccd_dynamic) = n;
+    }
     for (const HostScene::HShape& hs : sc.hshapes_) {
       fat.push_back(hs.fat);
+      aabbs.push_back(hs.aabb);
       moved.push_back(hs.proxy_moved ? 1 : 0);
       proxies.push_back(Proxy{hs.body, sc.bodies_[hs.body].type, hs.group, hs.speculative ? 1 : 0, hs.category, hs.mask});
     }
@@ -753,12 +809,18 @@ void BatchStepper::upload() {
   b.n_worlds = nw;
   b.enable_speculative = options.enable_speculative ? 1 : 0;
   b.restitution_iterations = options.restitution_iterations;
+  b.enable_continuous = options.enable_continuous ? 1 : 0;
+  b.any_bullets = any_bullets;
   b.sims = upload_vec(stream_, alloc<BodySim>(sims.size(), false), sims);
   b.states = upload_vec(stream_, alloc<BodyState>(states.size(), false), states);
   b.static_sims = upload_vec(stream_, alloc<BodySim>(statics.size(), false), statics);
   b.body_meta = upload_vec(stream_, alloc<BodyMeta>(bmeta.size(), false), bmeta);
   b.shapes = upload_vec(stream_, alloc<Shape>(shapes.size(), false), shapes);
   b.fat_aabbs = upload_vec(stream_, alloc<Aabb>(fat.size(), false), fat);
+  b.shape_aabbs = upload_vec(stream_, alloc<Aabb>(aabbs.size(), false), aabbs);
+  b.candidates = upload_vec(stream_, alloc<int32_t>(candidates.size(), false), candidates);
+  b.fast_ids = alloc<int32_t>(std::max<size_t>(sims.size(), 1), false);
+  b.awake_world = upload_vec(stream_, alloc<int32_t>(std::max<size_t>(awake_world.size(), 1), false), awake_world);
   b.shape_moved = upload_vec(stream_, alloc<uint8_t>(moved.size(), false), moved);
   b.proxies = upload_vec(stream_, alloc<Proxy>(proxies.size(), false), proxies);
   b.hulls = upload_vec(stream_, alloc<BoxHull>(hulls.size(), false), hulls);
@@ -778,7 +840,7 @@ void BatchStepper::upload() {
   b.worlds = upload_vec(stream_, alloc<BatchWorldDesc>(nw, false), desc_);
   b.wstate = alloc<BatchWorldState>(nw, true);
   b.order = upload_vec(stream_, alloc<int32_t>(nw, false), order);
-  b.counters = alloc<int32_t>(8, true);
+  b.counters = alloc<int32_t>(kCounterCount, true);
   b.work = alloc<unsigned long long>(4, true);
   BATCH_CHECK(hipStreamSynchronize(stream_));  // the host vectors are read by the copies
   total_awake_ = (int64_t)sims.size();

@@ -1,6 +1,7 @@
 // extern "C" boundary of the GPU stepper (declared in include/natsel_gpu.h).
 #include <exception>
 #include <new>
+#include <cstddef>
 #include <cstring>
 #include <string>
 
@@ -82,6 +83,8 @@ nsl::BodyDesc body_desc(const nsl_body_def* def) {
   d.angular_damping = def->angular_damping;
   d.gravity_scale = def->gravity_scale;
   d.enable_sleep = def->enable_sleep != 0;
+  d.is_bullet = def->is_bullet != 0;
+  d.safety_factor = def->safety_factor;
   return d;
 }
 
@@ -182,6 +185,53 @@ void nsl_default_revolute_joint_def(nsl_revolute_joint_def* out) {
   out->local_frame_b[6] = 1.0f;
   out->constraint_hertz = d.constraint_hertz;
   out->constraint_damping_ratio = d.constraint_damping_ratio;
+}
+
+void nsl_default_body_def(nsl_body_def* out) {
+  if (!out) return;
+  nsl::BodyDesc d;
+  std::memset(out, 0, sizeof(*out));
+  out->type = d.type;
+  out->rotation[3] = 1.0f;
+  out->linear_damping = d.linear_damping;
+  out->angular_damping = d.angular_damping;
+  out->gravity_scale = d.gravity_scale;
+  out->enable_sleep = d.enable_sleep ? 1 : 0;
+  out->is_bullet = d.is_bullet ? 1 : 0;
+  out->safety_factor = d.safety_factor;
+}
+
+static_assert(sizeof(nsl_toi_input) == sizeof(nsl::ToiQuery), "toi input layout");
+static_assert(offsetof(nsl_toi_input, sweep_a) == offsetof(nsl::ToiQuery, sweep_a), "toi input layout");
+static_assert(offsetof(nsl_toi_input, max_fraction) == offsetof(nsl::ToiQuery, max_fraction), "toi input layout");
+static_assert(sizeof(nsl_toi_output) == sizeof(nsl::ToiOutput), "toi output layout");
+static_assert(offsetof(nsl_toi_output, root_iterations) == offsetof(nsl::ToiOutput, root_iterations), "toi output layout");
+
+int nsl_time_of_impact(int device, const nsl_toi_input* in, nsl_toi_output* out, int32_t n) {
+  if (n < 0 || (n > 0 && (!in || !out))) return fail(NSL_ERR_INVALID, "null argument or negative count");
+  for (int32_t k = 0; k < n; ++k)
+    if (in[k].count_a < 1 || in[k].count_a > 8 || in[k].count_b < 1 || in[k].count_b > 8)
+      return fail(NSL_ERR_INVALID, "a proxy needs 1 to 8 points");
+  if (n == 0) return NSL_OK;
+  return guarded([&] {
+    auto check = [](hipError_t e) {
+      if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+    };
+    check(hipSetDevice(device));
+    nsl::ToiQuery* dq = nullptr;
+    nsl::ToiOutput* dout = nullptr;
+    check(hipMalloc(&dq, (size_t)n * sizeof(nsl::ToiQuery)));
+    hipError_t e = hipMalloc(&dout, (size_t)n * sizeof(nsl::ToiOutput));
+    if (e == hipSuccess) e = hipMemcpy(dq, in, (size_t)n * sizeof(nsl::ToiQuery), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      nsl::launch_time_of_impact(n, dq, dout, nullptr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)n * sizeof(nsl::ToiOutput), hipMemcpyDeviceToHost);
+    (void)hipFree(dq);
+    if (dout) (void)hipFree(dout);
+    check(e);
+  });
 }
 
 int nsl_create_revolute_joint(nsl_world* world, const nsl_revolute_joint_def* def, int32_t* out_joint) {
@@ -287,6 +337,12 @@ int nsl_world_get_contact_colors(nsl_world* world, int32_t* color, int32_t* loca
   return NSL_OK;
 }
 
+int nsl_world_get_contact_shapes(nsl_world* world, int32_t* shape_a, int32_t* shape_b) {
+  if (!world || !shape_a || !shape_b) return fail(NSL_ERR_INVALID, "null argument");
+  world->impl->get_contact_shapes(shape_a, shape_b);
+  return NSL_OK;
+}
+
 int nsl_world_get_profile(nsl_world* world, nsl_profile* out) {
   if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
   const nsl::StepProfile& p = world->impl->profile();
@@ -345,6 +401,8 @@ int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out) {
   out->pair_tests = k.pair_tests;
   for (int i = 0; i < 4; ++i) out->host_ms[i] = k.host_ms[i];
   out->joints = k.joints;
+  out->toi_calls = k.toi_calls;
+  out->toi_hits = k.toi_hits;
   return NSL_OK;
 }
 

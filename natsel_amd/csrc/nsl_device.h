@@ -23,18 +23,34 @@ struct DeviceWorld {
   int32_t* collide_ids;     // contacts to collide this step
   int32_t* solve_ids;       // coloured contact ids, colour-major (slot order)
   int32_t* overflow_ids;    // overflow contact ids in graph order
-  int32_t* counters;        // [0] fast bodies flagged, [1] sat calls, [2] sat cache hits, [3] recycled,
-                            // [4] manifold points prepared (coloured), [5] overflow points,
-                            // [6] proxies moved (fat AABB enlarged) since the last pair update
+  int32_t* counters;        // kCounterCount work counters, see kCnt*
   PairSet pairs;            // keys of the shape pairs that have a contact
   JointSim* joints;         // by joint id
   int32_t* joint_ids;       // joints in the graph, colour-major; the overflow colour's last
   PairSet joint_pairs;      // body-pair keys of joints without collide_connected (no contact allowed)
+  Aabb* shape_aabbs;        // by shape id: the shape's bounds from its body's last finalize or sweep
+  int32_t* fast_ids;        // this step's fast bodies as fast_base + awake index (count: counters[kCntFastList])
+  int32_t fast_base;        // 0 for a world; the world's offset into a batch's awake bodies
   int32_t n_awake;
 };
 
+// Device work counters (DeviceWorld::counters).
+enum : int {
+  kCntFastBodies = 0,   // bodies flagged fast by finalize (cumulative)
+  kCntSatCalls = 1,
+  kCntSatHits = 2,
+  kCntRecycled = 3,
+  kCntPoints = 4,       // manifold points prepared (coloured)
+  kCntOverflowPoints = 5,
+  kCntMovedProxies = 6, // proxies moved (fat AABB enlarged) since the last pair update
+  kCntFastList = 7,     // length of this step's fast_ids list (reset by the step kernel)
+  kCntToiCalls = 8,     // time-of-impact queries (cumulative)
+  kCntToiHits = 9,      // bodies moved back to an impact (cumulative)
+  kCounterCount = 16
+};
+
 // Launches timed with HIP events when kernel timing is on.
-enum KernelClass : int { kKInit = 0, kKCollide, kKSolve, kKernelClassCount };
+enum KernelClass : int { kKInit = 0, kKCollide, kKSolve, kKContinuous, kKernelClassCount };
 
 // Phases inside the step kernel; timed with the device wall clock at each grid barrier.
 enum PhaseClass : int {

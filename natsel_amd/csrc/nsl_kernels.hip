@@ -223,6 +223,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
 
   {
     int points = 0, ovf_points = 0;
+    if (tid == 0) d.counters[kCntFastList] = 0;  // finalize lists this step's fast bodies
     for (int k = tid; k < plan.n_joints; k += nthreads) prepare_joint_item(d, d.joint_ids[k], p);
     for (int slot = tid; slot < plan.n_constraints; slot += nthreads) points += prepare_item(d, slot, p);
     for (int k = tid; k < n_ovf; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);

@@ -150,6 +150,13 @@ NSL_HD Quat normalize_quat(Quat q) {
   }
   return quat_identity();
 }
+// functions.rs:511-520: normalised linear blend along the shorter arc
+NSL_HD Quat nlerp(Quat q1, Quat q2, float t) {
+  if (dot_quat(q1, q2) < 0.0f) q1 = Quat{{-q1.v.x, -q1.v.y, -q1.v.z}, -q1.s};
+  Vec3 v = lerp(q1.v, q2.v, t);
+  float s = (1.0f - t) * q1.s + t * q2.s;
+  return normalize_quat(Quat{v, s});
+}
 NSL_HD Quat integrate_rotation(Quat q1, Vec3 dr) {
   Quat qd = mul_quat(Quat{scale(0.5f, dr), 0.0f}, q1);
   return normalize_quat(Quat{add(q1.v, qd.v), qd.s + q1.s});
@@ -287,6 +294,7 @@ NSL_HD Aabb aabb_inflate(Aabb a, float e) {
   Vec3 r = {e, e, e};
   return {sub(a.lo, r), add(a.hi, r)};
 }
+NSL_HD Aabb aabb_union(Aabb a, Aabb b) { return {min3(a.lo, b.lo), max3(a.hi, b.hi)}; }
 NSL_HD bool aabb_overlaps(Aabb a, Aabb b) {
   if (a.hi.x < b.lo.x || a.lo.x > b.hi.x) return false;
   if (a.hi.y < b.lo.y || a.lo.y > b.hi.y) return false;

@@ -85,6 +85,7 @@ class HostScene {
     bool speculative = true;
     bool proxy_moved = false;
     Aabb fat{};
+    Aabb aabb{};  // the shape's bounds at its body's last finalize or sweep
   };
   int32_t encode(const HBody& b) const { return b.set == 2 ? b.local : -(b.local + 2); }
 

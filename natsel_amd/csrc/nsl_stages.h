@@ -946,8 +946,8 @@ __device__ __forceinline__ void overflow_store_item(const DeviceWorld& d, int k)
 // ------------------------------------------------------------------------------
 // finalize (solver_integration.rs:587-799)
 
-__device__ Aabb shape_local_aabb(const DeviceWorld& d, const Shape& s, Quat q) {
-  Xf t = Xf{zero3(), q};
+// compute_shape_aabb (shape_runtime.rs:370-387) at a transform
+__device__ Aabb shape_aabb_at(const DeviceWorld& d, const Shape& s, Xf t) {
   if (s.kind == kShapeHull) return aabb_transform(t, d.hulls[s.hull].aabb);
   float r = s.radius;
   Vec3 e = {r, r, r};
@@ -957,6 +957,9 @@ __device__ Aabb shape_local_aabb(const DeviceWorld& d, const Shape& s, Quat q) {
   }
   Vec3 c1 = xf_point(t, s.c1), c2 = xf_point(t, s.c2);
   return Aabb{sub(min3(c1, c2), e), add(max3(c1, c2), e)};
+}
+__device__ __forceinline__ Aabb shape_local_aabb(const DeviceWorld& d, const Shape& s, Quat q) {
+  return shape_aabb_at(d, s, Xf{zero3(), q});
 }
 
 __device__ __forceinline__ void finalize_item(const DeviceWorld& d, int i, const StepParams& p) {
@@ -986,7 +989,9 @@ __device__ __forceinline__ void finalize_item(const DeviceWorld& d, int i, const
     float max_motion = maxf(max_delta_position, max_velocity * time_step);
     if (sim.body_type == kBodyDynamic && p.enable_continuous && max_motion > sim.safety_factor * sim.min_extent) {
       sim.flags |= bflag::kIsFast;
-      atomicAdd(&d.counters[0], 1);
+      atomicAdd(&d.counters[kCntFastBodies], 1);
+      // the continuous pass sweeps it and updates its bounds (solver_integration.rs:696-710)
+      d.fast_ids[atomicAdd(&d.counters[kCntFastList], 1)] = d.fast_base + i;
     } else {
       sim.center0 = sim.center;
       sim.rotation0 = sim.transform.q;
@@ -1002,10 +1007,11 @@ __device__ __forceinline__ void finalize_item(const DeviceWorld& d, int i, const
     for (int32_t sid = sim.head_shape; sid != kNullIndex; sid = d.shapes[sid].next_shape) {
       const Shape& s = d.shapes[sid];
       Aabb aabb = offset_aabb(aabb_inflate(shape_local_aabb(d, s, sim.transform.q), kSpeculativeDistance), sim.transform.p);
+      d.shape_aabbs[sid] = aabb;
       if (!aabb_contains(d.fat_aabbs[sid], aabb)) {
         d.fat_aabbs[sid] = aabb_inflate(aabb, s.aabb_margin);
         d.shape_moved[sid] = 1;
-        atomicAdd(&d.counters[6], 1);
+        atomicAdd(&d.counters[kCntMovedProxies], 1);
       }
     }
   }

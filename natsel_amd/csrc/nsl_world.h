@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "nsl_broadphase.h"
+#include "nsl_ccd.h"
 #include "nsl_device.h"
 #include "nsl_scene.h"
 
@@ -52,6 +53,8 @@ class GpuWorld : public HostScene {
   // Per contact id: colour (-1 when not in the graph), local index and liveness.
   int32_t contact_capacity() const { return (int32_t)hcontacts_.size(); }
   void get_contact_colors(int32_t* color, int32_t* local, int32_t* alive) const;
+  // Per contact id: its shapes, -1 when the id is free.
+  void get_contact_shapes(int32_t* shape_a, int32_t* shape_b) const;
   const StepProfile& profile() const { return profile_; }
 
   // Device time per launch class (HIP events on the world's stream) and per phase of the
@@ -63,6 +66,7 @@ class GpuWorld : public HostScene {
     double phase_ms[kPhaseClassCount];
     int64_t phase_count[kPhaseClassCount];
     int64_t steps, bodies, collided, constraints, points, overflow_points, recycled, sat_calls, sat_hits, fast_bodies;
+    int64_t toi_calls, toi_hits;  // continuous collision: time-of-impact queries, bodies moved back
     int64_t barriers, solver_blocks, pair_tests;
     double host_ms[4];  // host wall time: pair update, collide (with its wait), state pass, solve+finalize (with its wait)
     int64_t joints;
@@ -129,8 +133,8 @@ class GpuWorld : public HostScene {
   int max_blocks_ = 0;
   double wall_clock_khz_ = 0.0;
   KernelStats kstats_{};
-  int32_t counters_host_[8] = {};
-  int32_t counters_base_[8] = {};
+  int32_t counters_host_[kCounterCount] = {};
+  int32_t counters_base_[kCounterCount] = {};
 
   std::vector<HContact> hcontacts_;
   std::vector<int32_t> contact_free_;
@@ -152,6 +156,14 @@ class GpuWorld : public HostScene {
   std::vector<uint8_t> change_host_;
   std::vector<uint8_t> moved_host_;  // staging for uploads and pulls
   std::vector<Aabb> fat_host_;
+  std::vector<Aabb> aabb_host_;
+  size_t cap_aabbs_ = 0;
+  // continuous collision: candidate shape ids (static, kinematic, dynamic; each ascending)
+  // and the fast-body list the step kernel fills
+  int32_t* d_candidates_ = nullptr;
+  size_t cap_candidates_ = 0, cap_fast_ = 0;
+  ContinuousArgs ccd_{};
+  int32_t bullet_bodies_ = 0;  // dynamic bullet bodies (the bullet pass runs only if any)
   // device pair finder and the pair-key set it checks (new contacts insert, disjoint ones erase)
   std::unique_ptr<PairFinder> finder_;
   Proxy* d_proxies_ = nullptr;

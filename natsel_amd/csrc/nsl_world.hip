@@ -29,8 +29,8 @@ GpuWorld::GpuWorld(int device) : device_(device) {
   timer_.capacity = 64;
   timer_.events = new hipEvent_t[2 * timer_.capacity];
   for (int i = 0; i < 2 * timer_.capacity; ++i) NSL_CHECK(hipEventCreate(&timer_.events[i]));
-  NSL_CHECK(hipMalloc(&d_.counters, 8 * sizeof(int32_t)));
-  NSL_CHECK(hipMemset(d_.counters, 0, 8 * sizeof(int32_t)));
+  NSL_CHECK(hipMalloc(&d_.counters, kCounterCount * sizeof(int32_t)));
+  NSL_CHECK(hipMemset(d_.counters, 0, kCounterCount * sizeof(int32_t)));
   NSL_CHECK(hipMalloc(&d_barrier_, sizeof(GridBarrier)));
   NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
   NSL_CHECK(hipMalloc(&d_phase_log_, 2 * kMaxPhaseLog * sizeof(uint64_t)));
@@ -51,7 +51,7 @@ GpuWorld::~GpuWorld() {
                   d_.contact_change, d_.constraints, d_.overflow, d_.collide_ids, d_.solve_ids, d_.overflow_ids,
                   d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
-                  d_ovf_items_, d_ovf_levels_};
+                  d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -102,6 +102,7 @@ void GpuWorld::upload_bodies() {
   ensure(d_.sims, cap_sims_, std::max<size_t>(na, 1), false);
   ensure(d_.states, cap_states_, std::max<size_t>(na, 1), false);
   ensure(d_.static_sims, cap_static_, std::max<size_t>(ns, 1), false);
+  ensure(d_.fast_ids, cap_fast_, std::max<size_t>(na, 1), false);
   if (na) {
     NSL_CHECK(hipMemcpyAsync(d_.sims, awake_host_.data(), na * sizeof(BodySim), hipMemcpyHostToDevice, stream_));
     NSL_CHECK(hipMemcpyAsync(d_.states, states_host_.data(), na * sizeof(BodyState), hipMemcpyHostToDevice, stream_));
@@ -114,13 +115,26 @@ void GpuWorld::upload_bodies() {
     ensure(d_.fat_aabbs, cap_fat_, std::max<size_t>(nsh, 1), false);
     ensure(d_.shape_moved, cap_moved_, std::max<size_t>(nsh, 1), false);
     ensure(d_proxies_, cap_proxies_, std::max<size_t>(nsh, 1), false);
+    ensure(d_.shape_aabbs, cap_aabbs_, std::max<size_t>(nsh, 1), false);
+    ensure(d_candidates_, cap_candidates_, std::max<size_t>(nsh, 1), false);
     ensure(d_.hulls, cap_hulls_, std::max<size_t>(hulls_.size(), 1), false);
     fat_host_.resize(nsh);
+    aabb_host_.resize(nsh);
     moved_host_.resize(nsh);
     proxies_host_.resize(nsh);
+    std::vector<int32_t> candidates;  // static, kinematic, dynamic shapes (continuous.rs:424-445)
+    int32_t per_type[3] = {0, 0, 0};
+    candidates.reserve(nsh);
+    for (int type = kBodyStatic; type <= kBodyDynamic; ++type)
+      for (size_t i = 0; i < nsh; ++i)
+        if (bodies_[hshapes_[i].body].type == type) {
+          candidates.push_back((int32_t)i);
+          per_type[type] += 1;
+        }
     for (size_t i = 0; i < nsh; ++i) {
       const HShape& hs = hshapes_[i];
       fat_host_[i] = hs.fat;
+      aabb_host_[i] = hs.aabb;
       moved_host_[i] = hs.proxy_moved ? 1 : 0;
       pairs_pending_ = pairs_pending_ || hs.proxy_moved;
       proxies_host_[i] = Proxy{hs.body, bodies_[hs.body].type, hs.group, 0, hs.category, hs.mask};
@@ -128,6 +142,8 @@ void GpuWorld::upload_bodies() {
     if (nsh) {
       NSL_CHECK(hipMemcpyAsync(d_.shapes, shapes_.data(), nsh * sizeof(Shape), hipMemcpyHostToDevice, stream_));
       NSL_CHECK(hipMemcpyAsync(d_.fat_aabbs, fat_host_.data(), nsh * sizeof(Aabb), hipMemcpyHostToDevice, stream_));
+      NSL_CHECK(hipMemcpyAsync(d_.shape_aabbs, aabb_host_.data(), nsh * sizeof(Aabb), hipMemcpyHostToDevice, stream_));
+      NSL_CHECK(hipMemcpyAsync(d_candidates_, candidates.data(), nsh * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
       NSL_CHECK(hipMemcpyAsync(d_.shape_moved, moved_host_.data(), nsh, hipMemcpyHostToDevice, stream_));
       NSL_CHECK(hipMemcpyAsync(d_proxies_, proxies_host_.data(), nsh * sizeof(Proxy), hipMemcpyHostToDevice, stream_));
     }
@@ -137,7 +153,11 @@ void GpuWorld::upload_bodies() {
     NSL_CHECK(hipStreamSynchronize(stream_));
     n_device_shapes_ = nsh;
     shapes_dirty_ = false;
+    ccd_ = ContinuousArgs{d_candidates_, per_type[kBodyStatic], per_type[kBodyKinematic], per_type[kBodyDynamic], d_proxies_};
   }
+  bullet_bodies_ = 0;
+  for (const BodySim& s : awake_host_)
+    if (s.body_type == kBodyDynamic && (s.flags & bflag::kIsBullet)) bullet_bodies_ += 1;
   bodies_dirty_ = false;
   device_newer_ = false;
 }
@@ -151,12 +171,15 @@ void GpuWorld::pull_bodies() {
   }
   if (n_device_shapes_) {  // finalize moves fat AABBs and marks proxies on the device
     fat_host_.resize(n_device_shapes_);
+    aabb_host_.resize(n_device_shapes_);
     moved_host_.resize(n_device_shapes_);
     NSL_CHECK(hipMemcpyAsync(fat_host_.data(), d_.fat_aabbs, n_device_shapes_ * sizeof(Aabb), hipMemcpyDeviceToHost, stream_));
+    NSL_CHECK(hipMemcpyAsync(aabb_host_.data(), d_.shape_aabbs, n_device_shapes_ * sizeof(Aabb), hipMemcpyDeviceToHost, stream_));
     NSL_CHECK(hipMemcpyAsync(moved_host_.data(), d_.shape_moved, n_device_shapes_, hipMemcpyDeviceToHost, stream_));
     NSL_CHECK(hipStreamSynchronize(stream_));
     for (size_t i = 0; i < n_device_shapes_; ++i) {
       hshapes_[i].fat = fat_host_[i];
+      hshapes_[i].aabb = aabb_host_[i];
       hshapes_[i].proxy_moved = moved_host_[i] != 0;
     }
   }
@@ -530,12 +553,13 @@ struct GpuWorld::Snapshot {
   BodySim* sims = nullptr;
   BodyState* states = nullptr;
   Aabb* fat = nullptr;
+  Aabb* aabbs = nullptr;
   uint8_t* moved = nullptr;
   Contact* contacts = nullptr;
   uint64_t* pair_slots = nullptr;
   uint32_t pair_mask = 0;
   JointSim* joints = nullptr;
-  int32_t counters[8] = {};
+  int32_t counters[kCounterCount] = {};
   std::vector<HBody> bodies;
   std::vector<BodySim> awake_host;
   std::vector<BodyState> states_host;
@@ -544,7 +568,7 @@ struct GpuWorld::Snapshot {
   std::vector<int32_t> contact_free, non_touching;
   int32_t contact_pool_next = 0;
   ColorSet colors[kColorCount];
-  int32_t counters_host[8] = {}, counters_base[8] = {};
+  int32_t counters_host[kCounterCount] = {}, counters_base[kCounterCount] = {};
   int64_t pair_set_tombs = 0;
   bool pairs_pending = false;
   uint64_t contact_serial = 0;
@@ -579,6 +603,7 @@ GpuWorld::Snapshot* GpuWorld::save_state() {
   s->sims = s->copy_of(d_.sims, s->n_awake, stream_);
   s->states = s->copy_of(d_.states, s->n_awake, stream_);
   s->fat = s->copy_of(d_.fat_aabbs, n_device_shapes_, stream_);
+  s->aabbs = s->copy_of(d_.shape_aabbs, n_device_shapes_, stream_);
   s->moved = s->copy_of(d_.shape_moved, n_device_shapes_, stream_);
   s->contacts = s->copy_of(d_.contacts, s->n_contacts, stream_);
   if (d_.pairs.slots) {
@@ -617,6 +642,7 @@ void GpuWorld::restore_state(const Snapshot& s) {
   back(d_.sims, s.sims, s.n_awake);
   back(d_.states, s.states, s.n_awake);
   back(d_.fat_aabbs, s.fat, s.n_device_shapes);
+  back(d_.shape_aabbs, s.aabbs, s.n_device_shapes);
   back(d_.shape_moved, s.moved, s.n_device_shapes);
   ensure(d_.contacts, cap_contacts_, std::max<size_t>(s.n_contacts, 1), false);
   back(d_.contacts, s.contacts, s.n_contacts);
@@ -780,6 +806,11 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   int blocks = options.solver_blocks > 0 ? options.solver_blocks : (widest + kSolveBlock - 1) / kSolveBlock;
   blocks = std::max(1, std::min(blocks, std::min(max_blocks_, 128)));
   launch_solve_step(d_, plan, p, blocks, stream_, tm);
+  // continuous collision of the bodies finalize flagged fast, then of the bullets
+  if (options.enable_continuous) {
+    launch_continuous(d_, ccd_, p, false, d_.n_awake, stream_, tm);
+    if (bullet_bodies_ > 0) launch_continuous(d_, ccd_, p, true, d_.n_awake, stream_, tm);
+  }
   device_newer_ = true;
 
   // counters[6] (proxies moved by finalize) tells the next pair update whether to run
@@ -824,6 +855,8 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   kstats_.recycled = counters_host_[3] - counters_base_[3];
   kstats_.points = counters_host_[4] - counters_base_[4];
   kstats_.overflow_points = counters_host_[5] - counters_base_[5];
+  kstats_.toi_calls = counters_host_[kCntToiCalls] - counters_base_[kCntToiCalls];
+  kstats_.toi_hits = counters_host_[kCntToiHits] - counters_base_[kCntToiHits];
   const double t4 = now_ms();
   kstats_.host_ms[0] += t1 - t0;
   kstats_.host_ms[1] += t2 - t1;
@@ -877,6 +910,13 @@ void GpuWorld::get_contact_colors(int32_t* color, int32_t* local, int32_t* alive
     color[i] = hcontacts_[i].color;
     local[i] = hcontacts_[i].local;
     alive[i] = hcontacts_[i].alive;
+  }
+}
+
+void GpuWorld::get_contact_shapes(int32_t* shape_a, int32_t* shape_b) const {
+  for (size_t i = 0; i < hcontacts_.size(); ++i) {
+    shape_a[i] = hcontacts_[i].alive ? hcontacts_[i].shape_a : kNullIndex;
+    shape_b[i] = hcontacts_[i].alive ? hcontacts_[i].shape_b : kNullIndex;
   }
 }
 

@@ -34,6 +34,8 @@ class BodySpec:
     linear_damping: float = 0.0
     angular_damping: float = 0.0
     gravity_scale: float = 1.0
+    is_bullet: bool = False
+    safety_factor: float = 0.5
 
 
 @dataclass
@@ -671,3 +673,65 @@ def run_rollback(world, steps: int = 200, dt: float = 1.0 / 60.0, substeps: int 
             states.clear()
         world.step(dt, substeps)
     return world
+
+
+# ---------------------------------------------------------------------------------
+# Continuous collision: fast bodies and bullets against thin walls and floors
+
+
+def ccd_wall(bullet: bool) -> Scene:
+    """A 0.4 m box fired at 200 m/s at a 0.2 m static wall, one substep per step
+    (ccd_tests.rs:30-55). The box is the last body."""
+    sc = Scene(name=f"ccd_wall_{'bullet' if bullet else 'fast'}", enable_sleep=False, substeps=1)
+    w = sc.add_body(BodySpec(STATIC, (0.0, 0.0, 0.0)))
+    sc.add_shape(ShapeSpec(w, BOX, (0.1, 2.0, 2.0)))
+    b = sc.add_body(BodySpec(DYNAMIC, (-5.0, 0.0, 0.0), linear_velocity=(200.0, 0.0, 0.0), is_bullet=bullet))
+    sc.add_shape(ShapeSpec(b, BOX, (0.2, 0.2, 0.2), density=1000.0))
+    return sc
+
+
+def bullet_wall(offset: float = 0.0, enable_continuous: bool = True) -> Scene:
+    """A 0.1 m sphere bullet at 200 m/s, gravity off, at a 0.1 m static wall 5 m away, with
+    every position offset by `offset` (large_world_tests.rs:169-200;
+    world_tuning_tests.rs:90-113 with continuous collision off). The bullet is the last body."""
+    sc = Scene(name=f"bullet_wall_{offset:g}", enable_continuous=enable_continuous)
+    w = sc.add_body(BodySpec(STATIC, (offset + 5.0, offset, offset)))
+    sc.add_shape(ShapeSpec(w, BOX, (0.05, 5.0, 5.0)))
+    b = sc.add_body(BodySpec(DYNAMIC, (offset, offset, offset), linear_velocity=(200.0, 0.0, 0.0), gravity_scale=0.0,
+                             is_bullet=True))
+    sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, 0.1), density=1.0))
+    return sc
+
+
+def overshoot(continuous: bool) -> Scene:
+    """A perfectly elastic unit cube dropped flat from 10 m on a floor narrower than itself,
+    four restitution iterations; with `continuous` its safety factor is 0.1 so it is swept
+    onto the floor (measure_overshoot, restitution_phase_tests.rs:311-380). The cube is the
+    last body."""
+    sc = Scene(name=f"overshoot_{continuous}", enable_sleep=False, restitution_iterations=4)
+    f = sc.add_body(BodySpec(STATIC, (0.0, -0.25, 0.0)))
+    sc.add_shape(ShapeSpec(f, BOX, (0.375, 0.25, 0.375)))
+    b = sc.add_body(BodySpec(DYNAMIC, (0.0, 10.0, 0.0), safety_factor=0.1 if continuous else 0.5))
+    sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), restitution=1.0))
+    return sc
+
+
+def fast_and_bullets(cubes: int = 640, shooters: int = 24) -> Scene:
+    """A grid of cubes on a ground between two static walls, with small spheres fired at
+    300 m/s across it, every third one a bullet (build_fast_and_sleepy,
+    parallel_solver_tests.rs:278-326; sleeping is not modelled here, so bodies stay awake)."""
+    sc = Scene(name="fast_and_bullets", enable_sleep=True)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -1.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (100.0, 1.0, 100.0)))
+    for x in (-16.0, 16.0):
+        w = sc.add_body(BodySpec(STATIC, (x, 4.0, 0.0)))
+        sc.add_shape(ShapeSpec(w, BOX, (0.25, 4.0, 20.0)))
+    for i in range(cubes):
+        b = sc.add_body(BodySpec(DYNAMIC, (float((i % 16) - 7.5), 0.4 + (i // 256) * 0.8, float(((i // 16) % 16) - 7.5))))
+        sc.add_shape(ShapeSpec(b, BOX, (0.4, 0.4, 0.4)))
+    for i in range(shooters):
+        v = (300.0 if i % 2 == 0 else -300.0, 0.0, float(i % 3) - 1.0)
+        b = sc.add_body(BodySpec(DYNAMIC, (-12.0 if i % 2 == 0 else 12.0, 1.0 + (i // 2) * 0.5, float((i % 5) - 2)),
+                                 linear_velocity=v, is_bullet=i % 3 == 0))
+        sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, 0.1)))
+    return sc

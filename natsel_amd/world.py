@@ -30,6 +30,8 @@ class BodyDef:  # src/types/body_def.rs:80-101 defaults
     angular_damping: float = 0.0
     gravity_scale: float = 1.0
     enable_sleep: bool = True
+    is_bullet: bool = False
+    safety_factor: float = 0.5
 
 
 @dataclass
@@ -158,6 +160,8 @@ class World:
         c.angular_velocity[:] = d.angular_velocity
         c.linear_damping, c.angular_damping, c.gravity_scale = d.linear_damping, d.angular_damping, d.gravity_scale
         c.enable_sleep = int(d.enable_sleep)
+        c.is_bullet = int(d.is_bullet)
+        c.safety_factor = d.safety_factor
         out = ctypes.c_int32()
         capi.check(self._lib.nsl_create_body(self._h, ctypes.byref(c), ctypes.byref(out)))
         return out.value
@@ -255,6 +259,15 @@ class World:
         capi.check(self._lib.nsl_world_get_contact_colors(self._h, f(color), f(local), f(alive)))
         return color, local, alive
 
+    def contact_shapes(self):
+        """Per contact id: the two shape ids (-1 when the id is free)."""
+        n = ctypes.c_int32()
+        capi.check(self._lib.nsl_world_contact_capacity(self._h, ctypes.byref(n)))
+        a, b = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        f = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        capi.check(self._lib.nsl_world_get_contact_shapes(self._h, f(a), f(b)))
+        return a, b
+
     def set_kernel_timing(self, on: bool) -> None:
         capi.check(self._lib.nsl_world_set_kernel_timing(self._h, int(on)))
 
@@ -288,7 +301,8 @@ def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0) -> World:
     ids = []
     for b in scene.bodies:
         ids.append(w.create_body(BodyDef(b.body_type, b.position, b.rotation, b.linear_velocity, b.angular_velocity,
-                                         b.linear_damping, b.angular_damping, b.gravity_scale, b.enable_sleep)))
+                                         b.linear_damping, b.angular_damping, b.gravity_scale, b.enable_sleep, b.is_bullet,
+                                         b.safety_factor)))
     for s in scene.shapes:
         d = ShapeDef(s.density, s.friction, s.restitution)
         if s.kind == BOX:

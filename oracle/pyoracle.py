@@ -39,7 +39,7 @@ def lib():
         L.nslo_set_options.argtypes = [P, I, I, I, I]
         L.nslo_set_gravity.argtypes = [P, F, F, F]
         L.nslo_set_restitution.argtypes = [P, I, F, I]
-        L.nslo_create_body.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), fp, fp, fp, I, F, F, F]
+        L.nslo_create_body.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), fp, fp, fp, I, F, F, F, I, F]
         L.nslo_create_box.argtypes = [P, I, F, F, F, F, F, F]
         L.nslo_create_sphere.argtypes = [P, I, fp, F, F, F, F]
         L.nslo_create_capsule.argtypes = [P, I, fp, fp, F, F, F, F]
@@ -63,6 +63,7 @@ def lib():
         L.nslo_save_state.argtypes = [P]
         L.nslo_save_state.restype = P
         L.nslo_restore_state.argtypes = [P, P]
+        L.nslo_time_of_impact.argtypes = [fp, I, F, fp, fp, I, F, fp, F, fp, ctypes.POINTER(ctypes.c_int32)]
         _LIB = L
     return _LIB
 
@@ -74,6 +75,29 @@ def set_threads(n: int) -> None:
 
 def max_threads() -> int:
     return int(lib().nslo_max_threads())
+
+
+TOI_STATES = ("unknown", "failed", "overlapped", "hit", "separated")
+
+
+def time_of_impact(points_a, radius_a, sweep_a, points_b, radius_b, sweep_b, max_fraction=1.0) -> dict:
+    """time_of_impact (time_of_impact.rs:406-598) on two point proxies. A sweep is
+    (local_center, c1, c2, q1, q2) with quaternions as (x, y, z, s)."""
+    L = lib()
+    fp = ctypes.POINTER(ctypes.c_float)
+    pa = np.ascontiguousarray(points_a, np.float32).reshape(-1, 3)
+    pb = np.ascontiguousarray(points_b, np.float32).reshape(-1, 3)
+    sa = np.concatenate([np.asarray(x, np.float32).ravel() for x in sweep_a])
+    sb = np.concatenate([np.asarray(x, np.float32).ravel() for x in sweep_b])
+    assert sa.size == 17 and sb.size == 17 and 1 <= len(pa) <= 8 and 1 <= len(pb) <= 8
+    fout = np.zeros(8, np.float32)
+    iout = np.zeros(4, np.int32)
+    L.nslo_time_of_impact(pa.ctypes.data_as(fp), len(pa), float(radius_a), sa.ctypes.data_as(fp),
+                          pb.ctypes.data_as(fp), len(pb), float(radius_b), sb.ctypes.data_as(fp), float(max_fraction),
+                          fout.ctypes.data_as(fp), iout.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return {"state": TOI_STATES[int(iout[0])], "point": fout[0:3].copy(), "normal": fout[3:6].copy(),
+            "fraction": np.float32(fout[6]), "distance": np.float32(fout[7]), "distance_iterations": int(iout[1]),
+            "push_back_iterations": int(iout[2]), "root_iterations": int(iout[3])}
 
 
 class _OracleSnapshot:
@@ -109,12 +133,13 @@ class OracleWorld:
             pass
 
     def create_body(self, body_type, position, rotation=(0, 0, 0, 1), v=(0, 0, 0), w=(0, 0, 0), enable_sleep=True,
-                    linear_damping=0.0, angular_damping=0.0, gravity_scale=1.0) -> int:
+                    linear_damping=0.0, angular_damping=0.0, gravity_scale=1.0, is_bullet=False, safety_factor=0.5) -> int:
         p = (ctypes.c_double * 3)(*position)
         q = (ctypes.c_float * 4)(*rotation)
         lv = (ctypes.c_float * 3)(*v)
         av = (ctypes.c_float * 3)(*w)
-        return self.L.nslo_create_body(self.h, body_type, p, q, lv, av, int(enable_sleep), linear_damping, angular_damping, gravity_scale)
+        return self.L.nslo_create_body(self.h, body_type, p, q, lv, av, int(enable_sleep), linear_damping, angular_damping,
+                                       gravity_scale, int(is_bullet), float(safety_factor))
 
     def create_box(self, body, hx, hy, hz, density=1000.0, friction=0.6, restitution=0.0) -> int:
         return self.L.nslo_create_box(self.h, body, hx, hy, hz, density, friction, restitution)
@@ -168,10 +193,10 @@ class OracleWorld:
         return pos, rot, v, w
 
     def stats(self) -> dict:
-        out = (ctypes.c_int64 * 9)()
+        out = (ctypes.c_int64 * 11)()
         self.L.nslo_get_stats(self.h, out)
         keys = ["pairs_created", "contacts_destroyed", "begin_touch", "end_touch", "recycled", "sat_calls", "sat_hits",
-                "fast_bodies", "overflow_constraints"]
+                "fast_bodies", "overflow_constraints", "toi_calls", "toi_hits"]
         return dict(zip(keys, list(out)))
 
     def phase_ms(self) -> dict:
@@ -208,7 +233,8 @@ def load_scene(scene) -> OracleWorld:
     w.L.nslo_set_restitution(w.h, int(scene.restitution_iterations), float(scene.restitution_threshold),
                              int(scene.restitution_propagation))
     ids = [w.create_body(b.body_type, b.position, b.rotation, b.linear_velocity, b.angular_velocity, b.enable_sleep,
-                         b.linear_damping, b.angular_damping, b.gravity_scale) for b in scene.bodies]
+                         b.linear_damping, b.angular_damping, b.gravity_scale, b.is_bullet, b.safety_factor)
+           for b in scene.bodies]
     for s in scene.shapes:
         if s.kind == BOX:
             w.create_box(ids[s.body], *s.params, s.density, s.friction, s.restitution)

@@ -35,8 +35,11 @@ void nslo_set_restitution(void* wp, int iterations, float threshold, int propaga
 }
 
 int nslo_create_body(void* wp, int type, const double* pos, const float* rot, const float* v, const float* av,
-                     int enable_sleep, float linear_damping, float angular_damping, float gravity_scale) {
+                     int enable_sleep, float linear_damping, float angular_damping, float gravity_scale,
+                     int is_bullet, float safety_factor) {
   BodyDef d;
+  d.is_bullet = is_bullet != 0;
+  d.safety_factor = safety_factor;
   d.type = (BodyType)type;
   d.position = Pos{pos[0], pos[1], pos[2]};
   d.rotation = Quat{{rot[0], rot[1], rot[2]}, rot[3]};
@@ -134,11 +137,37 @@ void nslo_get_bodies(void* wp, double* pos, float* rot, float* v, float* av) {
   }
 }
 
-// [pairs_created, destroyed, begin_touch, end_touch, recycled, sat_calls, sat_hits, fast_bodies, overflow]
+// [pairs_created, destroyed, begin_touch, end_touch, recycled, sat_calls, sat_hits, fast_bodies, overflow,
+//  toi_calls, toi_hits]
 void nslo_get_stats(void* wp, int64_t* out) {
   const StepStats& s = static_cast<World*>(wp)->stats;
   out[0] = s.pairs_created; out[1] = s.contacts_destroyed; out[2] = s.begin_touch; out[3] = s.end_touch;
   out[4] = s.recycled; out[5] = s.sat_calls; out[6] = s.sat_hits; out[7] = s.fast_bodies; out[8] = s.overflow_constraints;
+  out[9] = s.toi_calls; out[10] = s.toi_hits;
+}
+
+// One time-of-impact query (time_of_impact.rs:406-598). Sweeps are 17 floats: local
+// centre, c1, c2, q1 (x, y, z, s), q2. fout: point xyz, normal xyz, fraction, distance;
+// iout: state, distance, push-back and root iterations.
+static Sweep sweep_of(const float* s) {
+  return Sweep{{s[0], s[1], s[2]}, {s[3], s[4], s[5]}, {s[6], s[7], s[8]}, {{s[9], s[10], s[11]}, s[12]},
+               {{s[13], s[14], s[15]}, s[16]}};
+}
+void nslo_time_of_impact(const float* points_a, int count_a, float radius_a, const float* sweep_a,
+                         const float* points_b, int count_b, float radius_b, const float* sweep_b,
+                         float max_fraction, float* fout, int32_t* iout) {
+  ToiProxy pa{}, pb{};
+  for (int i = 0; i < count_a && i < 8; ++i) pa.points[i] = Vec3{points_a[3 * i], points_a[3 * i + 1], points_a[3 * i + 2]};
+  for (int i = 0; i < count_b && i < 8; ++i) pb.points[i] = Vec3{points_b[3 * i], points_b[3 * i + 1], points_b[3 * i + 2]};
+  pa.count = count_a < 8 ? count_a : 8;
+  pb.count = count_b < 8 ? count_b : 8;
+  pa.radius = radius_a;
+  pb.radius = radius_b;
+  ToiOutput o = time_of_impact(pa, sweep_of(sweep_a), pb, sweep_of(sweep_b), max_fraction);
+  fout[0] = o.point.x; fout[1] = o.point.y; fout[2] = o.point.z;
+  fout[3] = o.normal.x; fout[4] = o.normal.y; fout[5] = o.normal.z;
+  fout[6] = o.fraction; fout[7] = o.distance;
+  iout[0] = o.state; iout[1] = o.distance_iterations; iout[2] = o.push_back_iterations; iout[3] = o.root_iterations;
 }
 
 // Cumulative phase wall times (ms): pairs, collide, prepare, solve, finalize.

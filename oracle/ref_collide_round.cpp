@@ -36,7 +36,6 @@ int point_support(const Vec3* pts, int count, Vec3 axis) {
 
 struct SimplexVertex { Vec3 wa, wb, w; float a; int32_t index_a, index_b; };
 struct Simplex { SimplexVertex v[4]; int32_t count; };
-struct DistanceOutput { Vec3 point_a, point_b, normal; float distance; int32_t iterations; };
 
 struct Bary2 { float u, v, d; };
 struct Bary3 { float u, v, w, d; };
@@ -333,6 +332,12 @@ inline void set_point(LocalManifold& m, int i, Vec3 p, float sep, FeaturePair pa
 }
 
 }  // namespace
+
+// Entry points of the GJK restatement for the time-of-impact query.
+int gjk_point_support(const Vec3* pts, int count, Vec3 axis) { return point_support(pts, count, axis); }
+DistanceOutput gjk_distance(const Vec3* pa, int na, const Vec3* pb, int nb, Xf xf, SimplexCache& cache) {
+  return shape_distance(pa, na, pb, nb, xf, cache);
+}
 
 // convex_manifold_shapes.rs:21-71
 void collide_spheres(LocalManifold& m, Sphere a, Sphere b, Xf b_to_a) {

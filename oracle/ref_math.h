@@ -142,6 +142,13 @@ inline Quat normalize_quat(Quat q) {
   }
   return kQuatIdentity;
 }
+// functions.rs:511-520: normalised linear blend along the shorter arc
+inline Quat nlerp(Quat q1, Quat q2, float t) {
+  if (dot_quat(q1, q2) < 0.0f) q1 = Quat{{-q1.v.x, -q1.v.y, -q1.v.z}, -q1.s};
+  Vec3 v = lerp(q1.v, q2.v, t);
+  float s = (1.0f - t) * q1.s + t * q2.s;
+  return normalize_quat(Quat{v, s});
+}
 // internal_math.rs:322-325
 inline Quat integrate_rotation(Quat q1, Vec3 dr) {
   Quat qd = mul_quat(Quat{scale(0.5f, dr), 0.0f}, q1);

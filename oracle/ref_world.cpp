@@ -841,6 +841,8 @@ void World::finalize_bodies(const StepContext& ctx) {
   const float spec = speculative_distance();
   const float time_step = ctx.dt, inv_time_step = ctx.inv_dt;
   std::vector<uint8_t> moved_flags(awake_sims.size(), 0);
+  // 1: fast body for the continuous pass in body order, 2: bullet (solver_integration.rs:696-710)
+  std::vector<uint8_t> continuous(awake_sims.size(), 0);
   // each item writes its own body, its own shapes and fat AABBs
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < awake_sims.size(); ++i) {
@@ -872,9 +874,8 @@ void World::finalize_bodies(const StepContext& ctx) {
       body.sleep_time = 0.0f;
       float max_motion = maxf(max_delta_position, max_velocity * time_step);
       if (body.type == kDynamic && enable_continuous && max_motion > body.safety_factor * sim.min_extent) {
-        // Continuous collision is not restated; callers that pin results assert this
-        // counter stays zero or run with continuous collision off.
         sim.flags |= BF::IS_FAST;
+        continuous[i] = (sim.flags & BF::IS_BULLET) ? 2 : 1;
         __atomic_fetch_add(&stats.fast_bodies, 1, __ATOMIC_RELAXED);
       } else {
         sim.center0 = sim.center;
@@ -900,6 +901,19 @@ void World::finalize_bodies(const StepContext& ctx) {
       }
     }
   }
+  // The serial tail in body order: fast bodies sweep against the static shapes, and the
+  // world inverse inertia follows a rotation moved back to the impact
+  // (solver_integration.rs:538-566). Bullets run afterwards and see the finished bodies
+  // (solver_integration.rs:948-995).
+  for (size_t i = 0; i < awake_sims.size(); ++i) {
+    if (continuous[i] != 1) continue;
+    solve_continuous((int32_t)i, time_step);
+    BodySim& sim = awake_sims[i];
+    Mat3 rm = matrix_from_quat(sim.transform.q);
+    sim.inv_inertia_world = mul_mm(mul_mm(rm, sim.inv_inertia_local), transpose(rm));
+  }
+  for (size_t i = 0; i < awake_sims.size(); ++i)
+    if (continuous[i] == 2) solve_continuous((int32_t)i, time_step);
 }
 
 // ---------------------------------------------------------------------------------

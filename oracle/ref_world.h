@@ -336,9 +336,28 @@ struct ShapeDef {
   bool enable_speculative = true, invoke_contact_creation = true, enable_hit_events = false;
 };
 
+// GJK (src/collision/distance.rs:483-709) with use_radii false, in frame A
+struct DistanceOutput { Vec3 point_a, point_b, normal; float distance; int32_t iterations; };
+int gjk_point_support(const Vec3* pts, int count, Vec3 axis);
+DistanceOutput gjk_distance(const Vec3* pa, int na, const Vec3* pb, int nb, Xf xf, SimplexCache& cache);
+
+// Time of impact (src/types/queries.rs:179-234, src/collision/time_of_impact.rs)
+struct Sweep { Vec3 local_center, c1, c2; Quat q1, q2; };
+struct ToiProxy { Vec3 points[8]; int count; float radius; };  // ShapeProxy (queries.rs)
+enum ToiState : int32_t { kToiUnknown = 0, kToiFailed, kToiOverlapped, kToiHit, kToiSeparated };
+struct ToiOutput {
+  int32_t state = kToiUnknown;
+  Vec3 point{}, normal{};
+  float fraction = -1.0f, distance = 0.0f;
+  int32_t distance_iterations = 0, push_back_iterations = 0, root_iterations = 0;
+};
+ToiOutput time_of_impact(ToiProxy proxy_a, Sweep sweep_a, ToiProxy proxy_b, Sweep sweep_b, float max_fraction);
+
 struct StepStats {
   int64_t pairs_created = 0, contacts_destroyed = 0, begin_touch = 0, end_touch = 0;
   int64_t recycled = 0, sat_calls = 0, sat_hits = 0, fast_bodies = 0, overflow_constraints = 0;
+  // continuous collision: time-of-impact queries run, and fast bodies moved back to an impact
+  int64_t toi_calls = 0, toi_hits = 0;
   // cumulative wall time in ms (types/world_def.rs:158-184 phases): pairs, collide
   // (narrow phase and state pass), prepare, solve, finalize
   double ms_pairs = 0, ms_collide = 0, ms_prepare = 0, ms_solve = 0, ms_finalize = 0;
@@ -453,6 +472,10 @@ class World {
   void restitution_overflow(const StepContext& ctx);
   void store_overflow();
   void finalize_bodies(const StepContext& ctx);
+  ToiProxy shape_proxy(const Shape& s) const;
+  Vec3 shape_centroid(const Shape& s) const;
+  bool is_shape_fast(const Shape& s, Vec3 centroid1, Vec3 centroid2, float rotation_chord, float safety_factor) const;
+  void solve_continuous(int32_t sim_index, float dt);
 };
 
 // narrow phase (src/collision/convex_manifold.rs, convex_manifold_shapes.rs)
@@ -463,5 +486,6 @@ void collide_capsules(LocalManifold& m, Capsule a, Capsule b, Xf b_to_a);
 void collide_hull_and_sphere(LocalManifold& m, const Hull& a, Sphere b, Xf b_to_a, SimplexCache& cache);
 void collide_hull_and_capsule(LocalManifold& m, const Hull& a, Capsule b, Xf b_to_a, SimplexCache& cache);
 uint32_t make_feature_id(FeaturePair p);
+
 
 }  // namespace nslref

@@ -63,8 +63,8 @@ def twist_angle(rot):
 
 def chain_stretch(load, links=8, steps=300):
     """hanging_chain: per step the largest distance between the two anchor points of any
-    joint (anchor to first link included) and the contact count; finally the last link's
-    centre."""
+    joint (anchor to first link included) and the number of contacts between jointed links
+    (link k is shape k; links two apart may touch); finally the last link's centre."""
     sc = scenes.hanging_chain(links)
     w = load(sc)
     half = 0.5 * scenes.LINK_PITCH
@@ -78,5 +78,7 @@ def chain_stretch(load, links=8, steps=300):
             b = pos[i + 1] + rotate(rot[i + 1], np.array([-half, 0.0, 0.0]))
             err = max(err, float(np.linalg.norm(a - b)))
         worst.append(err)
-        contacts.append(w.contact_count())
+        a, b = w.contact_shapes()
+        live = a >= 0
+        contacts.append(int(np.sum(np.abs(a[live] - b[live]) == 1)))
     return np.array(worst), np.array(contacts), w.body_arrays()[0][links].copy()

@@ -19,9 +19,6 @@ from natsel_amd import scenes
 TIME_STEP = 1.0 / 60.0
 SUB_STEPS = 4
 IMPACT = float(scenes.IMPACT_SPEED)
-CCD_GAP = ("a ball falling from 20 m or more moves farther per step than its fat AABB margin; the reference "
-           "catches it with the continuous-collision sweep (continuous.rs:308-337), which is not built, so the "
-           "fast body keeps its fat AABB and passes through the ground")
 
 
 def _run(w, steps, substeps=SUB_STEPS):
@@ -116,3 +113,57 @@ def resting_stack(load, box_count=10):
         drift = max(drift, abs(float(np.float32(pos[top, 1]) - settled)))
         peak = max(peak, float(np.linalg.norm(v[1:top + 1].astype(np.float64), axis=1).max()))
     return drift, peak
+
+
+def overshoot(load, continuous):
+    """measure_overshoot (restitution_phase_tests.rs:311-380): (first apex, energy ratio at
+    the first apex, peak energy ratio, rebound/impact speed ratio, bounce count) of an
+    elastic unit cube (density 1000, so mass 1000 kg and inertia m/6) dropped from 10 m."""
+    w = load(scenes.overshoot(continuous))
+    mass = 1000.0
+    inertia = mass / 6.0
+    g = np.array([0.0, -10.0, 0.0], np.float32)
+    h = np.float32(TIME_STEP) / np.float32(SUB_STEPS)
+
+    def energy():
+        pos, rot, v, av = w.body_arrays()
+        vel = v[1].astype(np.float64)
+        omega = av[1].astype(np.float64)
+        kinetic = 0.5 * mass * vel.dot(vel) + 0.5 * inertia * omega.dot(omega)
+        potential = -mass * g.astype(np.float64).dot(pos[1])
+        phase = 0.5 * float(h) * mass * vel.dot(g.astype(np.float64))
+        return kinetic + potential + phase
+
+    start = energy()
+    first_apex = first_ratio = peak = 0.0
+    bounces = 0
+    impact = rebound = previous = 0.0
+    step = 0
+    while step < 600 and bounces < 2:
+        before = float(w.body_arrays()[2][1, 1])
+        w.step(TIME_STEP, SUB_STEPS)
+        speed = float(w.body_arrays()[2][1, 1])
+        if rebound == 0.0 and before < 0.0 and speed > 0.0:
+            impact, rebound = -before, speed
+        if rebound == 0.0:
+            previous = speed
+            step += 1
+            continue
+        ratio = energy() / start
+        peak = max(peak, ratio)
+        if previous > 0.0 and speed <= 0.0:
+            if bounces == 0:
+                first_apex = float(np.float32(w.body_arrays()[0][1, 1]))
+                first_ratio = ratio
+            bounces += 1
+        previous = speed
+        step += 1
+    return first_apex, first_ratio, peak, (rebound / impact if impact else 0.0), bounces
+
+
+def check_overshoot(discrete, continuous):
+    """The assertions of overshoot (restitution_phase_tests.rs:384-434)."""
+    assert discrete[3] <= 1.0 + 1e-5 and continuous[3] <= 1.0 + 1e-5, (discrete, continuous)
+    assert continuous[0] <= 10.0 and continuous[2] <= 1.001, continuous
+    assert discrete[0] <= 10.0 + 0.15 and discrete[1] <= 1.02, discrete
+    assert discrete[4] >= 2 and continuous[4] >= 1, (discrete, continuous)

@@ -1,0 +1,205 @@
+"""Continuous collision on the HIP path, bit for bit with the oracle.
+
+Time-of-impact queries run through nsl_time_of_impact (one lane each) and must return the
+oracle's state, fraction, point, normal, distance and iteration counts exactly, on the
+reference's known-answer sweeps (distance_tests.rs:182-284) and on seeded sweeps of boxes,
+capsules and spheres that translate and spin. The reference's continuous-collision world
+tests (ccd_tests.rs, large_world_tests.rs:169-220, world_tuning_tests.rs:90-113,
+restitution_phase_tests.rs overshoot and drop_apexes) run on the GPU world with the oracle
+stepping beside it: the state hashes, contact shape pairs and sweep counters agree after
+every step, and the reference's own assertions hold on the GPU's values. Batched worlds
+with fast bodies match their standalone oracle hashes.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ccd_cases as cc
+import restitution_cases as rc
+from natsel_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+I4 = (0.0, 0.0, 0.0, 1.0)
+Z3 = (0.0, 0.0, 0.0)
+
+
+class Twin:
+    """The GPU world and the oracle on one scene; every step compares hashes, and the
+    contact queries compare both sides."""
+
+    def __init__(self, scene):
+        from natsel_amd.world import load_scene
+        from oracle import pyoracle
+
+        self.name = scene.name
+        self.gpu = load_scene(scene)
+        self.ref = pyoracle.load_scene(scene)
+        self.steps = 0
+
+    def step(self, dt, substeps):
+        self.gpu.step(dt, substeps)
+        self.ref.step(dt, substeps)
+        self.steps += 1
+        g, r = self.gpu.world_state_hash(), self.ref.world_state_hash()
+        assert g == r, f"{self.name}: step {self.steps} hash {g:016x} != oracle {r:016x}"
+
+    def body_arrays(self):
+        return self.gpu.body_arrays()
+
+    def world_state_hash(self):
+        return self.gpu.world_state_hash()
+
+    def contact_count(self):
+        n = self.gpu.contact_count()
+        assert n == self.ref.contact_count()
+        return n
+
+    def counters(self):
+        k, s = self.gpu.kernel_stats(), self.ref.stats()
+        return (k["fast_bodies"], k["toi_calls"], k["toi_hits"]), (s["fast_bodies"], s["toi_calls"], s["toi_hits"])
+
+
+def _box(hx, hy=None, hz=None):
+    hy = hx if hy is None else hy
+    hz = hx if hz is None else hz
+    return np.array([[-hx if i & 1 else hx, -hy if i & 2 else hy, -hz if i & 4 else hz] for i in range(8)], np.float32)
+
+
+def _quat(rng):
+    a = rng.normal(size=3)
+    a /= np.linalg.norm(a)
+    t = rng.uniform(0.0, math.pi)
+    return tuple(float(np.float32(x)) for x in (*(a * math.sin(t / 2)), math.cos(t / 2)))
+
+
+def _queries():
+    q = [
+        (_box(1.0), 0.0, (Z3, Z3, Z3, I4, I4), _box(1.0), 0.0, (Z3, (5.0, 0.0, 0.0), Z3, I4, I4), 1.0),
+        (_box(1.0), 0.0, (Z3, Z3, Z3, I4, I4), _box(1.0), 0.0, (Z3, (5.0, 0.0, 0.0), (10.0, 0.0, 0.0), I4, I4), 1.0),
+        (_box(1.0), 0.0, (Z3, Z3, Z3, I4, I4), _box(1.0), 0.0,
+         (Z3, (4.0, 0.0, 0.0), Z3, I4, (0.0, 0.0, float(np.float32(math.sin(0.3926990815))),
+                                        float(np.float32(math.cos(0.3926990815))))), 1.0),
+    ]
+    rng = np.random.default_rng(2024)
+    for k in range(300):
+        kinds = []
+        for _ in range(2):
+            kind = rng.integers(0, 3)
+            if kind == 0:
+                pts, r = _box(*rng.uniform(0.1, 1.0, 3)), 0.0
+            elif kind == 1:
+                h = rng.uniform(0.1, 0.8)
+                pts, r = np.array([[0.0, -h, 0.0], [0.0, h, 0.0]], np.float32), float(rng.uniform(0.05, 0.4))
+            else:
+                pts, r = np.zeros((1, 3), np.float32), float(rng.uniform(0.05, 0.5))
+            pts = (pts + rng.uniform(-0.3, 0.3, 3)).astype(np.float32)
+            kinds.append((pts, r))
+        lc_a = tuple(float(np.float32(x)) for x in rng.uniform(-0.2, 0.2, 3))
+        lc_b = tuple(float(np.float32(x)) for x in rng.uniform(-0.2, 0.2, 3))
+        ca = tuple(float(np.float32(x)) for x in rng.uniform(-0.5, 0.5, 3))
+        ca2 = ca if k % 2 else tuple(float(np.float32(x)) for x in rng.uniform(-0.5, 0.5, 3))
+        qa1 = I4 if k % 3 == 0 else _quat(rng)
+        qa2 = qa1 if k % 2 else _quat(rng)
+        cb2 = tuple(float(np.float32(x)) for x in rng.uniform(-1.0, 1.0, 3))
+        cb1 = tuple(float(np.float32(c + d)) for c, d in zip(cb2, rng.normal(size=3) * 6.0))
+        sa = (lc_a, ca, ca2, qa1, qa2)
+        sb = (lc_b, cb1, cb2, _quat(rng), _quat(rng))
+        q.append((kinds[0][0], kinds[0][1], sa, kinds[1][0], kinds[1][1], sb, float(rng.choice([1.0, 0.7]))))
+    return q
+
+
+def test_time_of_impact_matches_oracle(gpu_available):
+    from natsel_amd.queries import time_of_impact
+    from oracle import pyoracle
+
+    queries = _queries()
+    gpu = time_of_impact(queries)
+    states = []
+    for k, (q, g) in enumerate(zip(queries, gpu)):
+        r = pyoracle.time_of_impact(*q)
+        states.append(r["state"])
+        assert g["state"] == r["state"], (k, g, r)
+        for key in ("fraction", "distance"):
+            assert np.float32(g[key]).tobytes() == np.float32(r[key]).tobytes(), (k, key, g[key], r[key])
+        for key in ("point", "normal"):
+            assert g[key].tobytes() == r[key].tobytes(), (k, key, g[key], r[key])
+        for key in ("distance_iterations", "push_back_iterations", "root_iterations"):
+            assert g[key] == r[key], (k, key, g[key], r[key])
+    # the known answers of distance_tests.rs:182-284
+    assert gpu[0]["state"] == "hit" and (gpu[0]["distance_iterations"], gpu[0]["push_back_iterations"],
+                                         gpu[0]["root_iterations"]) == (2, 1, 2)
+    assert gpu[1]["state"] == "separated" and gpu[1]["distance_iterations"] == 1
+    assert gpu[2]["state"] == "hit" and gpu[2]["root_iterations"] == 4
+    assert states.count("hit") > 50 and states.count("separated") > 20, {s: states.count(s) for s in set(states)}
+
+
+@pytest.mark.parametrize("bullet", [False, True])
+def test_fast_box_does_not_tunnel(gpu_available, bullet):
+    made = []
+
+    def load(scene):
+        made.append(Twin(scene))
+        return made[-1]
+
+    cc.check_ccd_wall(cc.ccd_wall(load, bullet))
+    got, want = made[0].counters()
+    assert got == want and got[2] > 0, (got, want)
+
+
+def test_bullet_stops_at_every_offset(gpu_available):
+    for offset in (0.0, 1.0e9, -1.0e9, 1.0e12):
+        made = []
+
+        def load(scene):
+            made.append(Twin(scene))
+            return made[-1]
+
+        assert cc.bullet_wall(load, offset) < 5.0, offset
+        got, want = made[0].counters()
+        assert got == want and got[2] > 0, (offset, got, want)
+    assert cc.bullet_wall(Twin, 0.0, enable_continuous=False) > 5.0
+
+
+def test_drop_apexes_and_overshoot(gpu_available):
+    for h in (20.0, 40.0):
+        rc.check_apexes(rc.drop_apexes(Twin, h), h)
+    rc.check_overshoot(rc.overshoot(Twin, False), rc.overshoot(Twin, True))
+
+
+def test_fast_and_bullets_scene(gpu_available):
+    """Spheres at 300 m/s through 640 cubes between two walls, a third of them bullets
+    (parallel_solver_tests.rs:278-326): fast bodies against the static shapes, bullets also
+    against the cubes."""
+    tw = Twin(scenes.fast_and_bullets())
+    for _ in range(30):
+        tw.step(1.0 / 60.0, 4)
+        tw.contact_count()
+    a, b = tw.gpu.contact_shapes()
+    ra, rb = tw.ref.contact_shapes()
+    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+    got, want = tw.counters()
+    assert got == want and got[1] > 0 and got[2] > 0, (got, want)
+    pos = tw.body_arrays()[0]
+    assert np.all(np.abs(pos[3:, 0]) < 16.0)
+
+
+def test_batched_drops_match_standalone(gpu_available):
+    """Balls dropped from 15 m to 45 m, one world each, in one batch: every world's hash
+    equals the oracle's standalone hash after every step."""
+    from natsel_amd.batch import load_batch
+    from oracle import pyoracle
+
+    heights = [15.0, 20.0, 25.0, 30.0, 35.0, 40.0, 45.0]
+    sc = [scenes.drop_ball(h) for h in heights]
+    batch = load_batch(sc)
+    refs = [pyoracle.load_scene(s) for s in sc]
+    for step in range(240):
+        batch.step(1.0 / 60.0, 4)
+        for r in refs:
+            r.step(1.0 / 60.0, 4)
+        got = batch.state_hashes()
+        want = [r.world_state_hash() for r in refs]
+        assert list(got) == want, (step, [i for i in range(len(sc)) if got[i] != want[i]])
+    assert sum(r.stats()["toi_hits"] for r in refs) > 0

@@ -62,6 +62,12 @@ class Twin:
         assert n == self.ref.contact_count()
         return n
 
+    def contact_shapes(self):
+        a, b = self.gpu.contact_shapes()
+        ra, rb = self.ref.contact_shapes()
+        assert np.array_equal(a, ra) and np.array_equal(b, rb)
+        return a, b
+
 
 def _twin_loader(made):
     def load(scene):
