@@ -246,8 +246,8 @@ __device__ void destroy_contact(const WorldRefs& r, int32_t id) {
   c.alive = 0;
   c.color = kNullIndex;
   c.local = kNullIndex;
-  if (id == ws->next_id - 1) ws->next_id -= 1;
-  else r.free_ids[ws->n_free++] = id;
+  // id_pool.rs:36-45 shrinks the pool only for id == next_index, which no live id equals
+  r.free_ids[ws->n_free++] = id;
 }
 
 // collide_pass.rs:386-478: one contact's transition (contacts are visited in id order).

@@ -301,8 +301,9 @@ void GpuWorld::destroy_contact(int32_t id) {
   c2.color = kNullIndex;
   c2.local = kNullIndex;
   c2.set = kNullIndex;
-  if (id == contact_pool_next_ - 1) contact_pool_next_ -= 1;  // id_pool.rs:36-45 (never true: ids below next)
-  else contact_free_.push_back(id);
+  // id_pool.rs:36-45 shrinks the pool only for id == next_index, which no live id equals:
+  // every freed id goes to the free list
+  contact_free_.push_back(id);
 }
 
 // constraint_graph.rs:113-197

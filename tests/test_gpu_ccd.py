@@ -110,6 +110,12 @@ def _queries():
     return q
 
 
+def _live_pairs(w):
+    """contact id -> (shape a, shape b) of the live contacts."""
+    a, b = w.contact_shapes()
+    return {int(k): (int(a[k]), int(b[k])) for k in np.nonzero(a >= 0)[0]}
+
+
 def test_time_of_impact_matches_oracle(gpu_available):
     from natsel_amd.queries import time_of_impact
     from oracle import pyoracle
@@ -176,13 +182,12 @@ def test_fast_and_bullets_scene(gpu_available):
     for _ in range(30):
         tw.step(1.0 / 60.0, 4)
         tw.contact_count()
-    a, b = tw.gpu.contact_shapes()
-    ra, rb = tw.ref.contact_shapes()
-    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+    assert _live_pairs(tw.gpu) == _live_pairs(tw.ref)
     got, want = tw.counters()
     assert got == want and got[1] > 0 and got[2] > 0, (got, want)
     pos = tw.body_arrays()[0]
-    assert np.all(np.abs(pos[3:, 0]) < 16.0)
+    behind = (np.abs(pos[3:, 0]) > 16.0) & (np.abs(pos[3:, 2]) < 20.0) & (pos[3:, 1] < 8.0)
+    assert not behind.any(), pos[3:][behind]  # none passed through a wall (some go round its ends)
 
 
 def test_batched_drops_match_standalone(gpu_available):

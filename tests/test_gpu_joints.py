@@ -65,7 +65,8 @@ class Twin:
     def contact_shapes(self):
         a, b = self.gpu.contact_shapes()
         ra, rb = self.ref.contact_shapes()
-        assert np.array_equal(a, ra) and np.array_equal(b, rb)
+        live, rlive = np.nonzero(a >= 0)[0], np.nonzero(ra >= 0)[0]
+        assert np.array_equal(live, rlive) and np.array_equal(a[live], ra[live]) and np.array_equal(b[live], rb[live])
         return a, b
 
 

@@ -237,9 +237,10 @@ def test_fast_and_bullets_scene_runs_the_sweep():
     from natsel_amd import scenes
 
     w = _load(scenes.fast_and_bullets())
-    for _ in range(20):
+    for _ in range(30):
         w.step(1.0 / 60.0, 4)
     s = w.stats()
     assert s["fast_bodies"] > 0 and s["toi_calls"] > 0 and s["toi_hits"] > 0, s
     pos = w.body_arrays()[0]
-    assert np.all(np.abs(pos[3:, 0]) < 16.0), pos[np.abs(pos[:, 0]) >= 16.0]
+    behind = (np.abs(pos[3:, 0]) > 16.0) & (np.abs(pos[3:, 2]) < 20.0) & (pos[3:, 1] < 8.0)
+    assert not behind.any(), pos[3:][behind]  # none passed through a wall (some go round its ends)
