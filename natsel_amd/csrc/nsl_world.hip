@@ -34,6 +34,8 @@ GpuWorld::GpuWorld(int device) : device_(device) {
   NSL_CHECK(hipMalloc(&d_barrier_, sizeof(GridBarrier)));
   NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
   NSL_CHECK(hipMalloc(&d_phase_log_, 2 * kMaxPhaseLog * sizeof(uint64_t)));
+  // zeroed: the host reads marks up to the first time stamp that goes backwards
+  NSL_CHECK(hipMemset(d_phase_log_, 0, 2 * kMaxPhaseLog * sizeof(uint64_t)));
   phase_log_host_.resize(2 * kMaxPhaseLog);
   max_blocks_ = solve_step_max_blocks(device_);
   if (max_blocks_ <= 0) throw std::runtime_error("step kernel does not fit on the device");
@@ -888,6 +890,10 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
 void GpuWorld::set_kernel_timing(bool on) {
   timing_ = on;
   timer_.used = 0;
+  if (on) {  // no marks of an earlier timed step may be read as this step's
+    NSL_CHECK(hipStreamSynchronize(stream_));
+    NSL_CHECK(hipMemset(d_phase_log_, 0, 2 * kMaxPhaseLog * sizeof(uint64_t)));
+  }
 }
 
 void GpuWorld::reset_kernel_stats() {
