@@ -298,6 +298,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups of the step kernel (0: sized from the largest phase)")
+    ap.add_argument("--small-colours", type=int, default=128,
+                    help="colours of at most this many items run on one workgroup (0: a grid barrier after every colour)")
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched measurement (0: skip it)")
     ap.add_argument("--batched-steps", type=int, default=240)
     ap.add_argument("--large-world-steps", type=int, default=500, help="steps of the large-world measurement (0: skip it)")
@@ -326,7 +328,7 @@ def main() -> int:
     from natsel_amd.world import load_scene
 
     scene = scenes.pile(args.bodies)
-    world = load_scene(scene, device=0, solver_blocks=args.blocks)
+    world = load_scene(scene, device=0, solver_blocks=args.blocks, small_colour_items=args.small_colours)
     for _ in range(args.warmup):
         world.step(scene.dt, scene.substeps)
     world.reset_kernel_stats()
@@ -364,6 +366,7 @@ def main() -> int:
         "avg_launch_us": 1e3 * ms["solve"] / launches,
         "algorithmic_bytes_per_launch": dom_bytes / launches,
         "grid_barriers_per_launch": ks["barriers"] / launches,
+        "small_colour_items": args.small_colours,
         "workgroups": ks["solver_blocks"],
     }
     out = {

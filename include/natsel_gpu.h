@@ -59,11 +59,14 @@ typedef struct nsl_options {
   float restitution_threshold;       /* 1 m/s */
   int32_t restitution_iterations;    /* 2 */
   int32_t enable_sleep;              /* 1 */
-  int32_t enable_continuous;         /* 1: fast bodies are counted (continuous collision is not run) */
+  int32_t enable_continuous;         /* 1: fast bodies are swept (continuous.rs) */
   int32_t enable_warm_starting;      /* 1 */
   int32_t enable_speculative;        /* 1 */
   int32_t solver_blocks;             /* workgroups of the step kernel; 0 = sized from the largest phase */
   int32_t enable_restitution_propagation; /* 0 (world_runtime.rs:1055) */
+  int32_t small_colour_items;        /* 128: consecutive colours of at most this many items run on one
+                                        workgroup with workgroup barriers, one grid barrier after the
+                                        run (0: every colour behind a grid barrier); results do not change */
 } nsl_options;
 
 typedef struct nsl_body_def {

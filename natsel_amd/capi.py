@@ -27,6 +27,7 @@ class Options(ctypes.Structure):
         ("enable_speculative", ctypes.c_int32),
         ("solver_blocks", ctypes.c_int32),
         ("enable_restitution_propagation", ctypes.c_int32),
+        ("small_colour_items", ctypes.c_int32),
     ]
 
 

@@ -57,6 +57,7 @@ void apply(nsl::WorldOptions& o, const nsl_options& in) {
   o.enable_speculative = in.enable_speculative != 0;
   o.solver_blocks = in.solver_blocks;
   o.enable_restitution_propagation = in.enable_restitution_propagation != 0;
+  o.small_colour_items = in.small_colour_items;
 }
 
 nsl::Material material(const nsl_material* m) {
@@ -119,6 +120,7 @@ void nsl_default_options(nsl_options* out) {
   out->enable_speculative = o.enable_speculative;
   out->solver_blocks = o.solver_blocks;
   out->enable_restitution_propagation = o.enable_restitution_propagation;
+  out->small_colour_items = o.small_colour_items;
 }
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out) {

@@ -109,6 +109,7 @@ struct SolvePlan {
   int32_t n_overflow;             // overflow constraints, solved in list order by one lane
   int32_t any_restitution;
   int32_t restitution_iterations;
+  int32_t small_colour_items;     // runs of colours this small go to workgroup 0 (0: off)
   GridBarrier* barrier;
   uint64_t* phase_log;            // null: no phase timing; else [2 * kMaxPhaseLog]: (class, wall clock)
 };
@@ -126,6 +127,6 @@ void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepPa
                        KernelTimer* timer);
 // Workgroups of the step kernel that fit on the device at once.
 int solve_step_max_blocks(int device);
-constexpr int kSolveBlock = 256;
+constexpr int kSolveBlock = 128;  // measured: 128 lanes beat 64, 256 and 512 on the 10k pile
 
 }  // namespace nsl

@@ -125,7 +125,25 @@ struct ColourTable {
   int end[kOverflowColor];
   int jbegin[kOverflowColor];
   int jend[kOverflowColor];
+  // group_end[a] > a + 1: colours a..group_end[a]-1 each have at most the plan's
+  // small_colour_items items and run on workgroup 0, a workgroup barrier between them
+  int group_end[kOverflowColor];
 };
+
+// Groups runs of two or more consecutive small colours (items counted with joints).
+__device__ void group_small_colours(ColourTable& t, int small_items) {
+  for (int a = 0; a < t.n;) {
+    int b = a;
+    while (b < t.n && (t.end[b] - t.begin[b]) + (t.jend[b] - t.jbegin[b]) <= small_items) ++b;
+    if (b - a >= 2) {
+      for (int c = a; c < b; ++c) t.group_end[c] = b;
+      a = b;
+    } else {
+      t.group_end[a] = a + 1;
+      a += 1;
+    }
+  }
+}
 
 // A colour's items are its joints (in joint_ids order) and then its contact slots: the
 // reference's unit order within a colour (solver_stages.rs:303-345). The two touch
@@ -143,32 +161,57 @@ __device__ __forceinline__ int2 first_indices(const DeviceWorld& d, const Colour
   return make_int2(d.constraints[slot].index_a, d.constraints[slot].index_b);
 }
 
-// One stage over the active colours in order, a grid barrier after each. A lane's body
-// indices for its first contact slot in the next colour are read before the barrier, so
-// each phase opens with its body-state loads. `joint` runs the colour's joints when
-// `with_joints` (warm start, push and relax; restitution has no joint rows).
+// One stage over the active colours in order, a grid barrier after each colour or group
+// of small colours. A lane's body indices for its first contact slot in the next colour are
+// read before the barrier, so each phase opens with its body-state loads. `joint` runs the
+// colour's joints when `with_joints` (warm start, push and relax; restitution has no joint
+// rows). A group of small colours runs on workgroup 0 alone, in colour order with a
+// workgroup barrier between colours (the workgroup shares one L1, so its stores are visible
+// to its loads after the barrier): the same items in the same order, fewer grid barriers.
 template <class JointItem, class Item>
 __device__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid, int nthreads, GridBarrier* bar, PhaseClock& clock,
                             int cls, bool with_joints, JointItem joint, Item item) {
   const int n_active = t.n;
   if (n_active == 0) return true;
   int2 pf = first_indices(d, t, 0, tid, with_joints);
-  for (int a = 0; a < n_active; ++a) {
-    const int nj = joints_in(t, a, with_joints);
-    const int jb = t.jbegin[a], cb = t.begin[a] - nj;
-    const int n_items = nj + (t.end[a] - t.begin[a]);
-    int k = tid;
-    if (k < nj) joint(d.joint_ids[jb + k]);
-    else if (k < n_items) item(cb + k, pf.x, pf.y);
-    for (k += nthreads; k < n_items; k += nthreads) {
-      if (k < nj) {
-        joint(d.joint_ids[jb + k]);
-      } else {
-        const int slot = cb + k;
-        item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
+  for (int a = 0; a < n_active;) {
+    const int g = t.group_end[a];
+    if (g > a + 1) {
+      if (blockIdx.x == 0) {
+        for (int c = a; c < g; ++c) {
+          const int nj = joints_in(t, c, with_joints);
+          const int jb = t.jbegin[c], cb = t.begin[c] - nj;
+          const int n_items = nj + (t.end[c] - t.begin[c]);
+          for (int k = (int)threadIdx.x; k < n_items; k += (int)blockDim.x) {
+            if (k < nj) {
+              joint(d.joint_ids[jb + k]);
+            } else {
+              const int slot = cb + k;
+              item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
+            }
+          }
+          if (c + 1 < g) __syncthreads();
+        }
       }
+      a = g;
+    } else {
+      const int nj = joints_in(t, a, with_joints);
+      const int jb = t.jbegin[a], cb = t.begin[a] - nj;
+      const int n_items = nj + (t.end[a] - t.begin[a]);
+      int k = tid;
+      if (k < nj) joint(d.joint_ids[jb + k]);
+      else if (k < n_items) item(cb + k, pf.x, pf.y);
+      for (k += nthreads; k < n_items; k += nthreads) {
+        if (k < nj) {
+          joint(d.joint_ids[jb + k]);
+        } else {
+          const int slot = cb + k;
+          item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
+        }
+      }
+      a += 1;
     }
-    if (a + 1 < n_active) pf = first_indices(d, t, a + 1, tid, with_joints);
+    if (a < n_active) pf = first_indices(d, t, a, tid, with_joints);
     if (!grid_sync(bar)) return false;
     clock.mark(cls);
   }
@@ -216,6 +259,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         ++n;
       }
     colours.n = n;
+    group_small_colours(colours, plan.small_colour_items);
   }
   __syncthreads();
   const bool any_ovf = plan.n_ovf_levels > 0;

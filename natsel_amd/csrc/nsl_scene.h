@@ -54,6 +54,9 @@ struct WorldOptions {
   float isotropic_inertia_tolerance = 0.2f;
   bool enable_sleep = true, enable_warm_starting = true, enable_continuous = true, enable_speculative = true;
   int32_t solver_blocks = 0;  // workgroups of the step kernel; 0 = sized from the largest phase
+  // consecutive colours of at most this many items (joints and contacts) run on one
+  // workgroup with workgroup barriers between them; 0 = every colour behind a grid barrier
+  int32_t small_colour_items = 128;
 };
 
 

@@ -801,13 +801,16 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   plan.n_overflow = (int32_t)ovf.size();
   plan.any_restitution = any_restitution ? 1 : 0;
   plan.restitution_iterations = options.restitution_iterations;
+  plan.small_colour_items = options.small_colour_items;
   plan.barrier = d_barrier_;
   plan.phase_log = timing_ ? d_phase_log_ : nullptr;
   int widest = (d_.n_awake + 3) / 4;
   for (int ci = 0; ci < kOverflowColor; ++ci)
     widest = std::max(widest, (cr.end[ci] - cr.begin[ci]) + (joint_ranges_.end[ci] - joint_ranges_.begin[ci]));
   int blocks = options.solver_blocks > 0 ? options.solver_blocks : (widest + kSolveBlock - 1) / kSolveBlock;
-  blocks = std::max(1, std::min(blocks, std::min(max_blocks_, 128)));
+  // more workgroups add grid-barrier arrivals faster than they add useful lanes: on the
+  // settled 132k-body world 96 workgroups of 128 lanes beat 48..80 and 112..256
+  blocks = std::max(1, std::min(blocks, std::min(max_blocks_, 96)));
   launch_solve_step(d_, plan, p, blocks, stream_, tm);
   // continuous collision of the bodies finalize flagged fast, then of the bullets
   if (options.enable_continuous) {
@@ -867,9 +870,23 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   kstats_.host_ms[3] += t4 - t3;
   kstats_.solver_blocks = blocks;
   {
-    const int n_active = colors_used, ovf_phase = ovf.empty() ? 0 : 1;
-    int64_t barriers = 2 + (int64_t)p.sub_steps * (2 + 3 * (n_active + ovf_phase));
-    if (any_restitution) barriers += (int64_t)options.restitution_iterations * (n_active + ovf_phase);
+    // grid barriers of one stage: one per colour, or per run of two or more small colours
+    // (run_colours in nsl_kernels.hip)
+    int phases = 0, run = 0;
+    for (int ci = 0; ci < kOverflowColor; ++ci) {
+      const int items = (cr.end[ci] - cr.begin[ci]) + (joint_ranges_.end[ci] - joint_ranges_.begin[ci]);
+      if (items == 0) continue;
+      if (items <= options.small_colour_items) {
+        run += 1;
+      } else {
+        phases += (run >= 2 ? 1 : run) + 1;
+        run = 0;
+      }
+    }
+    phases += run >= 2 ? 1 : run;
+    const int ovf_phase = plan.n_ovf_levels > 0 ? 1 : 0;
+    int64_t barriers = 2 + (int64_t)p.sub_steps * (2 + 3 * (phases + ovf_phase));
+    if (any_restitution) barriers += (int64_t)options.restitution_iterations * (phases + (ovf.empty() ? 0 : 1));
     kstats_.barriers += barriers;
   }
   profile_.pairs = (float)(t1 - t0);

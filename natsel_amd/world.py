@@ -82,6 +82,7 @@ class WorldOptions:  # world_runtime.rs:1243-1304
     enable_speculative: bool = True
     solver_blocks: int = 0  # workgroups of the step kernel; 0 = sized from the largest phase
     enable_restitution_propagation: bool = False
+    small_colour_items: int = 128  # colours this small run on one workgroup (0: off); results do not change
 
 
 def _material(d: ShapeDef) -> capi.Material:
@@ -99,6 +100,7 @@ def c_options(o: WorldOptions) -> capi.Options:
     c.enable_warm_starting, c.enable_speculative = int(o.enable_warm_starting), int(o.enable_speculative)
     c.solver_blocks = int(o.solver_blocks)
     c.enable_restitution_propagation = int(o.enable_restitution_propagation)
+    c.small_colour_items = int(o.small_colour_items)
     return c
 
 
@@ -291,9 +293,10 @@ class World:
         return {name: getattr(p, name) for name, _ in capi.Profile._fields_}
 
 
-def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0) -> World:
+def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0, small_colour_items: int = 128) -> World:
     """Build a World from a seeded scene description (natsel_amd.scenes)."""
     opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous, solver_blocks=solver_blocks,
+                        small_colour_items=small_colour_items,
                         gravity=tuple(scene.gravity), restitution_iterations=scene.restitution_iterations,
                         restitution_threshold=scene.restitution_threshold,
                         enable_restitution_propagation=scene.restitution_propagation)

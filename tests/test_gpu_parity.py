@@ -179,3 +179,25 @@ def test_step_kernel_workgroup_count_does_not_change_results(gpu_available):
         assert hashes == {o.world_state_hash()}, f"step {k}: {sorted(hashes)}"
     for b, w in zip(counts, worlds):
         assert w.kernel_stats()["solver_blocks"] == b
+
+
+def test_small_colour_groups_do_not_change_results(gpu_available):
+    """Runs of small colours on one workgroup (workgroup barriers between them) give the
+    same bits as a grid barrier after every colour, for any size threshold, with joints in
+    the colours; fewer grid barriers as the threshold grows."""
+    from natsel_amd.world import load_scene
+    from oracle import pyoracle
+
+    sc = scenes.chain_pile(120, chains=8, links=8, size_x=6, size_y=6, size_z=6)
+    limits = (0, 16, 256, 100000)
+    worlds = [load_scene(sc, small_colour_items=s) for s in limits]
+    o = pyoracle.load_scene(sc)
+    for k in range(120):
+        o.step(1.0 / 60.0, 4)
+        hashes = set()
+        for w in worlds:
+            w.step(1.0 / 60.0, 4)
+            hashes.add(w.world_state_hash())
+        assert hashes == {o.world_state_hash()}, f"step {k}: {sorted(hashes)}"
+    barriers = [w.kernel_stats()["barriers"] for w in worlds]
+    assert barriers[0] > barriers[2] >= barriers[3], barriers
