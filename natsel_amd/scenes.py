@@ -735,3 +735,56 @@ def fast_and_bullets(cubes: int = 640, shooters: int = 24) -> Scene:
                                  linear_velocity=v, is_bullet=i % 3 == 0))
         sc.add_shape(ShapeSpec(b, SPHERE, (0.0, 0.0, 0.0, 0.1)))
     return sc
+
+
+# ---------------------------------------------------------------------------------
+# Islands and sleep
+
+
+def box_sleeps() -> Scene:
+    """A unit box resting on a static slab, sleep on (sleep_tests.rs:33-99). The box is the
+    last body."""
+    sc = Scene(name="box_sleeps", enable_sleep=True)
+    g = sc.add_body(BodySpec(STATIC, (0.0, 0.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (10.0, 0.5, 10.0)))
+    b = sc.add_body(BodySpec(DYNAMIC, (0.0, 0.6, 0.0)))
+    sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), density=1000.0))
+    return sc
+
+
+def island_split() -> Scene:
+    """Two unit boxes stacked on a static slab, sleep off (island_split_tests.rs:46-76)."""
+    sc = Scene(name="island_split", enable_sleep=False)
+    g = sc.add_body(BodySpec(STATIC, (0.0, 0.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (10.0, 0.5, 10.0)))
+    for i in range(2):
+        b = sc.add_body(BodySpec(DYNAMIC, (0.0, 0.55 + i * 1.01, 0.0)))
+        sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), density=1000.0))
+    return sc
+
+
+def sleep_and_wake(drop_height: float = 8.0) -> Scene:
+    """A box settles and falls asleep on the ground; a ball dropped from `drop_height` lands
+    on it later and wakes it; both settle and sleep again. Bodies: ground, box, ball."""
+    sc = Scene(name="sleep_and_wake", enable_sleep=True)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -0.5, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (10.0, 0.5, 10.0)))
+    b = sc.add_body(BodySpec(DYNAMIC, (0.0, 0.5, 0.0)))
+    sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5)))
+    s = sc.add_body(BodySpec(DYNAMIC, (0.2, drop_height, 0.1)))
+    sc.add_shape(ShapeSpec(s, SPHERE, (0.0, 0.0, 0.0, 0.25)))
+    return sc
+
+
+def roll_off() -> Scene:
+    """A ball rolls off a resting box and away across the ground: the box-ball contact ends,
+    the island holding both is split once the box is sleepy, and the box falls asleep while
+    the ball still rolls. Bodies: ground, box, ball."""
+    sc = Scene(name="roll_off", enable_sleep=True)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -0.5, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (20.0, 0.5, 20.0)))
+    b = sc.add_body(BodySpec(DYNAMIC, (0.0, 0.5, 0.0)))
+    sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5)))
+    s = sc.add_body(BodySpec(DYNAMIC, (0.0, 1.25, 0.0), linear_velocity=(1.5, 0.0, 0.0)))
+    sc.add_shape(ShapeSpec(s, SPHERE, (0.0, 0.0, 0.0, 0.25), friction=0.1))
+    return sc

@@ -64,6 +64,13 @@ def lib():
         L.nslo_save_state.restype = P
         L.nslo_restore_state.argtypes = [P, P]
         L.nslo_time_of_impact.argtypes = [fp, I, F, fp, fp, I, F, fp, F, fp, ctypes.POINTER(ctypes.c_int32)]
+        L.nslo_body_location.argtypes = [P, I, ctypes.POINTER(ctypes.c_int32)]
+        L.nslo_island_summary.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
+        L.nslo_island_hash.argtypes = [P, I]
+        L.nslo_island_hash.restype = ctypes.c_uint64
+        L.nslo_first_island_contact.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
+        L.nslo_unlink_and_split.argtypes = [P, I, I]
+        L.nslo_check_islands.argtypes = [P]
         _LIB = L
     return _LIB
 
@@ -191,6 +198,32 @@ class OracleWorld:
         fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
         self.L.nslo_get_bodies(self.h, pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), fp(rot), fp(v), fp(w))
         return pos, rot, v, w
+
+    def body_location(self, body: int) -> tuple:
+        """(solver set, index in it, island id) of a body: set 0 static, 2 awake, 3+ sleeping."""
+        out = (ctypes.c_int32 * 3)()
+        self.L.nslo_body_location(self.h, body, out)
+        return tuple(out)
+
+    def island_summary(self) -> dict:
+        out = (ctypes.c_int64 * 5)()
+        self.L.nslo_island_summary(self.h, out)
+        return dict(zip(["island_slots", "islands", "awake_islands", "sleeping_sets", "split_island"], list(out)))
+
+    def island_hash(self, with_bodies: bool = False) -> int:
+        return int(self.L.nslo_island_hash(self.h, int(with_bodies)))
+
+    def first_island_contact(self) -> tuple:
+        out = (ctypes.c_int32 * 2)()
+        self.L.nslo_first_island_contact(self.h, out)
+        return tuple(out)
+
+    def unlink_and_split(self, contact: int, island: int) -> None:
+        self.L.nslo_unlink_and_split(self.h, contact, island)
+
+    def check_islands(self) -> int:
+        """0 when every island, solver-set and encoding record agrees; else the failed check."""
+        return int(self.L.nslo_check_islands(self.h))
 
     def stats(self) -> dict:
         out = (ctypes.c_int64 * 11)()

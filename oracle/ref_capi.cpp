@@ -146,6 +146,120 @@ void nslo_get_stats(void* wp, int64_t* out) {
   out[9] = s.toi_calls; out[10] = s.toi_hits;
 }
 
+// Where a body lives: solver set (0 static, 2 awake, 3+ sleeping), index in it, island id.
+void nslo_body_location(void* wp, int body, int32_t* out) {
+  const World* w = static_cast<World*>(wp);
+  const Body& b = w->bodies[(size_t)body];
+  out[0] = b.set_index;
+  out[1] = b.local_index;
+  out[2] = b.island_id;
+}
+
+// [island slots, live islands, awake islands, live sleeping sets, pending split island]
+void nslo_island_summary(void* wp, int64_t* out) {
+  const World* w = static_cast<World*>(wp);
+  int64_t live = 0, sets = 0;
+  for (const Island& is : w->islands) live += is.island_id != kNullIndex;
+  for (const SleepSet& s : w->sleep_sets) sets += s.set_index != kNullIndex;
+  out[0] = (int64_t)w->islands.size();
+  out[1] = live;
+  out[2] = (int64_t)w->awake_islands.size();
+  out[3] = sets;
+  out[4] = w->split_island_id;
+}
+
+// island_split_tests.rs:27-44: FNV-1a over every live island's id, set, body ids and
+// contact ids, then every live body's island id and index.
+uint64_t nslo_island_hash(void* wp, int with_bodies) {
+  const World* w = static_cast<World*>(wp);
+  uint64_t h = 14695981039346656037ull;
+  auto mix = [&](int32_t v) {
+    for (int k = 0; k < 4; ++k) {
+      h ^= (uint64_t)((uint32_t)v >> (8 * k) & 0xFF);
+      h *= 1099511628211ull;
+    }
+  };
+  for (const Island& is : w->islands) {
+    if (is.island_id == kNullIndex) continue;
+    mix(is.island_id);
+    mix(is.set_index);
+    for (int32_t b : is.bodies) mix(b);
+    for (const ContactLink& l : is.contacts) mix(l.contact_id);
+  }
+  if (with_bodies)
+    for (const Body& b : w->bodies) {
+      if (b.id == kNullIndex) continue;
+      mix(b.island_id);
+      mix(b.island_index);
+    }
+  return h;
+}
+
+// Cross-checks of the island and solver-set records; 0 when consistent, else the number of
+// the first failed check. Also checks that every awake-set contact's body encodings equal
+// its bodies' current encodings.
+int nslo_check_islands(void* wp) {
+  World* w = static_cast<World*>(wp);
+  for (size_t id = 0; id < w->islands.size(); ++id) {
+    const Island& is = w->islands[id];
+    if (is.island_id == kNullIndex) continue;
+    if (is.island_id != (int32_t)id) return 1;
+    const std::vector<int32_t>& list = w->island_list(is.set_index);
+    if (is.local_index < 0 || (size_t)is.local_index >= list.size() || list[(size_t)is.local_index] != (int32_t)id) return 2;
+    for (size_t k = 0; k < is.bodies.size(); ++k) {
+      const Body& b = w->bodies[(size_t)is.bodies[k]];
+      if (b.island_id != (int32_t)id || b.island_index != (int32_t)k) return 3;
+      if (b.set_index != is.set_index) return 4;
+    }
+    for (size_t k = 0; k < is.contacts.size(); ++k) {
+      const Contact& c = w->contacts[(size_t)is.contacts[k].contact_id];
+      if (c.island_id != (int32_t)id || c.island_index != (int32_t)k) return 5;
+    }
+    for (size_t k = 0; k < is.joints.size(); ++k) {
+      const Joint& j = w->joints[(size_t)is.joints[k].joint_id];
+      if (j.island_id != (int32_t)id || j.island_index != (int32_t)k) return 6;
+    }
+  }
+  for (const Body& b : w->bodies) {
+    if (b.id == kNullIndex) continue;
+    if (b.set_index == 2 && w->awake_sims[(size_t)b.local_index].body_id != b.id) return 7;
+    if (b.set_index >= 3 && w->sleep_sets[(size_t)b.set_index].body_sims[(size_t)b.local_index].body_id != b.id) return 8;
+    if (b.type != kStatic && b.island_id == kNullIndex) return 9;
+  }
+  for (const Contact& c : w->contacts) {
+    if (c.contact_id == kNullIndex) continue;
+    if (c.color_index == kNullIndex) {
+      const std::vector<int32_t>& list = w->contact_list(c.set_index);
+      if (list[(size_t)c.local_index] != c.contact_id) return 10;
+    }
+    if (c.set_index == 2) {
+      const int32_t ea = w->encode_body_sim_index(w->bodies[(size_t)c.edges[0].body_id]);
+      const int32_t eb = w->encode_body_sim_index(w->bodies[(size_t)c.edges[1].body_id]);
+      if (c.encoded_a != ea || c.encoded_b != eb) return 11;
+    }
+  }
+  return 0;
+}
+
+// The first touching contact in id order and its island (island_split_tests.rs:84-97).
+void nslo_first_island_contact(void* wp, int32_t* out) {
+  const World* w = static_cast<World*>(wp);
+  out[0] = out[1] = kNullIndex;
+  for (const Contact& c : w->contacts)
+    if (c.contact_id != kNullIndex && c.island_id != kNullIndex) {
+      out[0] = c.contact_id;
+      out[1] = c.island_id;
+      return;
+    }
+}
+
+// island_split_tests.rs:99-100: unlink a contact from its island, then split the island.
+void nslo_unlink_and_split(void* wp, int contact, int island) {
+  World* w = static_cast<World*>(wp);
+  w->unlink_contact(contact);
+  w->split_island(island);
+}
+
 // One time-of-impact query (time_of_impact.rs:406-598). Sweeps are 17 floats: local
 // centre, c1, c2, q1 (x, y, z, s), q2. fout: point xyz, normal xyz, fraction, distance;
 // iout: state, distance, push-back and root iterations.

@@ -174,6 +174,10 @@ int32_t World::assign_joint_color(int32_t a, int32_t b, BodyType ta, BodyType tb
 int32_t World::create_revolute_joint(const RevoluteJointDef& def) {
   const int32_t a = def.body_a, b = def.body_b;
   if (a == b || a < 0 || b < 0 || a >= (int32_t)bodies.size() || b >= (int32_t)bodies.size()) return kNullIndex;
+  // a joint with no awake body but a sleeping one stays asleep in the reference (joint.rs:294-
+  // 340, merging sleeping sets); not restated: refused
+  if (bodies[a].set_index != 2 && bodies[b].set_index != 2 && (bodies[a].set_index >= 3 || bodies[b].set_index >= 3))
+    return kNullIndex;
   if (!def.collide_connected) destroy_contacts_between_bodies(a, b);
 
   int32_t jid = joint_pool.alloc();
@@ -219,13 +223,18 @@ int32_t World::create_revolute_joint(const RevoluteJointDef& def) {
     j.local_index = (int32_t)static_joint_sims.size();
     static_joint_sims.push_back(sim);
   } else {
-    j.set_index = 2;  // awake set: into the graph
+    // joint.rs:272-293: wake a sleeping partner, then into the graph
+    const int32_t max_set = std::max(bodies[a].set_index, bodies[b].set_index);
+    joints[jid] = j;
+    if (max_set >= 3) wake_solver_set(max_set);
+    j.set_index = 2;
     int32_t color = assign_joint_color(a, b, bodies[a].type, bodies[b].type);
     j.color_index = color;
     j.local_index = (int32_t)colors[color].joint_sims.size();
     colors[color].joint_sims.push_back(sim);
   }
   joints[jid] = j;
+  if (j.set_index > 1) link_joint(jid);  // joint.rs:366-370
   return jid;
 }
 

@@ -22,11 +22,17 @@ Softness make_soft(float hertz, float zeta, float h) {
   return Softness{omega / a1, a2 * a3, a3};
 }
 
-World::World() {}
+World::World() { set_pool.next_index = 3; }  // 0 static, 1 disabled, 2 awake
 
-BodySim& World::sim_of(const Body& b) { return b.set_index == 0 ? static_sims[b.local_index] : awake_sims[b.local_index]; }
+BodySim& World::sim_of(const Body& b) {
+  if (b.set_index == 0) return static_sims[b.local_index];
+  if (b.set_index == 2) return awake_sims[b.local_index];
+  return sleep_sets[(size_t)b.set_index].body_sims[b.local_index];
+}
 const BodySim& World::sim_of(const Body& b) const {
-  return b.set_index == 0 ? static_sims[b.local_index] : awake_sims[b.local_index];
+  if (b.set_index == 0) return static_sims[b.local_index];
+  if (b.set_index == 2) return awake_sims[b.local_index];
+  return sleep_sets[(size_t)b.set_index].body_sims[b.local_index];
 }
 const BodySim& World::body_sim(int32_t id) const { return sim_of(bodies[id]); }
 BodyState World::body_state(int32_t id) const {
@@ -89,6 +95,8 @@ int32_t World::create_body(const BodyDef& def) {
   b.safety_factor = def.safety_factor;
   b.type = def.type;
   b.flags = sim.flags;
+  // world_runtime.rs:1626-1629: kinematic and dynamic bodies get an island
+  if (set_id >= 2) create_island_for_body(set_id, id);
   return id;
 }
 
@@ -505,8 +513,16 @@ void World::create_contact(int32_t ia, int32_t ib, int32_t child) {
   if ((ba.flags & BF::ENABLE_CONTACT_RECYCLING) && (bb.flags & BF::ENABLE_CONTACT_RECYCLING) &&
       (c.flags & CF::ENABLE_SPECULATIVE_POINTS))
     c.flags |= CF::RECYCLE;
-  c.edges[0] = ContactEdge{sa.body_id, kNullIndex, kNullIndex};
-  c.edges[1] = ContactEdge{sb.body_id, kNullIndex, kNullIndex};
+  // contact_creation.rs:357-384: head of each body's contact list
+  for (int e = 0; e < 2; ++e) {
+    Body& body = bodies[e == 0 ? sa.body_id : sb.body_id];
+    const int32_t key = (id << 1) | e;
+    c.edges[e] = ContactEdge{e == 0 ? sa.body_id : sb.body_id, kNullIndex, body.head_contact_key};
+    if (body.head_contact_key != kNullIndex)
+      contacts[(size_t)(body.head_contact_key >> 1)].edges[body.head_contact_key & 1].prev_key = key;
+    body.head_contact_key = key;
+    body.contact_count += 1;
+  }
   pair_set.insert(shape_pair_key(sa.id, sb.id, child));
   if (set_index == 2) awake_contact_indices.push_back(id);
   else disabled_contact_indices.push_back(id);
@@ -534,10 +550,20 @@ void World::destroy_contact(int32_t id) {
   Contact& c = contacts[id];
   if (c.contact_id == kNullIndex) return;
   pair_set.erase(shape_pair_key(c.shape_id_a, c.shape_id_b, c.child_index));
+  // contact_creation.rs:456-494: out of both bodies' contact lists
+  for (int e = 0; e < 2; ++e) {
+    const ContactEdge edge = contacts[(size_t)id].edges[e];
+    if (edge.prev_key != kNullIndex) contacts[(size_t)(edge.prev_key >> 1)].edges[edge.prev_key & 1].next_key = edge.next_key;
+    if (edge.next_key != kNullIndex) contacts[(size_t)(edge.next_key >> 1)].edges[edge.next_key & 1].prev_key = edge.prev_key;
+    Body& body = bodies[(size_t)edge.body_id];
+    if (body.head_contact_key == ((id << 1) | e)) body.head_contact_key = edge.next_key;
+    body.contact_count -= 1;
+  }
   c.manifold_count = 0;
   c.manifold.point_count = 0;
+  if (c.island_id != kNullIndex) unlink_contact(id);  // contact_creation.rs:501-508
   if (c.color_index == kNullIndex) {
-    std::vector<int32_t>& list = c.set_index == 2 ? awake_contact_indices : disabled_contact_indices;
+    std::vector<int32_t>& list = contact_list(c.set_index);
     int32_t local = c.local_index;
     int32_t moved = remove_swap(list, (size_t)local);
     if (moved != kNullIndex) contacts[list[local]].local_index = local;
@@ -725,14 +751,16 @@ void World::collide_one_contact(int32_t id, std::vector<uint8_t>& state_bits) {
     state_bits[id] = 1;
     return;
   }
-  auto resolve = [&](int32_t enc, bool* is_static) -> const BodySim& {
-    if (enc >= 0) { *is_static = false; return awake_sims[enc]; }
-    *is_static = true;
-    return static_sims[-enc - 2];
+  // collide_pass.rs:105-154: an index of kNullIndex (a sleeping body) is found through the body
+  auto resolve = [&](int32_t enc, int32_t body_id, bool* is_static) -> const BodySim& {
+    *is_static = enc < kNullIndex;
+    if (enc >= 0) return awake_sims[enc];
+    if (enc < kNullIndex) return static_sims[-enc - 2];
+    return sim_of(bodies[body_id]);
   };
   bool st_a, st_b;
-  const BodySim& sim_a = resolve(c.encoded_a, &st_a);
-  const BodySim& sim_b = resolve(c.encoded_b, &st_b);
+  const BodySim& sim_a = resolve(c.encoded_a, c.edges[0].body_id, &st_a);
+  const BodySim& sim_b = resolve(c.encoded_b, c.edges[1].body_id, &st_b);
   bool was_touching = (c.flags & CF::SIM_TOUCHING) != 0;
   WXf xf_a = sim_a.transform, xf_b = sim_b.transform;
   bool is_fast = ((sim_a.flags | sim_b.flags) & BF::IS_FAST) != 0;
@@ -819,7 +847,9 @@ void World::collide() {
       Contact& c = contacts[id];
       c.flags &= ~CF::SIM_STARTED_TOUCHING;
       c.flags |= CF::TOUCHING;
-      int32_t old_local = c.local_index;
+      // link first: this wakes a sleeping partner so the body sims are in place
+      link_contact((int32_t)id);
+      int32_t old_local = contacts[id].local_index;
       add_contact_to_graph((int32_t)id);
       remove_non_touching_contact(old_local);
       stats.begin_touch += 1;
@@ -828,6 +858,7 @@ void World::collide() {
       c.flags &= ~CF::SIM_STOPPED_TOUCHING;
       c.flags &= ~CF::TOUCHING;
       int32_t color = c.color_index, local = c.local_index;
+      unlink_contact((int32_t)id);
       add_non_touching_contact((int32_t)id);
       remove_contact_from_graph(c.edges[0].body_id, c.edges[1].body_id, color, local, (c.flags & CF::SIM_MESH_CONTACT) != 0);
       stats.end_touch += 1;
@@ -916,6 +947,29 @@ void World::finalize_bodies(const StepContext& ctx) {
     if (continuous[i] == 2) solve_continuous((int32_t)i, time_step);
 }
 
+// The island part of finalize (solver_integration.rs:735-752, merged over threads at
+// :522-538): a body still short of the time to sleep keeps its island awake; a sleepy body
+// in an island with removed constraints nominates it for splitting, the sleepiest (then the
+// largest island id) winning.
+void World::mark_awake_islands() {
+  awake_island_bits.assign(awake_islands.size(), 0);
+  split_candidate = kNullIndex;
+  split_candidate_time = 0.0f;
+  for (const BodySim& sim : awake_sims) {
+    const Body& body = bodies[(size_t)sim.body_id];
+    const Island& island = islands[(size_t)body.island_id];
+    if (body.sleep_time < 0.5f) {
+      awake_island_bits[(size_t)island.local_index] = 1;
+    } else if (island.constraint_remove_count > 0) {
+      if (body.sleep_time > split_candidate_time ||
+          (body.sleep_time == split_candidate_time && body.island_id > split_candidate)) {
+        split_candidate = body.island_id;
+        split_candidate_time = body.sleep_time;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // Step driver (solver_integration.rs:807-1015)
 void World::step(float time_step, int sub_step_count) {
@@ -941,12 +995,19 @@ void World::step(float time_step, int sub_step_count) {
   stats.ms_collide += 1e3 * (t2 - t1);
   if (time_step > 0.0f) {
     if (awake_sims.empty()) return;
+    // a split nominated by the last step (solver_integration.rs:879-884)
+    if (split_island_id != kNullIndex) {
+      split_island(split_island_id);
+      split_island_id = kNullIndex;
+    }
     prepare_joints(ctx);
     prepare_contacts(ctx);
     double t3 = omp_get_wtime();
     solve_stages(ctx);
     double t4 = omp_get_wtime();
     finalize_bodies(ctx);
+    mark_awake_islands();
+    if (enable_sleep) process_sleep();  // solver_integration.rs:1003-1005
     double t5 = omp_get_wtime();
     stats.ms_prepare += 1e3 * (t3 - t2);
     stats.ms_solve += 1e3 * (t4 - t3);

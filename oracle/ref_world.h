@@ -83,6 +83,7 @@ struct Body {
   int32_t id = kNullIndex;
   uint32_t flags = 0;
   BodyType type = kStatic;
+  int32_t island_id = kNullIndex, island_index = kNullIndex;
 };
 
 // Box hull with the reference's half-edge topology (src/collision/hull.rs:736-852).
@@ -179,6 +180,7 @@ struct Contact {
   float restitution = 0, rolling_resistance = 0;
   Vec3 tangent_velocity{};
   uint64_t created = 0;  // creation order: the order of a body's head-inserted contact list
+  int32_t island_id = kNullIndex, island_index = kNullIndex;
 };
 
 // src/containers/id_pool.rs: LIFO free list
@@ -212,6 +214,7 @@ struct Joint {
   int32_t joint_id = kNullIndex, set_index = kNullIndex, color_index = kNullIndex, local_index = kNullIndex;
   JointEdge edges[2];
   bool collide_connected = false;
+  int32_t island_id = kNullIndex, island_index = kNullIndex;
 };
 
 // JointDef + RevoluteJointDef (types/joint_defs.rs:30-58, :154-167)
@@ -353,6 +356,26 @@ struct ToiOutput {
 };
 ToiOutput time_of_impact(ToiProxy proxy_a, Sweep sweep_a, ToiProxy proxy_b, Sweep sweep_b, float max_fraction);
 
+// island.rs / world_runtime.rs Island, ContactLink, JointLink
+struct ContactLink { int32_t contact_id, body_id_a, body_id_b; };
+struct JointLink { int32_t joint_id, body_id_a, body_id_b; };
+struct Island {
+  int32_t island_id = kNullIndex, set_index = kNullIndex, local_index = kNullIndex;
+  std::vector<int32_t> bodies;
+  std::vector<ContactLink> contacts;
+  std::vector<JointLink> joints;
+  int32_t constraint_remove_count = 0;
+};
+// A sleeping solver set (world_runtime.rs SolverSet): body sims (no states), touching
+// contacts, joint sims and islands.
+struct SleepSet {
+  int32_t set_index = kNullIndex;
+  std::vector<BodySim> body_sims;
+  std::vector<int32_t> contact_indices;
+  std::vector<JointSim> joint_sims;
+  std::vector<int32_t> island_ids;
+};
+
 struct StepStats {
   int64_t pairs_created = 0, contacts_destroyed = 0, begin_touch = 0, end_touch = 0;
   int64_t recycled = 0, sat_calls = 0, sat_hits = 0, fast_bodies = 0, overflow_constraints = 0;
@@ -419,6 +442,17 @@ class World {
   uint64_t contact_serial = 0;
   std::vector<JointSim> static_joint_sims;  // joints with no dynamic body: never solved
 
+  // islands and sleeping sets (island.rs, sleep_island.rs, solver_set_ops.rs)
+  std::vector<Island> islands;
+  IDPool island_pool;
+  std::vector<int32_t> awake_islands;   // the awake set's islands (island ids by local index)
+  std::vector<SleepSet> sleep_sets;     // by solver set id (ids 0..2 are the static, disabled, awake sets)
+  IDPool set_pool;                      // starts past the three fixed sets
+  int32_t split_island_id = kNullIndex; // pending split, run at the next step
+  std::vector<uint8_t> awake_island_bits;  // by awake island local index, written by finalize
+  int32_t split_candidate = kNullIndex;
+  float split_candidate_time = 0.0f;
+
  private:
   void update_body_mass_data(int32_t body_id);
   void sync_body_flags(int32_t body_id);
@@ -428,8 +462,11 @@ class World {
   float compute_shape_margin(const Shape& s) const;
   BodySim& sim_of(const Body& b);
   const BodySim& sim_of(const Body& b) const;
+
+ public:
   int32_t encode_body_sim_index(const Body& b) const;
 
+ private:
   void update_broad_phase_pairs();
   bool should_create_pair(int32_t a, int32_t b) const;
   void create_contact(int32_t shape_a, int32_t shape_b, int32_t child);
@@ -472,6 +509,25 @@ class World {
   void restitution_overflow(const StepContext& ctx);
   void store_overflow();
   void finalize_bodies(const StepContext& ctx);
+  void mark_awake_islands();
+
+ public:
+  // islands and sleep (oracle/ref_island.cpp)
+  std::vector<int32_t>& island_list(int32_t set_index);
+  int32_t create_island(int32_t set_index);
+  void destroy_island(int32_t island_id);
+  void create_island_for_body(int32_t set_index, int32_t body_id);
+  int32_t merge_islands(int32_t a, int32_t b);
+  void link_contact(int32_t contact_id);
+  void unlink_contact(int32_t contact_id);
+  void link_joint(int32_t joint_id);
+  void refresh_body_contact_indices(int32_t body_id);
+  void split_island(int32_t base_id);
+  void try_sleep_island(int32_t island_id);
+  void process_sleep();
+  void wake_solver_set(int32_t set_index);
+  void add_joint_to_graph(const JointSim& sim, int32_t joint_id);
+  std::vector<int32_t>& contact_list(int32_t set_index);
   ToiProxy shape_proxy(const Shape& s) const;
   Vec3 shape_centroid(const Shape& s) const;
   bool is_shape_fast(const Shape& s, Vec3 centroid1, Vec3 centroid2, float rotation_chord, float safety_factor) const;
