@@ -17,6 +17,10 @@
  *   nsl_world_state_hash           World::world_state_hash      src/simulation/body_queries.rs:432
  *   nsl_world_get_contact_colors   constraint-graph colour of each contact (graph_tests.rs:152-172)
  *   nsl_world_get_contact_shapes   the shape pair of each contact id (world_runtime.rs:517-555)
+ *   nsl_world_body_location        Body::set_index / local_index / island_id; body_is_awake
+ *                                                               src/simulation/body_api.rs:836-839
+ *   nsl_world_island_summary       island pool and solver-set counts (island_split_tests.rs:46-117)
+ *   nsl_world_check_islands        the reference's index validation helpers
  *   nsl_time_of_impact             time_of_impact               src/collision/time_of_impact.rs:406-598
  *   nsl_world_get_profile          World::profile               src/types/world_def.rs:158-184
  *   nsl_world_*kernel_*            per-kernel device timing (no counterpart: the reference times phases
@@ -207,6 +211,18 @@ int nsl_world_get_contact_colors(nsl_world* world, int32_t* color, int32_t* loca
 /* Per contact id: its two shape ids (Contact::shape_id_a/_b, world_runtime.rs:517-555), -1 when
    the id is free. */
 int nsl_world_get_contact_shapes(nsl_world* world, int32_t* shape_a, int32_t* shape_b);
+/* A body's solver set (0 static, 2 awake, 3 and up sleeping), its index in that set and its
+   island id (-1 for a static body): Body::set_index / local_index / island_id
+   (world_runtime.rs:301-343), read by the reference's sleep tests (sleep_tests.rs:33-99);
+   body_is_awake (body_api.rs:836-839) is set == 2. */
+int nsl_world_body_location(nsl_world* world, int32_t body, int32_t* set, int32_t* local, int32_t* island);
+/* out[5]: island slots, live islands, awake islands, sleeping sets, the island to split at the
+   next step (-1: none) (island_split_tests.rs:46-117 reads the first two). */
+int nsl_world_island_summary(nsl_world* world, int64_t* out);
+/* *out = 0 when every island, solver-set and contact-list record agrees with the bodies' and
+   contacts' own records, else the number of the first failed check (the reference's
+   validate_graph_indices / validate_set_contacts / validate_islands, constraint_graph.rs:370-560). */
+int nsl_world_check_islands(nsl_world* world, int32_t* out);
 int nsl_world_get_profile(nsl_world* world, nsl_profile* out);
 /* Timing adds an event pair per launch and a wall-clock read per phase; results do not change. */
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on);

@@ -185,6 +185,9 @@ SIGNATURES = {
     "nsl_world_contact_capacity": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "nsl_world_get_contact_colors": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "nsl_world_get_contact_shapes": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "nsl_world_body_location": (ctypes.c_int, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "nsl_world_island_summary": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "nsl_world_check_islands": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "nsl_world_get_profile": (ctypes.c_int, [_P, ctypes.POINTER(Profile)]),
     "nsl_world_set_kernel_timing": (ctypes.c_int, [_P, _I]),
     "nsl_world_reset_kernel_stats": (ctypes.c_int, [_P]),

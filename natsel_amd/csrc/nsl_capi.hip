@@ -345,6 +345,23 @@ int nsl_world_get_contact_shapes(nsl_world* world, int32_t* shape_a, int32_t* sh
   return NSL_OK;
 }
 
+int nsl_world_body_location(nsl_world* world, int32_t body, int32_t* set, int32_t* local, int32_t* island) {
+  if (!world || !set || !local || !island) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { world->impl->body_location(body, set, local, island); });
+}
+
+int nsl_world_island_summary(nsl_world* world, int64_t* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  world->impl->island_summary(out);
+  return NSL_OK;
+}
+
+int nsl_world_check_islands(nsl_world* world, int32_t* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  *out = world->impl->check_islands();
+  return NSL_OK;
+}
+
 int nsl_world_get_profile(nsl_world* world, nsl_profile* out) {
   if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
   const nsl::StepProfile& p = world->impl->profile();

@@ -386,7 +386,7 @@ __device__ bool solve_continuous_body(const DeviceWorld& d, const ContinuousArgs
         continue;
       }
       const Shape cs = d.shapes[cid];
-      const BodySim& os = cs.body_enc >= 0 ? d.sims[cs.body_enc] : d.static_sims[-(cs.body_enc + 2)];
+      const BodySim& os = *sim_of(d, cs.body_enc, cid);
       if (os.flags & bflag::kIsBullet) continue;
       if (d.joint_pairs.slots != nullptr && pairset_contains(d.joint_pairs, pair_key(fp.body, cp.body))) continue;
       Vec3 store[2];

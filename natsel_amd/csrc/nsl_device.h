@@ -32,6 +32,8 @@ struct DeviceWorld {
   int32_t* fast_ids;        // this step's fast bodies as fast_base + awake index (count: counters[kCntFastList])
   int32_t fast_base;        // 0 for a world; the world's offset into a batch's awake bodies
   int32_t n_awake;
+  float* sleep_times;       // per awake index: the body's sleep time after finalize (null: not kept)
+  BodySim* sleep_sims;      // by body id: sims of sleeping bodies (encoding -1)
 };
 
 // Device work counters (DeviceWorld::counters).
@@ -114,6 +116,12 @@ struct SolvePlan {
   uint64_t* phase_log;            // null: no phase timing; else [2 * kMaxPhaseLog]: (class, wall clock)
 };
 constexpr int kMaxPhaseLog = 1024;
+
+// Location patches after bodies changed solver sets (nsl_kernels.hip): `items` holds
+// n_shapes (shape id, encoding) pairs, then n_contacts (contact id, enc_a, enc_b) and n_joints
+// (joint id, enc_a, enc_b) triples; `sims` holds n_sims sims copied to sleep_sims[sim.body_id].
+void launch_patch_locations(const DeviceWorld& d, const int32_t* items, int n_shapes, int n_contacts, int n_joints,
+                            const BodySim* sims, int n_sims, hipStream_t s);
 
 // kernels (nsl_kernels.hip)
 // `timer` may be null (no timing).

@@ -29,6 +29,34 @@ __global__ void k_init_contacts(DeviceWorld d, const int32_t* ids, const Contact
   pairset_insert(d.pairs, pair_key(c.shape_a, c.shape_b));
 }
 
+// One item per thread: shape encodings, contact encodings, joint encodings, sleeping sims.
+__global__ void k_patch_locations(DeviceWorld d, const int32_t* items, int n_shapes, int n_contacts, int n_joints,
+                                  const BodySim* sims, int n_sims) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_shapes) {
+    d.shapes[items[2 * t]].body_enc = items[2 * t + 1];
+    return;
+  }
+  t -= n_shapes;
+  const int32_t* tri = items + 2 * n_shapes;
+  if (t < n_contacts) {
+    Contact& c = d.contacts[tri[3 * t]];
+    c.enc_a = tri[3 * t + 1];
+    c.enc_b = tri[3 * t + 2];
+    return;
+  }
+  t -= n_contacts;
+  tri += 3 * n_contacts;
+  if (t < n_joints) {
+    JointSim& j = d.joints[tri[3 * t]];
+    j.enc_a = tri[3 * t + 1];
+    j.enc_b = tri[3 * t + 2];
+    return;
+  }
+  t -= n_joints;
+  if (t < n_sims) d.sleep_sims[sims[t].body_id] = sims[t];
+}
+
 __global__ void k_pairset_insert(PairSet set, const uint64_t* keys, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) pairset_insert(set, keys[t]);
@@ -354,6 +382,14 @@ inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
 
 void launch_init_contacts(const DeviceWorld& d, const int32_t* ids, const Contact* records, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_init_contacts, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, ids, records, n);
+}
+
+void launch_patch_locations(const DeviceWorld& d, const int32_t* items, int n_shapes, int n_contacts, int n_joints,
+                            const BodySim* sims, int n_sims, hipStream_t s) {
+  const int n = n_shapes + n_contacts + n_joints + n_sims;
+  if (n > 0)
+    hipLaunchKernelGGL(k_patch_locations, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, items, n_shapes, n_contacts, n_joints,
+                       sims, n_sims);
 }
 
 void launch_pairset_insert(PairSet set, const uint64_t* keys, int n, hipStream_t s) {

@@ -76,9 +76,12 @@ class HostScene {
   void get_bodies(double* pos, float* rot, float* v, float* w) const;
   uint64_t state_hash() const;
 
+  // set: 0 static, 2 awake, 3 and up a sleeping set (local: index in that set's bodies)
   struct HBody {
     int32_t type = kBodyStatic, set = 0, local = -1, head_shape = -1;
     uint32_t flags = 0;
+    int32_t head_contact = -1;               // contact_id << 1 | edge of the newest contact
+    int32_t island = -1, island_index = -1;  // kinematic and dynamic bodies
   };
   struct HShape {
     int32_t id = -1, body = -1, kind = kShapeHull, hull = -1;
@@ -90,10 +93,17 @@ class HostScene {
     Aabb fat{};
     Aabb aabb{};  // the shape's bounds at its body's last finalize or sweep
   };
-  int32_t encode(const HBody& b) const { return b.set == 2 ? b.local : -(b.local + 2); }
+  // Encoded body sim index (world_runtime.rs:903-925): awake index, static -(index + 2), and
+  // -1 for a sleeping body.
+  int32_t encode(const HBody& b) const { return b.set == 2 ? b.local : (b.set == 0 ? -(b.local + 2) : kNullIndex); }
+  const BodySim& sim_of(int32_t body_id) const {
+    const HBody& b = bodies_[(size_t)body_id];
+    return b.set == 2 ? awake_host_[(size_t)b.local] : (b.set == 0 ? static_host_[(size_t)b.local] : sleep_host_[(size_t)body_id]);
+  }
 
   std::vector<HBody> bodies_;
   std::vector<BodySim> awake_host_, static_host_;
+  std::vector<BodySim> sleep_host_;  // by body id: the sims of sleeping bodies
   std::vector<BodyState> states_host_;
   std::vector<Shape> shapes_;
   std::vector<HShape> hshapes_;

@@ -328,7 +328,7 @@ void HostScene::update_mass(int32_t body) {
 void HostScene::get_bodies(double* pos, float* rot, float* v, float* w) const {
   for (size_t i = 0; i < bodies_.size(); ++i) {
     const HBody& b = bodies_[i];
-    const BodySim& s = b.set == 2 ? awake_host_[b.local] : static_host_[b.local];
+    const BodySim& s = sim_of((int32_t)i);
     pos[3 * i] = s.transform.p.x;
     pos[3 * i + 1] = s.transform.p.y;
     pos[3 * i + 2] = s.transform.p.z;
@@ -351,7 +351,7 @@ uint64_t HostScene::state_hash() const {
   auto mix64 = [&](double f) { h = (h ^ bits_of(f)) * prime; };
   for (size_t i = 0; i < bodies_.size(); ++i) {
     const HBody& b = bodies_[i];
-    const BodySim& s = b.set == 2 ? awake_host_[b.local] : static_host_[b.local];
+    const BodySim& s = sim_of((int32_t)i);
     mix64(s.transform.p.x);
     mix64(s.transform.p.y);
     mix64(s.transform.p.z);

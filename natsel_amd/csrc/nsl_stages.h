@@ -42,15 +42,23 @@ struct BodyView {
   bool is_static;
 };
 
-__device__ __forceinline__ BodyView body_view(const DeviceWorld& d, int32_t enc) {
+// The sim behind an encoding; a sleeping body (-1) is found through its shape's body
+// (collide_pass.rs:105-154).
+__device__ __forceinline__ const BodySim* sim_of(const DeviceWorld& d, int32_t enc, int32_t shape) {
+  if (enc >= 0) return &d.sims[enc];
+  if (enc < kNullIndex) return &d.static_sims[-enc - 2];
+  return &d.sleep_sims[d.shapes[shape].body_id];
+}
+
+__device__ __forceinline__ BodyView body_view(const DeviceWorld& d, int32_t enc, int32_t shape) {
   BodyView v;
-  const BodySim* s = enc >= 0 ? &d.sims[enc] : &d.static_sims[-enc - 2];
+  const BodySim* s = sim_of(d, enc, shape);
   v.transform = s->transform;
   v.center = s->center;
   v.local_center = s->local_center;
   v.max_extent = s->max_extent;
   v.flags = s->flags;
-  v.is_static = enc < 0;
+  v.is_static = enc < kNullIndex;
   return v;
 }
 
@@ -137,7 +145,7 @@ __device__ uint8_t collide_contact(const DeviceWorld& d, int32_t id) {
     d.contacts[id].flags = c.flags;
     return kChangeDisjoint;
   }
-  BodyView ba = body_view(d, c.enc_a), bb = body_view(d, c.enc_b);
+  BodyView ba = body_view(d, c.enc_a, c.shape_a), bb = body_view(d, c.enc_b, c.shape_b);
   const bool was_touching = (c.flags & cflag::kSimTouching) != 0;
   const WXf xf_a = ba.transform, xf_b = bb.transform;
   const bool is_fast = ((ba.flags | bb.flags) & bflag::kIsFast) != 0;
@@ -1001,6 +1009,7 @@ __device__ __forceinline__ void finalize_item(const DeviceWorld& d, int i, const
     sim.rotation0 = sim.transform.q;
     sim.sleep_time += time_step;
   }
+  if (d.sleep_times) d.sleep_times[i] = sim.sleep_time;
   Mat3 rm = matrix_from_quat(sim.transform.q);
   sim.inv_inertia_world = mul_mm(mul_mm(rm, sim.inv_inertia_local), transpose(rm));
   if ((sim.flags & bflag::kIsFast) == 0) {

@@ -55,6 +55,13 @@ class GpuWorld : public HostScene {
   void get_contact_colors(int32_t* color, int32_t* local, int32_t* alive) const;
   // Per contact id: its shapes, -1 when the id is free.
   void get_contact_shapes(int32_t* shape_a, int32_t* shape_b) const;
+  // A body's solver set (0 static, 2 awake, 3 and up sleeping), index in it and island id.
+  void body_location(int32_t id, int32_t* set, int32_t* local, int32_t* island) const;
+  // Island slots, live islands, awake islands, sleeping sets, the island to split next step.
+  void island_summary(int64_t out[5]) const;
+  // 0 when the island, solver-set and encoding records agree with each other; else the
+  // number of the first failed check (the same checks as the oracle's).
+  int32_t check_islands() const;
   const StepProfile& profile() const { return profile_; }
 
   // Device time per launch class (HIP events on the world's stream) and per phase of the
@@ -87,20 +94,51 @@ class GpuWorld : public HostScene {
   void restore_state(const Snapshot& s);
 
  private:
+  // set: 1 disabled, 2 awake, 3 and up sleeping; local: index in the set's list or colour
   struct HContact {
     int32_t shape_a = -1, shape_b = -1, color = -1, local = -1, set = -1;
     int32_t alive = 0;
     float restitution = 0;
     uint64_t created = 0;  // creation order (a body's contact list is newest first)
+    // edges of the bodies' contact lists (contact_creation.rs:357-384): body, prev and next
+    // key (contact_id << 1 | edge)
+    int32_t body[2] = {-1, -1}, prev[2] = {-1, -1}, next[2] = {-1, -1};
+    int32_t island = -1, island_index = -1;
   };
+  // set: 0 between non-dynamic bodies (never solved), 2 in the graph, 3 and up sleeping
   struct HJoint {
-    int32_t body_a = -1, body_b = -1, color = -1;
+    int32_t body_a = -1, body_b = -1, color = -1, local = -1, set = 0;
+    int32_t island = -1, island_index = -1;
+    int32_t enc_a = -1, enc_b = -1;  // encodings held by the device record
+  };
+  // islands (island.rs) and sleeping solver sets (solver_set.rs)
+  struct ContactLink { int32_t contact, body_a, body_b; };
+  struct JointLink { int32_t joint, body_a, body_b; };
+  struct Island {
+    int32_t id = -1, set = -1, local = -1, remove_count = 0;
+    std::vector<int32_t> bodies;
+    std::vector<ContactLink> contacts;
+    std::vector<JointLink> joints;
+  };
+  struct SleepSet {
+    std::vector<int32_t> bodies, contacts, joints, islands;  // ids, in set order
+  };
+  struct IdPool {  // id_pool.rs: LIFO free list
+    std::vector<int32_t> free;
+    int32_t next = 0;
+    int32_t alloc() {
+      if (free.empty()) return next++;
+      const int32_t id = free.back();
+      free.pop_back();
+      return id;
+    }
+    void release(int32_t id) { free.push_back(id); }  // no live id equals next
   };
   struct ColorSet {
     std::vector<uint64_t> bits;
     std::vector<int32_t> convex;
     std::vector<int32_t> overflow;
-    std::vector<int32_t> joints;  // joint ids in creation order
+    std::vector<int32_t> joints;  // joint ids in graph order (appended, swap-removed)
     bool get(int32_t b) const { uint32_t w = (uint32_t)b >> 6; return w < bits.size() && ((bits[w] >> (b & 63)) & 1); }
     void set(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w >= bits.size()) bits.resize(w + 1, 0); bits[w] |= 1ull << (b & 63); }
     void clear(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w < bits.size()) bits[w] &= ~(1ull << (b & 63)); }
@@ -120,6 +158,29 @@ class GpuWorld : public HostScene {
   void remove_from_graph(int32_t body_a, int32_t body_b, int32_t color, int32_t local);
   void state_pass();
   int32_t assign_joint_color(int32_t a, int32_t b);
+
+  // islands and sleep (nsl_islands.hip)
+  std::vector<int32_t>& island_list(int32_t set);
+  std::vector<int32_t>& contact_list(int32_t set);
+  int32_t create_island(int32_t set);
+  void destroy_island(int32_t id);
+  int32_t merge_islands(int32_t a, int32_t b);
+  void link_contact(int32_t id);
+  void unlink_contact(int32_t id);
+  void link_joint(int32_t id);
+  void link_body_contact(int32_t id);
+  void unlink_body_contact(int32_t id);
+  void split_island(int32_t id);
+  void mark_awake_islands();
+  void process_sleep();
+  void try_sleep_island(int32_t id);
+  void wake_solver_set(int32_t set);
+  void add_joint_to_graph(int32_t id);
+  void remove_joint_from_graph(int32_t id);
+  void mark_body_moved(int32_t body);
+  // Brings the device up to the host's body locations: awake sims and states, sleeping sims,
+  // the encodings held by shapes, awake-set contacts and graph joints.
+  void flush_locations();
   void upload_joint_pairs();
 
   int device_;
@@ -178,6 +239,27 @@ class GpuWorld : public HostScene {
   StepProfile profile_{};
 
   std::vector<HJoint> hjoints_;
+  std::vector<Island> islands_;
+  IdPool island_pool_;
+  std::vector<int32_t> awake_islands_;
+  std::vector<SleepSet> sleep_sets_;  // by set id (3 and up)
+  IdPool set_pool_{{}, 3};
+  std::vector<int32_t> disabled_;     // contacts of the disabled set
+  int32_t split_island_ = -1;         // split at the start of the next solve
+  std::vector<uint8_t> island_awake_bits_;
+  int32_t split_candidate_ = -1;
+  float split_candidate_time_ = 0.0f;
+  std::vector<float> sleep_times_host_;  // per awake index, from the last finalize
+  size_t cap_sleep_times_ = 0, cap_sleep_sims_ = 0;
+  // bodies whose location changed since the last flush, and what follows from it
+  std::vector<uint8_t> body_moved_;
+  std::vector<int32_t> moved_bodies_;
+  std::vector<int32_t> slept_bodies_;  // sims to copy to the device's sleeping sims
+  bool locations_dirty_ = false;
+  std::vector<uint8_t> contact_mark_;
+  int32_t* d_patch_ = nullptr;
+  BodySim* d_patch_sims_ = nullptr;
+  size_t cap_patch_ = 0, cap_patch_sims_ = 0;
   uint64_t contact_serial_ = 0;
   size_t cap_joints_ = 0, cap_joint_ids_ = 0, cap_erase_keys_ = 0;
   uint64_t* d_erase_keys_ = nullptr;

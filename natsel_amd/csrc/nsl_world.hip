@@ -53,7 +53,8 @@ GpuWorld::~GpuWorld() {
                   d_.contact_change, d_.constraints, d_.overflow, d_.collide_ids, d_.solve_ids, d_.overflow_ids,
                   d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
-                  d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_};
+                  d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
+                  d_.sleep_sims, d_patch_, d_patch_sims_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -84,10 +85,19 @@ int32_t GpuWorld::create_body(const BodyDesc& desc) {
   if (device_newer_) pull_bodies();
   const int32_t id = HostScene::create_body(desc);
   bodies_dirty_ = true;
+  // world_runtime.rs:1626-1629: kinematic and dynamic bodies get an island
+  if (bodies_[(size_t)id].set == 2) {
+    const int32_t island = create_island(2);
+    islands_[(size_t)island].bodies.push_back(id);
+    bodies_[(size_t)id].island = island;
+    bodies_[(size_t)id].island_index = 0;
+  }
   return id;
 }
 
 int32_t GpuWorld::add_shape(int32_t body, Shape shape, const Material& m, float radius) {
+  if (body >= 0 && (size_t)body < bodies_.size() && bodies_[(size_t)body].set >= 3)
+    throw std::invalid_argument("shapes cannot be added to a sleeping body");
   if (device_newer_) pull_bodies();
   isotropic_inertia_tolerance = options.isotropic_inertia_tolerance;
   const int32_t id = HostScene::add_shape(body, shape, m, radius);
@@ -249,15 +259,21 @@ void GpuWorld::create_contact(int32_t a, int32_t b) {
     id = contact_pool_next_++;
   }
   if (id >= (int32_t)hcontacts_.size()) hcontacts_.resize(id + 1);
+  hcontacts_[id] = HContact{};
   HContact& hc = hcontacts_[id];
   hc.shape_a = a;
   hc.shape_b = b;
   hc.color = kNullIndex;
-  hc.set = 2;
+  // the awake list when a body is awake, else the disabled set
+  hc.set = (ba.set == 2 || bb.set == 2) ? 2 : 1;
   hc.alive = 1;
   hc.restitution = maxf(sa.restitution, sb.restitution);
-  hc.local = (int32_t)non_touching_.size();
-  non_touching_.push_back(id);
+  std::vector<int32_t>& list = hc.set == 2 ? non_touching_ : disabled_;
+  hc.local = (int32_t)list.size();
+  list.push_back(id);
+  hc.body[0] = sa.body;
+  hc.body[1] = sb.body;
+  link_body_contact(id);
   Contact c{};
   c.shape_a = a;
   c.shape_b = b;
@@ -292,9 +308,12 @@ void GpuWorld::destroy_contact(int32_t id) {
   HContact& c = hcontacts_[id];
   if (!c.alive) return;
   pair_set_tombs_ += 1;  // the collide kernel erased the key
+  unlink_body_contact(id);
+  if (c.island != kNullIndex) unlink_contact(id);  // contact_creation.rs:501-508
   if (c.color == kNullIndex) {
     int32_t local = c.local;
-    if (swap_remove(non_touching_, (size_t)local) != kNullIndex) hcontacts_[non_touching_[local]].local = local;
+    std::vector<int32_t>& list = contact_list(c.set);
+    if (swap_remove(list, (size_t)local) != kNullIndex) hcontacts_[list[local]].local = local;
   } else {
     remove_from_graph(hshapes_[c.shape_a].body, hshapes_[c.shape_b].body, c.color, c.local);
   }
@@ -361,10 +380,13 @@ void GpuWorld::state_pass() {
     if (ch == kChangeDisjoint) {
       destroy_contact((int32_t)id);
     } else if (ch == kChangeBegan) {
+      // linking first wakes a sleeping partner (island.rs:126-171)
+      link_contact((int32_t)id);
       int32_t old_local = hcontacts_[id].local;
       add_to_graph((int32_t)id);
       if (swap_remove(non_touching_, (size_t)old_local) != kNullIndex) hcontacts_[non_touching_[old_local]].local = old_local;
     } else if (ch == kChangeEnded) {
+      unlink_contact((int32_t)id);
       HContact& c = hcontacts_[id];
       int32_t color = c.color, local = c.local;
       c.color = kNullIndex;
@@ -422,12 +444,16 @@ void GpuWorld::upload_joint_pairs() {
   NSL_CHECK(hipStreamSynchronize(stream_));
 }
 
-// joint.rs:164-375 + revolute_joint.rs:149-174 (no sleeping sets: a joint with a dynamic
-// body goes into the graph, one between non-dynamic bodies is kept but never solved).
+// joint.rs:164-375 + revolute_joint.rs:149-174: a joint with a dynamic body wakes a
+// sleeping one, goes into the graph and joins the bodies' islands; one between non-dynamic
+// bodies is kept but never solved.
 int32_t GpuWorld::create_revolute_joint(const JointDesc& desc) {
   const int32_t a = desc.body_a, b = desc.body_b;
   if (a == b || a < 0 || b < 0 || a >= (int32_t)bodies_.size() || b >= (int32_t)bodies_.size())
     throw std::invalid_argument("joint bodies must be two distinct existing bodies");
+  // the reference keeps such a joint asleep by merging sleeping sets (joint.rs:294-340)
+  if (bodies_[a].set != 2 && bodies_[b].set != 2 && (bodies_[a].set >= 3 || bodies_[b].set >= 3))
+    throw std::invalid_argument("a joint between a sleeping body and no awake body is not supported");
   NSL_CHECK(hipSetDevice(device_));
 
   if (!desc.collide_connected) {
@@ -460,13 +486,17 @@ int32_t GpuWorld::create_revolute_joint(const JointDesc& desc) {
   HJoint hj;
   hj.body_a = a;
   hj.body_b = b;
-  hj.color = bodies_[a].type != kBodyDynamic && bodies_[b].type != kBodyDynamic ? kNullIndex : assign_joint_color(a, b);
-  if (hj.color != kNullIndex) colors_[hj.color].joints.push_back(id);
   hjoints_.push_back(hj);
+  if (bodies_[a].type == kBodyDynamic || bodies_[b].type == kBodyDynamic) {
+    const int32_t max_set = std::max(bodies_[a].set, bodies_[b].set);
+    if (max_set >= 3) wake_solver_set(max_set);
+    hjoints_[id].set = 2;
+    add_joint_to_graph(id);
+  }
 
   JointSim j{};
-  j.enc_a = encode(bodies_[a]);
-  j.enc_b = encode(bodies_[b]);
+  j.enc_a = hjoints_[id].enc_a = encode(bodies_[a]);
+  j.enc_b = hjoints_[id].enc_b = encode(bodies_[b]);
   j.local_frame_a = desc.local_frame_a;
   j.local_frame_b = desc.local_frame_b;
   j.constraint_hertz = desc.constraint_hertz;
@@ -486,6 +516,7 @@ int32_t GpuWorld::create_revolute_joint(const JointDesc& desc) {
   NSL_CHECK(hipMemcpyAsync(&d_.joints[id], &j, sizeof(JointSim), hipMemcpyHostToDevice, stream_));
   NSL_CHECK(hipStreamSynchronize(stream_));
   joints_dirty_ = true;
+  if (hjoints_[id].set > 1) link_joint(id);  // joint.rs:366-370
   return id;
 }
 
@@ -562,8 +593,17 @@ struct GpuWorld::Snapshot {
   uint64_t* pair_slots = nullptr;
   uint32_t pair_mask = 0;
   JointSim* joints = nullptr;
+  BodySim* sleep_sims = nullptr;
+  size_t n_sleep_sims = 0;
   int32_t counters[kCounterCount] = {};
   std::vector<HBody> bodies;
+  std::vector<HJoint> hjoints;
+  std::vector<BodySim> sleep_host;
+  std::vector<Island> islands;
+  IdPool island_pool, set_pool;
+  std::vector<int32_t> awake_islands, disabled;
+  std::vector<SleepSet> sleep_sets;
+  int32_t split_island = -1;
   std::vector<BodySim> awake_host;
   std::vector<BodyState> states_host;
   std::vector<HShape> hshapes;
@@ -593,7 +633,7 @@ void GpuWorld::destroy_snapshot(Snapshot* s) { delete s; }
 
 GpuWorld::Snapshot* GpuWorld::save_state() {
   NSL_CHECK(hipSetDevice(device_));
-  if (bodies_dirty_ || shapes_dirty_) upload_bodies();
+  flush_locations();
   if (device_newer_) pull_bodies();
   auto s = std::make_unique<Snapshot>();
   s->device = device_;
@@ -614,6 +654,10 @@ GpuWorld::Snapshot* GpuWorld::save_state() {
     s->pair_slots = s->copy_of(d_.pairs.slots, (size_t)d_.pairs.mask + 1, stream_);
   }
   s->joints = s->copy_of(d_.joints, hjoints_.size(), stream_);
+  if (d_.sleep_sims) {
+    s->n_sleep_sims = std::min(cap_sleep_sims_, bodies_.size());
+    s->sleep_sims = s->copy_of(d_.sleep_sims, s->n_sleep_sims, stream_);
+  }
   NSL_CHECK(hipMemcpyAsync(s->counters, d_.counters, sizeof(s->counters), hipMemcpyDeviceToHost, stream_));
   NSL_CHECK(hipStreamSynchronize(stream_));
   s->bodies = bodies_;
@@ -630,6 +674,15 @@ GpuWorld::Snapshot* GpuWorld::save_state() {
   s->pair_set_tombs = pair_set_tombs_;
   s->pairs_pending = pairs_pending_;
   s->contact_serial = contact_serial_;
+  s->hjoints = hjoints_;
+  s->sleep_host = sleep_host_;
+  s->islands = islands_;
+  s->island_pool = island_pool_;
+  s->set_pool = set_pool_;
+  s->awake_islands = awake_islands_;
+  s->disabled = disabled_;
+  s->sleep_sets = sleep_sets_;
+  s->split_island = split_island_;
   return s.release();
 }
 
@@ -639,6 +692,13 @@ void GpuWorld::restore_state(const Snapshot& s) {
   NSL_CHECK(hipSetDevice(device_));
   if (bodies_dirty_ || shapes_dirty_) upload_bodies();
   NSL_CHECK(hipStreamSynchronize(stream_));
+  // the awake set's size and order come back with the snapshot's host records
+  if (s.n_awake > cap_sims_ || s.n_awake > cap_states_) {
+    ensure(d_.sims, cap_sims_, s.n_awake, false);
+    ensure(d_.states, cap_states_, s.n_awake, false);
+  }
+  ensure(d_.fast_ids, cap_fast_, std::max<size_t>(s.n_awake, 1), false);
+  if (s.sleep_sims) ensure(d_.sleep_sims, cap_sleep_sims_, s.n_sleep_sims, true);
   auto back = [&](auto* dst, const auto* src, size_t n) {
     if (n) NSL_CHECK(hipMemcpyAsync(dst, src, n * sizeof(*src), hipMemcpyDeviceToDevice, stream_));
   };
@@ -658,6 +718,7 @@ void GpuWorld::restore_state(const Snapshot& s) {
     back(d_.pairs.slots, s.pair_slots, (size_t)s.pair_mask + 1);
   }
   back(d_.joints, s.joints, s.n_joints);
+  if (s.sleep_sims) back(d_.sleep_sims, s.sleep_sims, s.n_sleep_sims);
   NSL_CHECK(hipMemcpyAsync(d_.counters, s.counters, sizeof(s.counters), hipMemcpyHostToDevice, stream_));
   NSL_CHECK(hipStreamSynchronize(stream_));
   bodies_ = s.bodies;
@@ -674,8 +735,27 @@ void GpuWorld::restore_state(const Snapshot& s) {
   pair_set_tombs_ = s.pair_set_tombs;
   pairs_pending_ = s.pairs_pending;
   contact_serial_ = s.contact_serial;
+  hjoints_ = s.hjoints;
+  sleep_host_ = s.sleep_host;
+  islands_ = s.islands;
+  island_pool_ = s.island_pool;
+  set_pool_ = s.set_pool;
+  awake_islands_ = s.awake_islands;
+  disabled_ = s.disabled;
+  sleep_sets_ = s.sleep_sets;
+  split_island_ = s.split_island;
+  d_.n_awake = (int32_t)s.n_awake;
+  bullet_bodies_ = 0;
+  for (const BodySim& b : awake_host_)
+    if (b.body_type == kBodyDynamic && (b.flags & bflag::kIsBullet)) bullet_bodies_ += 1;
   device_newer_ = false;
   joints_dirty_ = true;  // the colour lists of joints are rebuilt from colors_
+  // shapes hold their bodies' encodings: bring them to the restored locations
+  moved_bodies_.clear();
+  slept_bodies_.clear();
+  body_moved_.assign(bodies_.size(), 0);
+  for (size_t b = 0; b < bodies_.size(); ++b) mark_body_moved((int32_t)b);
+  flush_locations();
 }
 
 // ---------------------------------------------------------------------------------
@@ -722,6 +802,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     init_ids_.clear();
     init_records_.clear();
   }
+  flush_locations();  // bodies put to sleep by the last step, bodies woken by joints
   if (dt <= 0.0f) return;
 
   // collide list: colours' contacts then the non-touching contacts (collide_pass.rs:175-190)
@@ -743,9 +824,19 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   }
   const double t2 = now_ms();
   state_pass();
+  flush_locations();  // bodies woken by contacts that began touching
   const double t3 = now_ms();
 
   if (d_.n_awake == 0) return;
+  // an island nominated by the last step is split before the solve (solver_integration.rs:879-884)
+  if (split_island_ != kNullIndex) {
+    split_island(split_island_);
+    split_island_ = kNullIndex;
+  }
+  if (options.enable_sleep) {
+    ensure(d_.sleep_times, cap_sleep_times_, (size_t)d_.n_awake, false);
+    sleep_times_host_.resize((size_t)d_.n_awake);
+  }
 
   // solve lists: colour-major slots (solver_prepare.rs:335-474)
   std::vector<int32_t> solve;
@@ -811,13 +902,19 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   // more workgroups add grid-barrier arrivals faster than they add useful lanes: on the
   // settled 132k-body world 96 workgroups of 128 lanes beat 48..80 and 112..256
   blocks = std::max(1, std::min(blocks, std::min(max_blocks_, 96)));
-  launch_solve_step(d_, plan, p, blocks, stream_, tm);
+  // taken after every buffer above is in place: the kernels get the pointers of this step
+  DeviceWorld dw = d_;
+  if (!options.enable_sleep) dw.sleep_times = nullptr;
+  launch_solve_step(dw, plan, p, blocks, stream_, tm);
   // continuous collision of the bodies finalize flagged fast, then of the bullets
   if (options.enable_continuous) {
-    launch_continuous(d_, ccd_, p, false, d_.n_awake, stream_, tm);
-    if (bullet_bodies_ > 0) launch_continuous(d_, ccd_, p, true, d_.n_awake, stream_, tm);
+    launch_continuous(dw, ccd_, p, false, d_.n_awake, stream_, tm);
+    if (bullet_bodies_ > 0) launch_continuous(dw, ccd_, p, true, d_.n_awake, stream_, tm);
   }
   device_newer_ = true;
+  if (options.enable_sleep)
+    NSL_CHECK(hipMemcpyAsync(sleep_times_host_.data(), d_.sleep_times, sleep_times_host_.size() * sizeof(float),
+                             hipMemcpyDeviceToHost, stream_));
 
   // counters[6] (proxies moved by finalize) tells the next pair update whether to run
   NSL_CHECK(hipMemcpyAsync(counters_host_, d_.counters, sizeof(counters_host_), hipMemcpyDeviceToHost, stream_));
@@ -830,6 +927,11 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     barrier_host_ = GridBarrier{};
     NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
     throw std::runtime_error("step kernel: grid barrier wait exceeded its limit (workgroups not co-resident?)");
+  }
+  // solver_integration.rs:1003-1005
+  if (options.enable_sleep) {
+    mark_awake_islands();
+    process_sleep();
   }
   if (timing_) {
     for (int i = 0; i < timer_.used; ++i) {

@@ -9,6 +9,7 @@ References:
 """
 from __future__ import annotations
 
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -787,4 +788,32 @@ def roll_off() -> Scene:
     sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5)))
     s = sc.add_body(BodySpec(DYNAMIC, (0.0, 1.25, 0.0), linear_velocity=(1.5, 0.0, 0.0)))
     sc.add_shape(ShapeSpec(s, SPHERE, (0.0, 0.0, 0.0, 0.25), friction=0.1))
+    return sc
+
+
+def jointed_sleepers(drop_height: float = 8.0) -> Scene:
+    """Two boxes resting side by side on the ground, hinged together along their shared edge
+    at the top (so one island holds both), fall asleep together; a ball dropped from
+    `drop_height` onto the first later wakes both, and all settle and sleep again. Bodies:
+    ground, box, box, ball; one joint."""
+    sc = Scene(name="jointed_sleepers", enable_sleep=True)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -0.5, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (10.0, 0.5, 10.0)))
+    a = sc.add_body(BodySpec(DYNAMIC, (-0.55, 0.5, 0.0)))
+    sc.add_shape(ShapeSpec(a, BOX, (0.5, 0.5, 0.5)))
+    b = sc.add_body(BodySpec(DYNAMIC, (0.55, 0.5, 0.0)))
+    sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5)))
+    sc.joints.append(JointSpec(a, b, frame_a=(0.55, 0.5, 0.0, 0.0, 0.0, 0.0, 1.0), frame_b=(-0.55, 0.5, 0.0, 0.0, 0.0, 0.0, 1.0)))
+    s = sc.add_body(BodySpec(DYNAMIC, (-0.45, drop_height, 0.1)))
+    sc.add_shape(ShapeSpec(s, SPHERE, (0.0, 0.0, 0.0, 0.25)))
+    return sc
+
+
+def sleepy_stacks(count: int, seed: int) -> Scene:
+    """`stacks` with sleep on for the world and every body: towers topple, bodies come to rest
+    in many islands that merge, split, fall asleep and are woken by falling neighbours."""
+    sc = stacks(count, seed)
+    sc.name = f"sleepy_stacks_{count}"
+    sc.enable_sleep = True
+    sc.bodies = [dataclasses.replace(b, enable_sleep=True) for b in sc.bodies]
     return sc

@@ -270,6 +270,27 @@ class World:
         capi.check(self._lib.nsl_world_get_contact_shapes(self._h, f(a), f(b)))
         return a, b
 
+    def body_location(self, body: int) -> tuple:
+        """(solver set, index in it, island id) of a body: set 0 static, 2 awake, 3+ sleeping."""
+        out = [ctypes.c_int32() for _ in range(3)]
+        capi.check(self._lib.nsl_world_body_location(self._h, int(body), *[ctypes.byref(x) for x in out]))
+        return tuple(x.value for x in out)
+
+    def body_is_awake(self, body: int) -> bool:
+        """World::body_is_awake (body_api.rs:836-839)."""
+        return self.body_location(body)[0] == 2
+
+    def island_summary(self) -> dict:
+        out = (ctypes.c_int64 * 5)()
+        capi.check(self._lib.nsl_world_island_summary(self._h, out))
+        return dict(zip(["island_slots", "islands", "awake_islands", "sleeping_sets", "split_island"], list(out)))
+
+    def check_islands(self) -> int:
+        """0 when the island and solver-set records agree; else the number of the failed check."""
+        out = ctypes.c_int32()
+        capi.check(self._lib.nsl_world_check_islands(self._h, ctypes.byref(out)))
+        return out.value
+
     def set_kernel_timing(self, on: bool) -> None:
         capi.check(self._lib.nsl_world_set_kernel_timing(self._h, int(on)))
 

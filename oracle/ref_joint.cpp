@@ -240,7 +240,9 @@ int32_t World::create_revolute_joint(const RevoluteJointDef& def) {
 
 const JointSim& World::joint_sim(int32_t jid) const {
   const Joint& j = joints[jid];
-  return j.set_index == 2 ? colors[j.color_index].joint_sims[j.local_index] : static_joint_sims[j.local_index];
+  if (j.set_index == 2) return colors[j.color_index].joint_sims[j.local_index];
+  if (j.set_index >= 3) return sleep_sets[(size_t)j.set_index].joint_sims[(size_t)j.local_index];
+  return static_joint_sims[j.local_index];
 }
 
 // revolute_joint.rs:176-261 (body data from solver_prepare.rs:217-241)

@@ -99,3 +99,46 @@ def test_pyramid_sleeps_entirely():
     for _ in range(30):
         w.step(DT, SUB)
     assert np.array_equal(w.body_arrays()[0], pos)
+
+
+def test_jointed_bodies_sleep_and_wake_together():
+    """A joint joins its bodies' islands: both boxes fall asleep in one set with the joint out
+    of the graph (colour -1) and its impulses held, and the ball wakes both and returns the
+    joint to the graph."""
+    w = _load(scenes.jointed_sleepers())
+    asleep = woke = held = None
+    for i in range(600):
+        w.step(DT, SUB)
+        assert w.check_islands() == 0, i
+        a, b, ball = w.body_location(1), w.body_location(2), w.body_location(3)
+        if asleep is None and a[0] >= 3:
+            asleep = i
+            assert b[0] == a[0] and ball[0] == 2 and w.joint(0)[1] == -1
+            held = w.joint(0)[0]
+        if asleep is not None and woke is None:
+            if a[0] >= 3:
+                assert np.array_equal(w.joint(0)[0], held)
+            else:
+                woke = i
+                assert b[0] == 2 and w.joint(0)[1] >= 0
+        if woke is not None and a[0] >= 3 and ball[0] >= 3:
+            break
+    assert asleep is not None and woke is not None and a[0] == b[0] == ball[0] >= 3, (asleep, woke, a, b, ball)
+    assert held.any()
+
+
+def test_sleepy_stacks_keep_their_records():
+    """Toppling towers with sleep on: islands merge, split, sleep and wake (all of it happens
+    within 600 steps) while every island, set and list record stays consistent."""
+    w = _load(scenes.sleepy_stacks(64, 7))
+    slept = woke = split = 0
+    prev = w.island_summary()
+    for i in range(600):
+        w.step(DT, SUB)
+        assert w.check_islands() == 0, i
+        s = w.island_summary()
+        slept += s["sleeping_sets"] > prev["sleeping_sets"]
+        woke += s["sleeping_sets"] < prev["sleeping_sets"]
+        split += s["split_island"] >= 0
+        prev = s
+    assert slept and woke and split, (slept, woke, split)
