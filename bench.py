@@ -112,6 +112,7 @@ def cpu_baseline(scene, seconds: float, max_steps: int) -> dict:
     threads = pyoracle.max_threads()
     value, steps, dt, phases = _oracle_rate(scene, threads, 0.7 * seconds, max_steps)
     value1, steps1, dt1, _ = _oracle_rate(scene, 1, 0.3 * seconds, max_steps)
+    pyoracle.set_threads(threads)
     return {
         "value": value,
         "unit": "body-substeps/s",
@@ -229,6 +230,65 @@ def large_world_measurement(steps: int) -> dict:
     }
 
 
+def defaults_measurement(steps: int, cpu_seconds: float) -> dict:
+    """The 10k pile with the reference's defaults (sleep and continuous collision on for the
+    world and every body): body-substeps/s counting awake bodies, steps 1..`steps` after one
+    warmup step; the oracle on the host threads from step 1 for a bounded sample, counting
+    the same way; and the GPU's hash after the sample's steps against the oracle's."""
+    import torch
+
+    from natsel_amd import scenes
+    from natsel_amd.world import load_scene
+    from oracle import pyoracle
+
+    sc = scenes.pile_defaults(10_000)
+    world = load_scene(sc)
+    world.step(sc.dt, sc.substeps)
+    world.reset_kernel_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        world.step(sc.dt, sc.substeps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ks = world.kernel_stats()
+    summary = world.island_summary()
+    world.close()
+    awake_substeps = ks["bodies"] * sc.substeps
+
+    pyoracle.set_threads(pyoracle.max_threads())
+    ref = pyoracle.load_scene(sc)
+    awake, n = 0, 0
+    c0 = time.perf_counter()
+    while n < steps and time.perf_counter() - c0 < cpu_seconds:
+        awake += ref.awake_body_count()  # awake bodies as the step starts (the GPU counts them as solved)
+        ref.step(sc.dt, sc.substeps)
+        n += 1
+    cpu_s = time.perf_counter() - c0
+    check = load_scene(sc)
+    for _ in range(n):
+        check.step(sc.dt, sc.substeps)
+    same = check.world_state_hash() == ref.world_state_hash()
+    check.close()
+    return {
+        "metric": "body-substeps/s (10k pile, reference defaults)",
+        "value": awake_substeps / elapsed,
+        "unit": "body-substeps/s",
+        "ms_per_step": 1e3 * elapsed / steps,
+        "steps": steps,
+        "config": {"workload": "the 10k random pile with sleep and continuous collision on (the reference's defaults), "
+                               "dt 1/60, 4 substeps; awake bodies counted per step", "bodies": sc.dynamic_count},
+        "awake_bodies_per_step": ks["bodies"] / max(ks["steps"], 1),
+        "toi_calls": ks["toi_calls"],
+        "toi_hits": ks["toi_hits"],
+        "islands_at_end": summary,
+        "host_ms_per_step": {k: v / steps for k, v in ks["host_ms"].items()},
+        "cpu_baseline": {"value": awake * sc.substeps / cpu_s, "unit": "body-substeps/s", "cores": pyoracle.max_threads(),
+                         "kind": "port", "sample": f"CPU oracle, steps 1..{n} of the same scene", "steps": n, "seconds": cpu_s},
+        "hash_equal_to_oracle_after_sample": same,
+    }
+
+
 def adjoint_measurement(worlds: int, steps: int, fd_worlds: int = 16) -> dict:
     """Differentiable rollout: `worlds` worlds of 32 cubes pushed across the ground
     (natsel_amd.scenes.pushed_boxes), `steps` steps; loss |com(T) - target|^2; gradients
@@ -305,6 +365,8 @@ def main() -> int:
     ap.add_argument("--large-world-steps", type=int, default=500, help="steps of the large-world measurement (0: skip it)")
     ap.add_argument("--adjoint-worlds", type=int, default=1024, help="worlds of the differentiable rollout (0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=64)
+    ap.add_argument("--defaults-steps", type=int, default=1000,
+                    help="steps of the pile with sleep and continuous collision on (0: skip it)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (reference kernel for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
@@ -409,6 +471,8 @@ def main() -> int:
         out["adjoint"] = adjoint_measurement(args.adjoint_worlds, args.adjoint_steps)
     if args.large_world_steps > 0:
         out["large_world"] = large_world_measurement(args.large_world_steps)
+    if args.defaults_steps > 0:
+        out["reference_defaults"] = defaults_measurement(args.defaults_steps, 0.4 * args.cpu_seconds)
     print(json.dumps(out))
     return 0
 

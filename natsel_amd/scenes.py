@@ -817,3 +817,14 @@ def sleepy_stacks(count: int, seed: int) -> Scene:
     sc.enable_sleep = True
     sc.bodies = [dataclasses.replace(b, enable_sleep=True) for b in sc.bodies]
     return sc
+
+
+def pile_defaults(count: int = 10_000, seed: int = 0x5EED, **kw) -> Scene:
+    """`pile` with the reference's defaults for the world and every body: sleep on and
+    continuous collision on (falling bodies are swept; settled bodies fall asleep)."""
+    sc = pile(count, seed, **kw)
+    sc.name = f"pile_defaults_{count}"
+    sc.enable_sleep = True
+    sc.enable_continuous = True
+    sc.bodies = [dataclasses.replace(b, enable_sleep=True) for b in sc.bodies]
+    return sc

@@ -66,6 +66,7 @@ def lib():
         L.nslo_time_of_impact.argtypes = [fp, I, F, fp, fp, I, F, fp, F, fp, ctypes.POINTER(ctypes.c_int32)]
         L.nslo_body_location.argtypes = [P, I, ctypes.POINTER(ctypes.c_int32)]
         L.nslo_island_summary.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
+        L.nslo_awake_body_count.argtypes = [P]
         L.nslo_island_hash.argtypes = [P, I]
         L.nslo_island_hash.restype = ctypes.c_uint64
         L.nslo_first_island_contact.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
@@ -204,6 +205,9 @@ class OracleWorld:
         out = (ctypes.c_int32 * 3)()
         self.L.nslo_body_location(self.h, body, out)
         return tuple(out)
+
+    def awake_body_count(self) -> int:
+        return int(self.L.nslo_awake_body_count(self.h))
 
     def island_summary(self) -> dict:
         out = (ctypes.c_int64 * 5)()

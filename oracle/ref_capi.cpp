@@ -156,6 +156,9 @@ void nslo_body_location(void* wp, int body, int32_t* out) {
 }
 
 // [island slots, live islands, awake islands, live sleeping sets, pending split island]
+// Bodies in the awake set (world_events.rs:36 awake_body_count).
+int nslo_awake_body_count(void* wp) { return (int)static_cast<World*>(wp)->awake_sims.size(); }
+
 void nslo_island_summary(void* wp, int64_t* out) {
   const World* w = static_cast<World*>(wp);
   int64_t live = 0, sets = 0;

@@ -50,6 +50,11 @@ class SleepTwin:
         assert self.gpu.check_islands() == 0, tag
         assert self.ref.check_islands() == 0, tag
 
+    def _hash_only(self):
+        g, r = self.gpu.world_state_hash(), self.ref.world_state_hash()
+        assert g == r, f"{self.name} step {self.steps}: hash {g:016x} != oracle {r:016x}"
+        assert self.gpu.island_summary() == self.ref.island_summary(), f"{self.name} step {self.steps}"
+
     def step(self, n=1):
         for _ in range(n):
             self.gpu.step(DT, SUB)
@@ -138,3 +143,16 @@ def test_snapshot_with_sleeping_sets():
         t.gpu.step(DT, SUB)
         assert t.gpu.world_state_hash() == hashes[k], k
     assert t.gpu.check_islands() == 0
+
+
+def test_pile_with_reference_defaults():
+    """1000 mixed bodies dropped into a container with the reference's defaults (sleep and
+    continuous collision on): falling bodies are swept, islands form and fall asleep."""
+    t = SleepTwin(scenes.pile_defaults(1000, size_x=10, size_y=20, size_z=10))
+    full = t.compare
+    for k in range(600):
+        t.compare = full if k % 20 == 19 else t._hash_only
+        t.step()
+    t.compare = full
+    s = t.gpu.island_summary()
+    assert s["sleeping_sets"] >= 4 and t.gpu.kernel_stats()["toi_hits"] > 0, s
