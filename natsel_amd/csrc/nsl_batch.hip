@@ -92,6 +92,7 @@ __device__ DeviceWorld world_view(const BatchDevice& b, const BatchWorldDesc& w,
   d.fast_ids = b.fast_ids;
   d.fast_base = w.awake_off;
   d.n_awake = w.n_awake;
+  d.change_list = nullptr;
   d.sleep_times = nullptr;
   d.sleep_sims = d.sims;  // never read: batch worlds do not sleep, so no encoding is -1
   return d;

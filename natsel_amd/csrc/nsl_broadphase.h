@@ -4,8 +4,8 @@
 // does not depend on the tree: every pair of shapes whose fat AABBs overlap, with at least
 // one proxy moved, at least one dynamic body, different bodies, a passing collision filter
 // (shape_runtime.rs:285-292) and no existing contact. Here that set is found with a sort
-// along x and a load-balanced sweep, and returned as sorted pair keys, which is the order
-// the reference creates contacts in.
+// along x and a load-balanced sweep over the x-overlapping pairs that hold a moved proxy,
+// and returned as sorted pair keys, which is the order the reference creates contacts in.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -50,6 +50,10 @@ class PairFinder {
   int32_t* ids_out_ = nullptr;
   int64_t* counts_ = nullptr;
   int64_t* offsets_ = nullptr;
+  int32_t* mv_ = nullptr;         // per sorted position: moved (0/1)
+  int32_t* mprefix_ = nullptr;    // exclusive sum of mv_
+  int32_t* moved_pos_ = nullptr;  // the moved positions in order
+  size_t cap_mv_ = 0, cap_mprefix_ = 0, cap_moved_pos_ = 0;
   size_t cap_keys_in_ = 0, cap_keys_out_ = 0, cap_ids_in_ = 0, cap_ids_out_ = 0, cap_counts_ = 0,
          cap_offsets_ = 0;
   uint64_t* pairs_ = nullptr;        // found keys, unsorted

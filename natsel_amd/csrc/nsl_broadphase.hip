@@ -33,15 +33,29 @@ __global__ void k_bp_keys(int n, const Aabb* fat, uint32_t* keys, int32_t* ids) 
   ids[i] = i;
 }
 
-// counts[p]: shapes after sorted position p whose lower x does not exceed p's upper x
-// (exactly the later shapes that overlap p along x). counts[n] = 0 closes the scan.
-__global__ void k_bp_count(int n, const int32_t* ids, const uint32_t* keys, const Aabb* fat, int64_t* counts) {
+// mv[p]: 1 when the shape at sorted position p is marked moved; mv[n] = 0 closes the scan.
+__global__ void k_bp_moved(int n, const int32_t* ids, const uint8_t* moved, int32_t* mv) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) mv[p] = moved[ids[p]] ? 1 : 0;
+  else if (p == n) mv[n] = 0;
+}
+
+// The later shapes that overlap sorted position p along x are positions p+1 .. end-1, end
+// being the first whose lower x exceeds p's upper x. A pair needs a moved proxy: a moved p
+// takes all of them (counts[p] = end - p - 1); any other p takes only the moved ones, the
+// entries mprefix[p+1] .. mprefix[end]-1 of moved_pos (mprefix: exclusive count of moved
+// positions). Every x-overlapping pair with a moved proxy is counted exactly once, at its
+// lower position. counts[n] = 0 closes the scan.
+__global__ void k_bp_count(int n, const int32_t* ids, const uint32_t* keys, const Aabb* fat, const int32_t* mprefix,
+                           int32_t* moved_pos, int64_t* counts) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > n) return;
   if (p == n) {
     counts[n] = 0;
     return;
   }
+  const bool is_moved = mprefix[p + 1] != mprefix[p];
+  if (is_moved) moved_pos[mprefix[p]] = p;
   const uint32_t hi = order_key(fat[ids[p]].hi.x);
   int lo = p + 1, up = n;  // first q in [p+1, n) with keys[q] > hi
   while (lo < up) {
@@ -49,12 +63,15 @@ __global__ void k_bp_count(int n, const int32_t* ids, const uint32_t* keys, cons
     if (keys[mid] <= hi) lo = mid + 1;
     else up = mid;
   }
-  counts[p] = (int64_t)(lo - p - 1);
+  counts[p] = is_moved ? (int64_t)(lo - p - 1) : (int64_t)(mprefix[lo] - mprefix[p + 1]);
 }
 
-// One candidate (p, q) per index t < offsets[n]; p is the last position with offsets[p] <= t.
-__global__ void k_bp_test(int n, const int64_t* offsets, const int32_t* ids, const Aabb* fat, const uint8_t* moved,
-                          const Proxy* proxies, PairSet set, PairSet blocked, uint64_t* out, int64_t cap, int64_t* found) {
+// One candidate (p, q) per index t < offsets[n]; p is the last position with offsets[p] <= t,
+// q its (t - offsets[p])-th later overlapping position (any one when p moved, else a moved
+// one).
+__global__ void k_bp_test(int n, const int64_t* offsets, const int32_t* ids, const int32_t* mprefix, const int32_t* moved_pos,
+                          const Aabb* fat, const Proxy* proxies, PairSet set, PairSet blocked, uint64_t* out, int64_t cap,
+                          int64_t* found) {
   const int64_t total = offsets[n];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t tests = 0;
@@ -66,10 +83,10 @@ __global__ void k_bp_test(int n, const int64_t* offsets, const int32_t* ids, con
       else hi = mid;
     }
     const int p = lo;
-    const int q = p + 1 + (int)(t - offsets[p]);
+    const int j = (int)(t - offsets[p]);
+    const int q = mprefix[p + 1] != mprefix[p] ? p + 1 + j : moved_pos[mprefix[p + 1] + j];
     const int32_t a = ids[p], b = ids[q];
     ++tests;
-    if (!(moved[a] | moved[b])) continue;
     const Proxy pa = proxies[a], pb = proxies[b];
     if (pa.body_type != kBodyDynamic && pb.body_type != kBodyDynamic) continue;
     if (pa.body == pb.body) continue;
@@ -98,7 +115,8 @@ PairFinder::PairFinder(hipStream_t stream) : stream_(stream) {
 }
 
 PairFinder::~PairFinder() {
-  void* dev[] = {keys_in_, keys_out_, ids_in_, ids_out_, counts_, offsets_, pairs_, pairs_sorted_, found_, temp_};
+  void* dev[] = {keys_in_, keys_out_, ids_in_, ids_out_, counts_, offsets_, pairs_, pairs_sorted_, found_, temp_, mv_,
+                 mprefix_, moved_pos_};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (host_) (void)hipHostFree(host_);
@@ -131,26 +149,34 @@ void PairFinder::find(const DeviceWorld& d, const Proxy* proxies, int n, std::ve
   ensure(ids_out_, cap_ids_out_, n);
   ensure(counts_, cap_counts_, n + 1);
   ensure(offsets_, cap_offsets_, n + 1);
+  ensure(mv_, cap_mv_, n + 1);
+  ensure(mprefix_, cap_mprefix_, n + 1);
+  ensure(moved_pos_, cap_moved_pos_, n);
   if (!pairs_) ensure(pairs_, cap_pairs_, 4096);
 
-  size_t sort_bytes = 0, scan_bytes = 0;
+  size_t sort_bytes = 0, scan_bytes = 0, mscan_bytes = 0;
   BP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_in_, keys_out_, ids_in_, ids_out_, n, 0, 32, stream_));
   BP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts_, offsets_, n + 1, stream_));
-  ensure(temp_, cap_temp_, std::max(sort_bytes, scan_bytes));
+  BP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, mscan_bytes, mv_, mprefix_, n + 1, stream_));
+  ensure(temp_, cap_temp_, std::max(std::max(sort_bytes, scan_bytes), mscan_bytes));
 
   const int nb = (n + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_bp_keys, dim3(nb), dim3(kBlock), 0, stream_, n, d.fat_aabbs, keys_in_, ids_in_);
   sort_bytes = cap_temp_;
   BP_CHECK(hipcub::DeviceRadixSort::SortPairs(temp_, sort_bytes, keys_in_, keys_out_, ids_in_, ids_out_, n, 0, 32, stream_));
+  hipLaunchKernelGGL(k_bp_moved, dim3((n + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n, ids_out_, d.shape_moved,
+                     mv_);
+  mscan_bytes = cap_temp_;
+  BP_CHECK(hipcub::DeviceScan::ExclusiveSum(temp_, mscan_bytes, mv_, mprefix_, n + 1, stream_));
   hipLaunchKernelGGL(k_bp_count, dim3((n + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n, ids_out_, keys_out_,
-                     d.fat_aabbs, counts_);
+                     d.fat_aabbs, mprefix_, moved_pos_, counts_);
   scan_bytes = cap_temp_;
   BP_CHECK(hipcub::DeviceScan::ExclusiveSum(temp_, scan_bytes, counts_, offsets_, n + 1, stream_));
 
   for (int attempt = 0; attempt < 2; ++attempt) {
     BP_CHECK(hipMemsetAsync(found_, 0, 2 * sizeof(int64_t), stream_));
-    hipLaunchKernelGGL(k_bp_test, dim3(kTestBlocks), dim3(kBlock), 0, stream_, n, offsets_, ids_out_, d.fat_aabbs,
-                       d.shape_moved, proxies, d.pairs, d.joint_pairs, pairs_, (int64_t)cap_pairs_, found_);
+    hipLaunchKernelGGL(k_bp_test, dim3(kTestBlocks), dim3(kBlock), 0, stream_, n, offsets_, ids_out_, mprefix_, moved_pos_,
+                       d.fat_aabbs, proxies, d.pairs, d.joint_pairs, pairs_, (int64_t)cap_pairs_, found_);
     BP_CHECK(hipMemcpyAsync(host_, found_, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
     BP_CHECK(hipStreamSynchronize(stream_));
     if (host_[0] <= (int64_t)cap_pairs_) break;

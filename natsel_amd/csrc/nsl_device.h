@@ -32,6 +32,7 @@ struct DeviceWorld {
   int32_t* fast_ids;        // this step's fast bodies as fast_base + awake index (count: counters[kCntFastList])
   int32_t fast_base;        // 0 for a world; the world's offset into a batch's awake bodies
   int32_t n_awake;
+  int32_t* change_list;     // contacts whose state changed in k_collide: id << 2 | ContactChange
   float* sleep_times;       // per awake index: the body's sleep time after finalize (null: not kept)
   BodySim* sleep_sims;      // by body id: sims of sleeping bodies (encoding -1)
 };
@@ -48,6 +49,7 @@ enum : int {
   kCntFastList = 7,     // length of this step's fast_ids list (reset by the step kernel)
   kCntToiCalls = 8,     // time-of-impact queries (cumulative)
   kCntToiHits = 9,      // bodies moved back to an impact (cumulative)
+  kCntChangeList = 10,  // length of k_collide's change_list (reset by the host before it)
   kCounterCount = 16
 };
 

@@ -72,7 +72,7 @@ __global__ void k_collide(DeviceWorld d, int n) {
   if (t >= n) return;
   const int32_t id = d.collide_ids[t];
   const uint8_t change = collide_contact(d, id);
-  d.contact_change[id] = change;
+  if (change != kChangeNone) d.change_list[atomicAdd(&d.counters[kCntChangeList], 1)] = (id << 2) | change;
   if (change == kChangeDisjoint) {
     const Contact& c = d.contacts[id];
     pairset_erase(d.pairs, pair_key(c.shape_a, c.shape_b));  // the host destroys this contact

@@ -214,7 +214,11 @@ class GpuWorld : public HostScene {
   size_t cap_overflow_ids_ = 0, cap_init_ids_ = 0, cap_init_rec_ = 0;
   int32_t* d_init_ids_ = nullptr;
   Contact* d_init_records_ = nullptr;
-  std::vector<uint8_t> change_host_;
+  // contacts whose state changed in the last collide: id << 2 | ContactChange, ascending
+  std::vector<int32_t> changes_;
+  int32_t* changes_pinned_ = nullptr;  // download staging: [0] count, then the list
+  size_t cap_changes_pinned_ = 0, cap_change_list_ = 0, last_changes_ = 0;
+  void ensure_pinned_changes(size_t n);
   std::vector<uint8_t> moved_host_;  // staging for uploads and pulls
   std::vector<Aabb> fat_host_;
   std::vector<Aabb> aabb_host_;

@@ -54,10 +54,11 @@ GpuWorld::~GpuWorld() {
                   d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
                   d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
-                  d_.sleep_sims, d_patch_, d_patch_sims_};
+                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (changes_pinned_) (void)hipHostFree(changes_pinned_);
   for (int i = 0; i < 2 * timer_.capacity; ++i) (void)hipEventDestroy(timer_.events[i]);
   delete[] timer_.events;
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -374,9 +375,10 @@ void GpuWorld::remove_from_graph(int32_t body_a, int32_t body_b, int32_t color, 
 
 // collide_pass.rs:386-478: transitions in contact-id order.
 void GpuWorld::state_pass() {
-  for (size_t id = 0; id < hcontacts_.size(); ++id) {
-    uint8_t ch = change_host_[id];
-    if (ch == kChangeNone || !hcontacts_[id].alive) continue;
+  for (const int32_t e : changes_) {
+    const size_t id = (size_t)(e >> 2);
+    const uint8_t ch = (uint8_t)(e & 3);
+    if (!hcontacts_[id].alive) continue;
     if (ch == kChangeDisjoint) {
       destroy_contact((int32_t)id);
     } else if (ch == kChangeBegan) {
@@ -791,7 +793,6 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
 
   const size_t ncap = hcontacts_.size();
   ensure(d_.contacts, cap_contacts_, std::max<size_t>(ncap, 1), true);
-  ensure(d_.contact_change, cap_change_, std::max<size_t>(ncap, 1), false);
   if (!init_ids_.empty()) {
     ensure_pair_set();
     ensure(d_init_ids_, cap_init_ids_, init_ids_.size(), false);
@@ -813,14 +814,28 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     ids.insert(ids.end(), colors_[ci].overflow.begin(), colors_[ci].overflow.end());
   }
   ids.insert(ids.end(), non_touching_.begin(), non_touching_.end());
-  change_host_.assign(ncap, 0);
+  changes_.clear();
   if (!ids.empty()) {
     ensure(d_.collide_ids, cap_collide_, ids.size(), false);
+    ensure(d_.change_list, cap_change_list_, ids.size(), false);
     NSL_CHECK(hipMemcpyAsync(d_.collide_ids, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
-    NSL_CHECK(hipMemsetAsync(d_.contact_change, 0, ncap, stream_));
+    NSL_CHECK(hipMemsetAsync(d_.counters + kCntChangeList, 0, sizeof(int32_t), stream_));
     launch_collide(d_, (int)ids.size(), stream_, timing_ ? &timer_ : nullptr);
-    NSL_CHECK(hipMemcpyAsync(change_host_.data(), d_.contact_change, ncap, hipMemcpyDeviceToHost, stream_));
+    // the list's length and a first part of it in one download; the rest only if longer
+    const size_t guess = std::min(ids.size(), std::max<size_t>(1024, 2 * last_changes_));
+    ensure_pinned_changes(guess + 1);
+    NSL_CHECK(hipMemcpyAsync(changes_pinned_, d_.counters + kCntChangeList, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+    NSL_CHECK(hipMemcpyAsync(changes_pinned_ + 1, d_.change_list, guess * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
     NSL_CHECK(hipStreamSynchronize(stream_));
+    const size_t m = (size_t)changes_pinned_[0];
+    if (m > guess) {
+      ensure_pinned_changes(m + 1);
+      NSL_CHECK(hipMemcpyAsync(changes_pinned_ + 1, d_.change_list, m * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+      NSL_CHECK(hipStreamSynchronize(stream_));
+    }
+    changes_.assign(changes_pinned_ + 1, changes_pinned_ + 1 + m);
+    std::sort(changes_.begin(), changes_.end());  // contact-id order (id in the high bits)
+    last_changes_ = m;
   }
   const double t2 = now_ms();
   state_pass();
@@ -1004,6 +1019,16 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   profile_.sat_calls = counters_host_[1];
   profile_.sat_hits = counters_host_[2];
   profile_.recycled = counters_host_[3];
+}
+
+void GpuWorld::ensure_pinned_changes(size_t n) {
+  if (n <= cap_changes_pinned_) return;
+  if (changes_pinned_) {
+    NSL_CHECK(hipStreamSynchronize(stream_));
+    NSL_CHECK(hipHostFree(changes_pinned_));
+  }
+  cap_changes_pinned_ = std::max<size_t>(2 * n, 4096);
+  NSL_CHECK(hipHostMalloc((void**)&changes_pinned_, cap_changes_pinned_ * sizeof(int32_t), hipHostMallocDefault));
 }
 
 void GpuWorld::set_kernel_timing(bool on) {
