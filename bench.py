@@ -71,8 +71,9 @@ def step_bytes(counts: dict, substeps: int) -> float:
 
 
 def solve_bytes(ks: dict, substeps: int) -> float:
-    """Algorithmic bytes of the step kernel (prepare .. finalize) over the measured steps."""
-    return sum(stage_bytes(k, ks, substeps) for k in BYTE_MODEL if k != "collide")
+    """Algorithmic bytes of the step kernel (the substep stages and restitution; prepare runs
+    before it and store/finalize after it as launches of their own) over the measured steps."""
+    return sum(stage_bytes(k, ks, substeps) for k in BYTE_MODEL if k not in ("collide", "prepare", "store", "finalize"))
 
 
 TRAFFIC_PROFILE = os.path.join("profiles", "k_solve_step_pile10k_pmc_bytes.md")
@@ -196,7 +197,7 @@ def large_world_measurement(steps: int) -> dict:
     elapsed, _ = runs[0]
     ks = runs[1][1]
     stage_ms = {
-        "prepare": ks["phase_ms"]["prepare"],
+        "prepare": ks["ms"]["prepare"],
         "integrate_velocities": ks["phase_ms"]["integrate_velocities"],
         "warm_start": ks["phase_ms"]["warm_start"],
         "push": ks["phase_ms"]["push"],
@@ -209,7 +210,7 @@ def large_world_measurement(steps: int) -> dict:
         b = stage_bytes(st, ks, sc.substeps)
         per_stage[st] = {"ms_per_step": ms / steps, "gb_per_step": b / steps / 1e9, "gbs": b / (ms / 1e3) / 1e9 if ms > 0 else None}
     sf_bytes = stage_bytes("store", ks, sc.substeps) + stage_bytes("finalize", ks, sc.substeps)
-    sf_ms = ks["phase_ms"]["store_finalize"]
+    sf_ms = ks["ms"]["finalize"]
     per_stage["store_finalize"] = {"ms_per_step": sf_ms / steps, "gb_per_step": sf_bytes / steps / 1e9,
                                    "gbs": sf_bytes / (sf_ms / 1e3) / 1e9 if sf_ms > 0 else None}
     return {

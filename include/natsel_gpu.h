@@ -96,11 +96,14 @@ typedef struct nsl_profile {
   int32_t fast_bodies, sat_calls, sat_hits, recycled;
 } nsl_profile;
 
-/* Launch classes of nsl_kernel_stats.ms / launches (HIP events around each launch). */
-enum { NSL_K_INIT = 0, NSL_K_COLLIDE, NSL_K_SOLVE, NSL_K_CONTINUOUS, NSL_K_CLASS_COUNT };
+/* Launch classes of nsl_kernel_stats.ms / launches (HIP events around each launch):
+   NSL_K_SOLVE is the step kernel, NSL_K_PREPARE the constraint preparation before it and
+   NSL_K_FINALIZE the impulse store and body finalize after it. */
+enum { NSL_K_INIT = 0, NSL_K_COLLIDE, NSL_K_SOLVE, NSL_K_CONTINUOUS, NSL_K_PREPARE, NSL_K_FINALIZE, NSL_K_CLASS_COUNT };
 
 /* Phases of the step kernel (NSL_K_SOLVE), timed with the device wall clock at its grid
- * barriers; each phase's time includes the barrier that ends it. */
+ * barriers; each phase's time includes the barrier that ends it. NSL_P_PREPARE and
+ * NSL_P_STORE_FINALIZE stay 0 (those are the NSL_K_PREPARE / NSL_K_FINALIZE launches). */
 enum {
   NSL_P_PREPARE = 0, NSL_P_INTEGRATE_VELOCITIES, NSL_P_WARM_START, NSL_P_PUSH, NSL_P_INTEGRATE_POSITIONS,
   NSL_P_RELAX, NSL_P_RESTITUTION, NSL_P_OVERFLOW, NSL_P_STORE_FINALIZE, NSL_P_CLASS_COUNT

@@ -85,7 +85,7 @@ class Profile(ctypes.Structure):
     ]
 
 
-KERNEL_CLASSES = ["init", "collide", "solve", "continuous"]
+KERNEL_CLASSES = ["init", "collide", "solve", "continuous", "prepare", "finalize"]
 PHASE_CLASSES = ["prepare", "integrate_velocities", "warm_start", "push", "integrate_positions", "relax", "restitution",
                  "overflow", "store_finalize"]
 HOST_PHASES = ["pairs", "collide", "state_pass", "solve_finalize"]

@@ -54,7 +54,7 @@ enum : int {
 };
 
 // Launches timed with HIP events when kernel timing is on.
-enum KernelClass : int { kKInit = 0, kKCollide, kKSolve, kKContinuous, kKernelClassCount };
+enum KernelClass : int { kKInit = 0, kKCollide, kKSolve, kKContinuous, kKPrepare, kKFinalize, kKernelClassCount };
 
 // Phases inside the step kernel; timed with the device wall clock at each grid barrier.
 enum PhaseClass : int {
@@ -131,10 +131,15 @@ void launch_init_contacts(const DeviceWorld& d, const int32_t* ids, const Contac
 void launch_collide(const DeviceWorld& d, int n, hipStream_t s, KernelTimer* timer);
 void launch_pairset_insert(PairSet set, const uint64_t* keys, int n, hipStream_t s);
 void launch_pairset_erase(PairSet set, const uint64_t* keys, int n, hipStream_t s);
-// One launch (all workgroups resident): prepare, the substep schedule, restitution, store
-// and finalize.
+// Prepare of joints, contact and overflow constraints, one lane per item (before the step kernel).
+void launch_prepare(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, hipStream_t s, KernelTimer* timer);
+// One launch (all workgroups resident): the substep schedule and restitution.
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
                        KernelTimer* timer);
+// Store of contact and overflow impulses and finalize of the bodies, one lane per item (after
+// the step kernel; finalize lists the fast bodies for the continuous pass).
+void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, hipStream_t s,
+                           KernelTimer* timer);
 // Workgroups of the step kernel that fit on the device at once.
 int solve_step_max_blocks(int device);
 constexpr int kSolveBlock = 128;  // measured: 128 lanes beat 64, 256 and 512 on the 10k pile

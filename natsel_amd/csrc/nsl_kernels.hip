@@ -1,9 +1,11 @@
 // gfx950 kernels of the single-world rigid-body step: contact initialisation, the
-// narrow phase (collide) and the step kernel (preparation, the coloured soft-step
-// stages, the sequential overflow colour, integration, impulse storage, finalisation).
+// narrow phase (collide), preparation, the step kernel (the coloured soft-step stages,
+// the overflow colour's levels, integration, restitution) and impulse storage with
+// finalisation.
 // The per-item functions, with their reference citations, are in nsl_stages.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -262,11 +264,11 @@ __device__ void run_overflow_levels(const SolvePlan& plan, JointItem joint, Item
   }
 }
 
-// Stage schedule of one step (solver_stages.rs:543-589, solver_integration.rs:807-1015):
-// prepare; per substep: integrate velocities, warm start, push, integrate positions,
-// relax, each stage running the overflow list first and then colours 0..22 in order;
-// restitution iterations; store impulses; finalize. A grid barrier separates dependent
-// phases; the overflow list is walked by one lane.
+// Stage schedule of one step (solver_stages.rs:543-589, solver_integration.rs:807-1015)
+// between k_prepare and k_store_finalize: per substep, integrate velocities, warm start,
+// push, integrate positions, relax, each stage running the overflow colour's levels first
+// and then colours 0..22 in order; then the restitution iterations. A grid barrier
+// separates dependent phases.
 __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, SolvePlan plan, StepParams p) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
@@ -292,18 +294,6 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   __syncthreads();
   const bool any_ovf = plan.n_ovf_levels > 0;
   auto no_joint = [](int32_t) {};
-
-  {
-    int points = 0, ovf_points = 0;
-    if (tid == 0) d.counters[kCntFastList] = 0;  // finalize lists this step's fast bodies
-    for (int k = tid; k < plan.n_joints; k += nthreads) prepare_joint_item(d, d.joint_ids[k], p);
-    for (int slot = tid; slot < plan.n_constraints; slot += nthreads) points += prepare_item(d, slot, p);
-    for (int k = tid; k < n_ovf; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);
-    block_add(&d.counters[4], points);
-    block_add(&d.counters[5], ovf_points);
-  }
-  if (!grid_sync(bar)) return;
-  clock.mark(kPPrepare);
 
   for (int sub = 0; sub < p.sub_steps; ++sub) {
     for (int i = tid; i < d.n_awake; i += nthreads) integrate_velocities_item(d, i, p);
@@ -360,12 +350,31 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     }
   }
 
-  // impulses go to the contacts and bodies are finalised: disjoint data, one phase
-  for (int k = tid; k < n_ovf; k += nthreads) overflow_store_item(d, k);
+}
+
+// Prepare (contact_solver.rs:395-662, overflow_contact_solver.rs, revolute_joint.rs:176-261):
+// every joint, contact constraint and overflow constraint is independent, so this runs at the
+// width of the work, before the step kernel. It also restarts this step's fast-body list.
+__global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan plan, StepParams p) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  int points = 0, ovf_points = 0;
+  if (tid == 0) d.counters[kCntFastList] = 0;  // finalize lists this step's fast bodies
+  for (int k = tid; k < plan.n_joints; k += nthreads) prepare_joint_item(d, d.joint_ids[k], p);
+  for (int slot = tid; slot < plan.n_constraints; slot += nthreads) points += prepare_item(d, slot, p);
+  for (int k = tid; k < plan.n_overflow; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);
+  block_add(&d.counters[kCntPoints], points);
+  block_add(&d.counters[kCntOverflowPoints], ovf_points);
+}
+
+// Store impulses to the contacts (contact_solver.rs:1258-1299) and finalize the bodies
+// (solver_integration.rs:587-799): disjoint data, after the step kernel, at full width.
+__global__ __launch_bounds__(kBlock) void k_store_finalize(DeviceWorld d, SolvePlan plan, StepParams p) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  for (int k = tid; k < plan.n_overflow; k += nthreads) overflow_store_item(d, k);
   for (int slot = tid; slot < plan.n_constraints; slot += nthreads) store_item(d, slot);
   for (int i = tid; i < d.n_awake; i += nthreads) finalize_item(d, i, p);
-  if (!grid_sync(bar)) return;
-  clock.mark(kPStoreFinalize);
 }
 
 inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
@@ -411,6 +420,16 @@ int solve_step_max_blocks(int device) {
     return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
   return per_cu * cus;
+}
+
+void launch_prepare(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, hipStream_t s, KernelTimer* t) {
+  const int n = std::max(std::max(plan.n_joints, plan.n_constraints), std::max(plan.n_overflow, 1));
+  NSL_TIMED(t, kKPrepare, s, hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, plan, p));
+}
+
+void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, hipStream_t s, KernelTimer* t) {
+  const int n = std::max(std::max(plan.n_constraints, d.n_awake), std::max(plan.n_overflow, 1));
+  NSL_TIMED(t, kKFinalize, s, hipLaunchKernelGGL(k_store_finalize, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, plan, p));
 }
 
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
@@ -214,6 +215,24 @@ class GpuWorld : public HostScene {
   size_t cap_overflow_ids_ = 0, cap_init_ids_ = 0, cap_init_rec_ = 0;
   int32_t* d_init_ids_ = nullptr;
   Contact* d_init_records_ = nullptr;
+  // A growable int list in pinned host memory: the staging of a per-step upload.
+  struct PinnedInts {
+    int32_t* ptr = nullptr;
+    size_t n = 0, cap = 0;
+    ~PinnedInts() { if (ptr) (void)hipHostFree(ptr); }
+    void clear() { n = 0; }
+    bool empty() const { return n == 0; }
+    size_t size() const { return n; }
+    const int32_t* data() const { return ptr; }
+    void append(const std::vector<int32_t>& v) {
+      if (n + v.size() > cap) grow(n + v.size());
+      if (!v.empty()) std::memcpy(ptr + n, v.data(), v.size() * sizeof(int32_t));
+      n += v.size();
+    }
+    void grow(size_t want);  // keeps the contents; the caller's stream has no copy from it in flight
+  };
+  PinnedInts collide_stage_, solve_stage_;
+  int32_t graph_restitution_ = 0;  // graph contacts with restitution > 0
   // contacts whose state changed in the last collide: id << 2 | ContactChange, ascending
   std::vector<int32_t> changes_;
   int32_t* changes_pinned_ = nullptr;  // download staging: [0] count, then the list

@@ -353,6 +353,7 @@ void GpuWorld::add_to_graph(int32_t id) {
     }
   }
   c.color = color;
+  graph_restitution_ += c.restitution > 0.0f;
   if (color == kOverflowColor) {
     c.local = (int32_t)colors_[color].overflow.size();
     colors_[color].overflow.push_back(id);
@@ -364,6 +365,8 @@ void GpuWorld::add_to_graph(int32_t id) {
 
 // constraint_graph.rs:200-251
 void GpuWorld::remove_from_graph(int32_t body_a, int32_t body_b, int32_t color, int32_t local) {
+  const int32_t id = color == kOverflowColor ? colors_[color].overflow[(size_t)local] : colors_[color].convex[(size_t)local];
+  graph_restitution_ -= hcontacts_[(size_t)id].restitution > 0.0f;
   if (color != kOverflowColor) {
     colors_[color].clear(body_a);
     colors_[color].clear(body_b);
@@ -732,6 +735,11 @@ void GpuWorld::restore_state(const Snapshot& s) {
   non_touching_ = s.non_touching;
   contact_pool_next_ = s.contact_pool_next;
   for (int c = 0; c < kColorCount; ++c) colors_[c] = s.colors[c];
+  graph_restitution_ = 0;
+  for (int c = 0; c < kColorCount; ++c) {
+    for (int32_t id : colors_[c].convex) graph_restitution_ += hcontacts_[(size_t)id].restitution > 0.0f;
+    for (int32_t id : colors_[c].overflow) graph_restitution_ += hcontacts_[(size_t)id].restitution > 0.0f;
+  }
   std::memcpy(counters_host_, s.counters_host, sizeof(counters_host_));
   std::memcpy(counters_base_, s.counters_base, sizeof(counters_base_));
   pair_set_tombs_ = s.pair_set_tombs;
@@ -806,14 +814,15 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   flush_locations();  // bodies put to sleep by the last step, bodies woken by joints
   if (dt <= 0.0f) return;
 
-  // collide list: colours' contacts then the non-touching contacts (collide_pass.rs:175-190)
-  std::vector<int32_t> ids;
-  ids.reserve(ncap);
+  // collide list: colours' contacts then the non-touching contacts (collide_pass.rs:175-190),
+  // staged in pinned memory (the previous step's copy from it has completed)
+  PinnedInts& ids = collide_stage_;
+  ids.clear();
   for (int ci = 0; ci < kColorCount; ++ci) {
-    ids.insert(ids.end(), colors_[ci].convex.begin(), colors_[ci].convex.end());
-    ids.insert(ids.end(), colors_[ci].overflow.begin(), colors_[ci].overflow.end());
+    ids.append(colors_[ci].convex);
+    ids.append(colors_[ci].overflow);
   }
-  ids.insert(ids.end(), non_touching_.begin(), non_touching_.end());
+  ids.append(non_touching_);
   changes_.clear();
   if (!ids.empty()) {
     ensure(d_.collide_ids, cap_collide_, ids.size(), false);
@@ -853,22 +862,19 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     sleep_times_host_.resize((size_t)d_.n_awake);
   }
 
-  // solve lists: colour-major slots (solver_prepare.rs:335-474)
-  std::vector<int32_t> solve;
+  // solve lists: colour-major slots (solver_prepare.rs:335-474), staged in pinned memory
+  PinnedInts& solve = solve_stage_;
+  solve.clear();
   ColorRanges cr{};
-  bool any_restitution = false;
+  const bool any_restitution = graph_restitution_ > 0;
   int colors_used = 0;
   for (int ci = 0; ci < kOverflowColor; ++ci) {
     cr.begin[ci] = (int32_t)solve.size();
-    for (int32_t id : colors_[ci].convex) {
-      solve.push_back(id);
-      any_restitution = any_restitution || hcontacts_[id].restitution > 0.0f;
-    }
+    solve.append(colors_[ci].convex);
     cr.end[ci] = (int32_t)solve.size();
     colors_used += cr.end[ci] > cr.begin[ci];
   }
   const std::vector<int32_t>& ovf = colors_[kOverflowColor].overflow;
-  for (int32_t id : ovf) any_restitution = any_restitution || hcontacts_[id].restitution > 0.0f;
   cr.begin[kOverflowColor] = cr.end[kOverflowColor] = (int32_t)solve.size();
   if (!solve.empty()) {
     ensure(d_.solve_ids, cap_solve_, solve.size(), false);
@@ -920,7 +926,9 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   // taken after every buffer above is in place: the kernels get the pointers of this step
   DeviceWorld dw = d_;
   if (!options.enable_sleep) dw.sleep_times = nullptr;
+  launch_prepare(dw, plan, p, stream_, tm);
   launch_solve_step(dw, plan, p, blocks, stream_, tm);
+  launch_store_finalize(dw, plan, p, stream_, tm);
   // continuous collision of the bodies finalize flagged fast, then of the bullets
   if (options.enable_continuous) {
     launch_continuous(dw, ccd_, p, false, d_.n_awake, stream_, tm);
@@ -1002,7 +1010,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     }
     phases += run >= 2 ? 1 : run;
     const int ovf_phase = plan.n_ovf_levels > 0 ? 1 : 0;
-    int64_t barriers = 2 + (int64_t)p.sub_steps * (2 + 3 * (phases + ovf_phase));
+    int64_t barriers = (int64_t)p.sub_steps * (2 + 3 * (phases + ovf_phase));
     if (any_restitution) barriers += (int64_t)options.restitution_iterations * (phases + (ovf.empty() ? 0 : 1));
     kstats_.barriers += barriers;
   }
@@ -1019,6 +1027,16 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   profile_.sat_calls = counters_host_[1];
   profile_.sat_hits = counters_host_[2];
   profile_.recycled = counters_host_[3];
+}
+
+void GpuWorld::PinnedInts::grow(size_t want) {
+  const size_t c = std::max<size_t>(2 * want, 4096);
+  int32_t* p = nullptr;
+  NSL_CHECK(hipHostMalloc((void**)&p, c * sizeof(int32_t), hipHostMallocDefault));
+  if (n) std::memcpy(p, ptr, n * sizeof(int32_t));
+  if (ptr) NSL_CHECK(hipHostFree(ptr));
+  ptr = p;
+  cap = c;
 }
 
 void GpuWorld::ensure_pinned_changes(size_t n) {
