@@ -81,6 +81,7 @@ __device__ DeviceWorld world_view(const BatchDevice& b, const BatchWorldDesc& w,
   d.constraints = b.constraints + w.contact_off;
   d.overflow = b.overflow + w.overflow_off;
   d.collide_ids = nullptr;
+  d.contact_active = nullptr;
   d.solve_ids = b.solve_ids + w.contact_off;
   d.overflow_ids = b.overflow_ids + w.overflow_off;
   d.counters = b.counters;

@@ -20,7 +20,8 @@ struct DeviceWorld {
   uint8_t* contact_change;  // by contact id: ContactChange of this step
   Constraint* constraints;  // coloured slots, colour-major
   OverflowConstraint* overflow;
-  int32_t* collide_ids;     // contacts to collide this step
+  int32_t* collide_ids;     // unused by the single world (its contacts carry contact_active)
+  uint8_t* contact_active;  // by contact id: 1 for a contact of the awake set (collided each step)
   int32_t* solve_ids;       // coloured contact ids, colour-major (slot order)
   int32_t* overflow_ids;    // overflow contact ids in graph order
   int32_t* counters;        // kCounterCount work counters, see kCnt*
@@ -120,8 +121,9 @@ struct SolvePlan {
 constexpr int kMaxPhaseLog = 1024;
 
 // Location patches after bodies changed solver sets (nsl_kernels.hip): `items` holds
-// n_shapes (shape id, encoding) pairs, then n_contacts (contact id, enc_a, enc_b) and n_joints
-// (joint id, enc_a, enc_b) triples; `sims` holds n_sims sims copied to sleep_sims[sim.body_id].
+// n_shapes (shape id, encoding) pairs, then n_contacts (contact id, enc_a, enc_b, awake-set
+// flag) quads and n_joints (joint id, enc_a, enc_b) triples; `sims` holds n_sims sims copied
+// to sleep_sims[sim.body_id].
 void launch_patch_locations(const DeviceWorld& d, const int32_t* items, int n_shapes, int n_contacts, int n_joints,
                             const BodySim* sims, int n_sims, hipStream_t s);
 

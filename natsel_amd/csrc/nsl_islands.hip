@@ -438,12 +438,36 @@ void GpuWorld::wake_solver_set(int32_t set) {
   bodies_dirty_ = true;
 }
 
+void GpuWorld::mark_contact_dirty(int32_t id) {
+  dirty_contacts_.push_back(id);
+  locations_dirty_ = true;
+}
+
 void GpuWorld::flush_locations() {
   if (bodies_dirty_ || shapes_dirty_) upload_bodies();
   if (!locations_dirty_) return;
   std::vector<int32_t> shape_items, contact_items, joint_items;
   if (contact_mark_.size() < hcontacts_.size()) contact_mark_.resize(hcontacts_.size(), 0);
   std::vector<int32_t> marked;
+  // a contact's encodings and awake-set flag, from its current record
+  auto patch_contact = [&](int32_t cid) {
+    if (contact_mark_[(size_t)cid]) return;
+    contact_mark_[(size_t)cid] = 1;
+    marked.push_back(cid);
+    const HContact& c = hcontacts_[(size_t)cid];
+    contact_items.push_back(cid);
+    if (c.alive) {
+      contact_items.push_back(encode(bodies_[(size_t)c.body[0]]));
+      contact_items.push_back(encode(bodies_[(size_t)c.body[1]]));
+      contact_items.push_back(c.set == kAwakeSet ? 1 : 0);
+    } else {
+      contact_items.push_back(kNullIndex);
+      contact_items.push_back(kNullIndex);
+      contact_items.push_back(0);
+    }
+  };
+  for (int32_t cid : dirty_contacts_) patch_contact(cid);
+  dirty_contacts_.clear();
   for (int32_t b : moved_bodies_) {
     const HBody& body = bodies_[(size_t)b];
     const int32_t enc = encode(body);
@@ -454,14 +478,8 @@ void GpuWorld::flush_locations() {
     }
     for (int32_t key = body.head_contact; key != kNullIndex;) {
       const int32_t cid = key >> 1;
-      const HContact& c = hcontacts_[(size_t)cid];
-      key = c.next[key & 1];
-      if (c.set != kAwakeSet || contact_mark_[(size_t)cid]) continue;
-      contact_mark_[(size_t)cid] = 1;
-      marked.push_back(cid);
-      contact_items.push_back(cid);
-      contact_items.push_back(encode(bodies_[(size_t)c.body[0]]));
-      contact_items.push_back(encode(bodies_[(size_t)c.body[1]]));
+      key = hcontacts_[(size_t)cid].next[key & 1];
+      patch_contact(cid);  // every set: a contact leaving or joining the awake set changes its flag
     }
     body_moved_[(size_t)b] = 0;
   }
@@ -499,7 +517,7 @@ void GpuWorld::flush_locations() {
     ensure(d_patch_, cap_patch_, items.size(), false);
     NSL_CHECK(hipMemcpyAsync(d_patch_, items.data(), items.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
   }
-  launch_patch_locations(d_, d_patch_, (int)(shape_items.size() / 2), (int)(contact_items.size() / 3),
+  launch_patch_locations(d_, d_patch_, (int)(shape_items.size() / 2), (int)(contact_items.size() / 4),
                          (int)(joint_items.size() / 3), d_patch_sims_, (int)sims.size(), stream_);
   NSL_CHECK(hipStreamSynchronize(stream_));  // the staging vectors are read by the copies above
 }

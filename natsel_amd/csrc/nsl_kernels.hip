@@ -28,10 +28,12 @@ __global__ void k_init_contacts(DeviceWorld d, const int32_t* ids, const Contact
   if (t >= n) return;
   const Contact& c = records[t];
   d.contacts[ids[t]] = c;
+  d.contact_active[ids[t]] = 1;  // a contact made in the disabled set is switched off by the host's next patch
   pairset_insert(d.pairs, pair_key(c.shape_a, c.shape_b));
 }
 
-// One item per thread: shape encodings, contact encodings, joint encodings, sleeping sims.
+// One item per thread: shape encodings, contact encodings with the awake-set flag, joint
+// encodings, sleeping sims.
 __global__ void k_patch_locations(DeviceWorld d, const int32_t* items, int n_shapes, int n_contacts, int n_joints,
                                   const BodySim* sims, int n_sims) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,15 +42,17 @@ __global__ void k_patch_locations(DeviceWorld d, const int32_t* items, int n_sha
     return;
   }
   t -= n_shapes;
-  const int32_t* tri = items + 2 * n_shapes;
+  const int32_t* quad = items + 2 * n_shapes;
   if (t < n_contacts) {
-    Contact& c = d.contacts[tri[3 * t]];
-    c.enc_a = tri[3 * t + 1];
-    c.enc_b = tri[3 * t + 2];
+    const int32_t id = quad[4 * t];
+    Contact& c = d.contacts[id];
+    c.enc_a = quad[4 * t + 1];
+    c.enc_b = quad[4 * t + 2];
+    d.contact_active[id] = (uint8_t)quad[4 * t + 3];
     return;
   }
   t -= n_contacts;
-  tri += 3 * n_contacts;
+  const int32_t* tri = quad + 4 * n_contacts;
   if (t < n_joints) {
     JointSim& j = d.joints[tri[3 * t]];
     j.enc_a = tri[3 * t + 1];
@@ -69,13 +73,16 @@ __global__ void k_pairset_erase(PairSet set, const uint64_t* keys, int n) {
   if (t < n) pairset_erase(set, keys[t]);
 }
 
+// Every contact id below n whose awake-set flag is on: the awake set's contacts, touching or
+// not (collide_pass.rs:175-190; each is independent, so the order of the reference's list
+// does not matter). A disjoint contact turns its own flag off: the host destroys it.
 __global__ void k_collide(DeviceWorld d, int n) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int32_t id = d.collide_ids[t];
+  const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n || !d.contact_active[id]) return;
   const uint8_t change = collide_contact(d, id);
   if (change != kChangeNone) d.change_list[atomicAdd(&d.counters[kCntChangeList], 1)] = (id << 2) | change;
   if (change == kChangeDisjoint) {
+    d.contact_active[id] = 0;
     const Contact& c = d.contacts[id];
     pairset_erase(d.pairs, pair_key(c.shape_a, c.shape_b));  // the host destroys this contact
   }

@@ -154,7 +154,8 @@ class GpuWorld : public HostScene {
   void update_pairs();
   void ensure_pair_set();
   void create_contact(int32_t a, int32_t b);
-  void destroy_contact(int32_t id);
+  void destroy_contact(int32_t id, bool device_off = false);
+  void mark_contact_dirty(int32_t id);
   void add_to_graph(int32_t id);
   void remove_from_graph(int32_t body_a, int32_t body_b, int32_t color, int32_t local);
   void state_pass();
@@ -231,7 +232,7 @@ class GpuWorld : public HostScene {
     }
     void grow(size_t want);  // keeps the contents; the caller's stream has no copy from it in flight
   };
-  PinnedInts collide_stage_, solve_stage_;
+  PinnedInts solve_stage_;
   int32_t graph_restitution_ = 0;  // graph contacts with restitution > 0
   // contacts whose state changed in the last collide: id << 2 | ContactChange, ascending
   std::vector<int32_t> changes_;
@@ -278,6 +279,8 @@ class GpuWorld : public HostScene {
   std::vector<uint8_t> body_moved_;
   std::vector<int32_t> moved_bodies_;
   std::vector<int32_t> slept_bodies_;  // sims to copy to the device's sleeping sims
+  std::vector<int32_t> dirty_contacts_;  // contacts whose device flag or encodings need a patch
+  size_t cap_active_ = 0;
   bool locations_dirty_ = false;
   std::vector<uint8_t> contact_mark_;
   int32_t* d_patch_ = nullptr;

@@ -54,7 +54,7 @@ GpuWorld::~GpuWorld() {
                   d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
                   d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
-                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list};
+                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -275,6 +275,7 @@ void GpuWorld::create_contact(int32_t a, int32_t b) {
   hc.body[0] = sa.body;
   hc.body[1] = sb.body;
   link_body_contact(id);
+  if (hc.set != 2) mark_contact_dirty(id);  // the device turns new contacts on
   Contact c{};
   c.shape_a = a;
   c.shape_b = b;
@@ -304,10 +305,12 @@ static int32_t swap_remove(std::vector<T>& v, size_t index) {
   return kNullIndex;
 }
 
-// contact_creation.rs:428-552
-void GpuWorld::destroy_contact(int32_t id) {
+// contact_creation.rs:428-552. `device_off`: the collide kernel already turned the
+// contact's flag off (it found it disjoint).
+void GpuWorld::destroy_contact(int32_t id, bool device_off) {
   HContact& c = hcontacts_[id];
   if (!c.alive) return;
+  if (!device_off) mark_contact_dirty(id);
   pair_set_tombs_ += 1;  // the collide kernel erased the key
   unlink_body_contact(id);
   if (c.island != kNullIndex) unlink_contact(id);  // contact_creation.rs:501-508
@@ -383,7 +386,7 @@ void GpuWorld::state_pass() {
     const uint8_t ch = (uint8_t)(e & 3);
     if (!hcontacts_[id].alive) continue;
     if (ch == kChangeDisjoint) {
-      destroy_contact((int32_t)id);
+      destroy_contact((int32_t)id, true);
     } else if (ch == kChangeBegan) {
       // linking first wakes a sleeping partner (island.rs:126-171)
       link_contact((int32_t)id);
@@ -760,6 +763,16 @@ void GpuWorld::restore_state(const Snapshot& s) {
     if (b.body_type == kBodyDynamic && (b.flags & bflag::kIsBullet)) bullet_bodies_ += 1;
   device_newer_ = false;
   joints_dirty_ = true;  // the colour lists of joints are rebuilt from colors_
+  // the awake-set flags of every contact id, from the restored records
+  {
+    std::vector<uint8_t> active(hcontacts_.size(), 0);
+    for (size_t i = 0; i < hcontacts_.size(); ++i) active[i] = hcontacts_[i].alive && hcontacts_[i].set == 2;
+    ensure(d_.contact_active, cap_active_, std::max<size_t>(active.size(), 1), false);
+    if (!active.empty())
+      NSL_CHECK(hipMemcpyAsync(d_.contact_active, active.data(), active.size(), hipMemcpyHostToDevice, stream_));
+    NSL_CHECK(hipStreamSynchronize(stream_));
+  }
+  dirty_contacts_.clear();
   // shapes hold their bodies' encodings: bring them to the restored locations
   moved_bodies_.clear();
   slept_bodies_.clear();
@@ -801,6 +814,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
 
   const size_t ncap = hcontacts_.size();
   ensure(d_.contacts, cap_contacts_, std::max<size_t>(ncap, 1), true);
+  ensure(d_.contact_active, cap_active_, std::max<size_t>(ncap, 1), true);
   if (!init_ids_.empty()) {
     ensure_pair_set();
     ensure(d_init_ids_, cap_init_ids_, init_ids_.size(), false);
@@ -814,24 +828,17 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   flush_locations();  // bodies put to sleep by the last step, bodies woken by joints
   if (dt <= 0.0f) return;
 
-  // collide list: colours' contacts then the non-touching contacts (collide_pass.rs:175-190),
-  // staged in pinned memory (the previous step's copy from it has completed)
-  PinnedInts& ids = collide_stage_;
-  ids.clear();
-  for (int ci = 0; ci < kColorCount; ++ci) {
-    ids.append(colors_[ci].convex);
-    ids.append(colors_[ci].overflow);
-  }
-  ids.append(non_touching_);
+  // collide: every contact of the awake set, graph and non-touching (collide_pass.rs:175-190),
+  // found on the device by its flag
+  size_t awake_contacts = non_touching_.size();
+  for (int ci = 0; ci < kColorCount; ++ci) awake_contacts += colors_[ci].convex.size() + colors_[ci].overflow.size();
   changes_.clear();
-  if (!ids.empty()) {
-    ensure(d_.collide_ids, cap_collide_, ids.size(), false);
-    ensure(d_.change_list, cap_change_list_, ids.size(), false);
-    NSL_CHECK(hipMemcpyAsync(d_.collide_ids, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+  if (awake_contacts > 0) {
+    ensure(d_.change_list, cap_change_list_, ncap, false);  // at most one entry per contact id
     NSL_CHECK(hipMemsetAsync(d_.counters + kCntChangeList, 0, sizeof(int32_t), stream_));
-    launch_collide(d_, (int)ids.size(), stream_, timing_ ? &timer_ : nullptr);
+    launch_collide(d_, (int)ncap, stream_, timing_ ? &timer_ : nullptr);
     // the list's length and a first part of it in one download; the rest only if longer
-    const size_t guess = std::min(ids.size(), std::max<size_t>(1024, 2 * last_changes_));
+    const size_t guess = std::min(awake_contacts, std::max<size_t>(1024, 2 * last_changes_));
     ensure_pinned_changes(guess + 1);
     NSL_CHECK(hipMemcpyAsync(changes_pinned_, d_.counters + kCntChangeList, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
     NSL_CHECK(hipMemcpyAsync(changes_pinned_ + 1, d_.change_list, guess * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
@@ -977,7 +984,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   }
   kstats_.steps += 1;
   kstats_.bodies += (int64_t)d_.n_awake;
-  kstats_.collided += (int64_t)ids.size();
+  kstats_.collided += (int64_t)awake_contacts;
   kstats_.constraints += (int64_t)solve.size();
   kstats_.joints += (int64_t)joint_ids_host_.size();
   kstats_.fast_bodies = counters_host_[0] - counters_base_[0];
