@@ -227,7 +227,10 @@ int nsl_world_island_summary(nsl_world* world, int64_t* out);
    validate_graph_indices / validate_set_contacts / validate_islands, constraint_graph.rs:370-560). */
 int nsl_world_check_islands(nsl_world* world, int32_t* out);
 int nsl_world_get_profile(nsl_world* world, nsl_profile* out);
-/* Timing adds an event pair per launch and a wall-clock read per phase; results do not change. */
+/* on: bit 0 times every launch with an event pair (nsl_kernel_stats.ms / launches); bit 1
+   (with bit 0) also runs the step kernel's phase clock, a wall-clock read per grid barrier
+   and a download of the marks after each step (phase_ms / phase_count). Results do not
+   change. */
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on);
 int nsl_world_reset_kernel_stats(nsl_world* world);
 int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out);

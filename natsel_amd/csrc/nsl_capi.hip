@@ -384,7 +384,7 @@ int nsl_world_get_profile(nsl_world* world, nsl_profile* out) {
 
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on) {
   if (!world) return fail(NSL_ERR_INVALID, "null world");
-  return guarded([&] { world->impl->set_kernel_timing(on != 0); });
+  return guarded([&] { world->impl->set_kernel_timing((on & 1) != 0, (on & 2) != 0); });
 }
 
 int nsl_world_reset_kernel_stats(nsl_world* world) {

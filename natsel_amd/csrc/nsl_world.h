@@ -79,7 +79,9 @@ class GpuWorld : public HostScene {
     double host_ms[4];  // host wall time: pair update, collide (with its wait), state pass, solve+finalize (with its wait)
     int64_t joints;
   };
-  void set_kernel_timing(bool on);
+  // Launch events when `on`; the step kernel's phase clock (a device wall-clock read at
+  // each grid barrier and a download of the marks per step) also when `phases`.
+  void set_kernel_timing(bool on, bool phases = true);
   const KernelStats& kernel_stats() const { return kstats_; }
   void reset_kernel_stats();
 
@@ -188,7 +190,12 @@ class GpuWorld : public HostScene {
   int device_;
   hipStream_t stream_ = nullptr;
   KernelTimer timer_{};
-  bool timing_ = false;
+  bool timing_ = false, phase_timing_ = false;
+  struct StepReadback {  // device layout: the work counters, then the grid barrier
+    int32_t counters[kCounterCount];
+    GridBarrier barrier;
+  };
+  StepReadback* readback_ = nullptr;  // pinned
   GridBarrier* d_barrier_ = nullptr;
   uint64_t* d_phase_log_ = nullptr;
   std::vector<uint64_t> phase_log_host_;

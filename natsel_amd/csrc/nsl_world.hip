@@ -29,10 +29,12 @@ GpuWorld::GpuWorld(int device) : device_(device) {
   timer_.capacity = 64;
   timer_.events = new hipEvent_t[2 * timer_.capacity];
   for (int i = 0; i < 2 * timer_.capacity; ++i) NSL_CHECK(hipEventCreate(&timer_.events[i]));
-  NSL_CHECK(hipMalloc(&d_.counters, kCounterCount * sizeof(int32_t)));
-  NSL_CHECK(hipMemset(d_.counters, 0, kCounterCount * sizeof(int32_t)));
-  NSL_CHECK(hipMalloc(&d_barrier_, sizeof(GridBarrier)));
-  NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
+  // the work counters and the grid barrier in one block, read back with one copy per step
+  static_assert(sizeof(StepReadback) == kCounterCount * sizeof(int32_t) + sizeof(GridBarrier), "readback layout");
+  NSL_CHECK(hipMalloc(&d_.counters, sizeof(StepReadback)));
+  NSL_CHECK(hipMemset(d_.counters, 0, sizeof(StepReadback)));
+  d_barrier_ = reinterpret_cast<GridBarrier*>(d_.counters + kCounterCount);
+  NSL_CHECK(hipHostMalloc((void**)&readback_, sizeof(StepReadback), hipHostMallocDefault));
   NSL_CHECK(hipMalloc(&d_phase_log_, 2 * kMaxPhaseLog * sizeof(uint64_t)));
   // zeroed: the host reads marks up to the first time stamp that goes backwards
   NSL_CHECK(hipMemset(d_phase_log_, 0, 2 * kMaxPhaseLog * sizeof(uint64_t)));
@@ -51,7 +53,7 @@ GpuWorld::~GpuWorld() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   void* ptrs[] = {d_.sims, d_.states, d_.static_sims, d_.shapes, d_.fat_aabbs, d_.shape_moved, d_.hulls, d_.contacts,
                   d_.contact_change, d_.constraints, d_.overflow, d_.collide_ids, d_.solve_ids, d_.overflow_ids,
-                  d_.counters, d_init_ids_, d_init_records_, d_barrier_, d_phase_log_, d_proxies_,
+                  d_.counters, d_init_ids_, d_init_records_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
                   d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
                   d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active};
@@ -59,6 +61,7 @@ GpuWorld::~GpuWorld() {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (changes_pinned_) (void)hipHostFree(changes_pinned_);
+  if (readback_) (void)hipHostFree(readback_);
   for (int i = 0; i < 2 * timer_.capacity; ++i) (void)hipEventDestroy(timer_.events[i]);
   delete[] timer_.events;
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -922,7 +925,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   plan.restitution_iterations = options.restitution_iterations;
   plan.small_colour_items = options.small_colour_items;
   plan.barrier = d_barrier_;
-  plan.phase_log = timing_ ? d_phase_log_ : nullptr;
+  plan.phase_log = phase_timing_ ? d_phase_log_ : nullptr;
   int widest = (d_.n_awake + 3) / 4;
   for (int ci = 0; ci < kOverflowColor; ++ci)
     widest = std::max(widest, (cr.end[ci] - cr.begin[ci]) + (joint_ranges_.end[ci] - joint_ranges_.begin[ci]));
@@ -947,12 +950,13 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
                              hipMemcpyDeviceToHost, stream_));
 
   // counters[6] (proxies moved by finalize) tells the next pair update whether to run
-  NSL_CHECK(hipMemcpyAsync(counters_host_, d_.counters, sizeof(counters_host_), hipMemcpyDeviceToHost, stream_));
-  NSL_CHECK(hipMemcpyAsync(&barrier_host_, d_barrier_, sizeof(GridBarrier), hipMemcpyDeviceToHost, stream_));
-  if (timing_)
+  NSL_CHECK(hipMemcpyAsync(readback_, d_.counters, sizeof(StepReadback), hipMemcpyDeviceToHost, stream_));
+  if (phase_timing_)
     NSL_CHECK(hipMemcpyAsync(phase_log_host_.data(), d_phase_log_, phase_log_host_.size() * sizeof(uint64_t),
                              hipMemcpyDeviceToHost, stream_));
   NSL_CHECK(hipStreamSynchronize(stream_));
+  std::memcpy(counters_host_, readback_->counters, sizeof(counters_host_));
+  barrier_host_ = readback_->barrier;
   if (barrier_host_.timed_out) {
     barrier_host_ = GridBarrier{};
     NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
@@ -971,6 +975,8 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
       kstats_.launches[timer_.classes[i]] += 1;
     }
     timer_.used = 0;
+  }
+  if (phase_timing_) {
     // marks: (class of the phase that just ended, wall clock); the first is the kernel start
     int marks = 0;
     for (int i = 0; i < kMaxPhaseLog; ++i, ++marks)
@@ -1056,10 +1062,11 @@ void GpuWorld::ensure_pinned_changes(size_t n) {
   NSL_CHECK(hipHostMalloc((void**)&changes_pinned_, cap_changes_pinned_ * sizeof(int32_t), hipHostMallocDefault));
 }
 
-void GpuWorld::set_kernel_timing(bool on) {
+void GpuWorld::set_kernel_timing(bool on, bool phases) {
   timing_ = on;
+  phase_timing_ = on && phases;
   timer_.used = 0;
-  if (on) {  // no marks of an earlier timed step may be read as this step's
+  if (phase_timing_) {  // no marks of an earlier timed step may be read as this step's
     NSL_CHECK(hipStreamSynchronize(stream_));
     NSL_CHECK(hipMemset(d_phase_log_, 0, 2 * kMaxPhaseLog * sizeof(uint64_t)));
   }

@@ -291,8 +291,9 @@ class World:
         capi.check(self._lib.nsl_world_check_islands(self._h, ctypes.byref(out)))
         return out.value
 
-    def set_kernel_timing(self, on: bool) -> None:
-        capi.check(self._lib.nsl_world_set_kernel_timing(self._h, int(on)))
+    def set_kernel_timing(self, on: bool, phases: bool = True) -> None:
+        """Launch events when `on`; the step kernel's per-phase clock as well when `phases`."""
+        capi.check(self._lib.nsl_world_set_kernel_timing(self._h, (1 | (2 if phases else 0)) if on else 0))
 
     def reset_kernel_stats(self) -> None:
         capi.check(self._lib.nsl_world_reset_kernel_stats(self._h))
