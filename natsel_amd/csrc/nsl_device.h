@@ -83,11 +83,11 @@ struct KernelTimer {
   }
 };
 
-// Grid barrier state of the step kernel (zeroed at creation; count returns to 0 after
-// every completed barrier).
+// Grid barrier state of the step kernel (zeroed at creation; count holds the arrivals of
+// the current launch, zeroed by k_prepare before each).
 struct GridBarrier {
   uint32_t count;
-  uint32_t gen;
+  uint32_t gen;  // unused
   uint32_t timed_out;
   uint32_t pad;
 };

@@ -94,26 +94,26 @@ __global__ void k_collide(DeviceWorld d, int n) {
 // Polls of a barrier wait before the kernel gives up (a few seconds; far above any phase).
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 
-// Grid-wide barrier over a launch whose workgroups are all resident. The last
-// workgroup to arrive resets the count and advances the generation; agent-scope
-// release/acquire make every store issued before the barrier visible to every load after
-// it, across the XCDs' L2 caches. A wait that exceeds the spin limit raises `timed_out`
-// and every workgroup leaves the kernel at its next barrier; returns false then.
+// Grid-wide barrier over a launch whose workgroups are all resident. Arrivals count up
+// for the whole launch (k_prepare zeroes the count before it): barrier k of the launch is
+// complete when the count reaches (k + 1) * gridDim.x, which each workgroup derives from
+// the count it saw on arrival, so the last arrival's own increment releases the waiters.
+// Each workgroup's agent-scope release before it arrives and acquire after the barrier
+// make every store issued before the barrier visible to every load after it, across the
+// XCDs' L2 caches. A wait that exceeds the spin limit raises `timed_out` and every
+// workgroup leaves the kernel at its next barrier; returns false then.
 __device__ bool grid_sync(GridBarrier* b) {
   __shared__ int s_ok;
   __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached L2 before lane 0 releases them
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
-    const uint32_t g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const uint32_t arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == gridDim.x - 1) {
-      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
+    const uint32_t target = (arrived / gridDim.x + 1u) * gridDim.x;
+    if (arrived + 1u != target) {
       uint32_t spins = 0;
-      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+      while (__hip_atomic_load(&b->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         ++spins;
         if (spins > kBarrierSpinLimit ||
             ((spins & 1023u) == 0 && __hip_atomic_load(&b->timed_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
@@ -366,7 +366,12 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
   int points = 0, ovf_points = 0;
-  if (tid == 0) d.counters[kCntFastList] = 0;  // finalize lists this step's fast bodies
+  if (tid == 0) {
+    d.counters[kCntFastList] = 0;  // finalize lists this step's fast bodies
+    // the step kernel's barrier counts its arrivals from zero (stream order: the previous
+    // step kernel has finished, the next starts after this kernel)
+    __hip_atomic_store(&plan.barrier->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   for (int k = tid; k < plan.n_joints; k += nthreads) prepare_joint_item(d, d.joint_ids[k], p);
   for (int slot = tid; slot < plan.n_constraints; slot += nthreads) points += prepare_item(d, slot, p);
   for (int k = tid; k < plan.n_overflow; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);
