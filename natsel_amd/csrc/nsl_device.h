@@ -97,6 +97,11 @@ struct ColorRanges {
   int32_t end[kColorCount];
 };
 
+// One item's warm-start velocity change for one of its bodies.
+struct WarmDelta {
+  Vec3 dv, dw;
+};
+
 // Everything the step kernel needs beyond the world pointers.
 struct SolvePlan {
   ColorRanges cr;                 // coloured slots per colour (colour-major)
@@ -115,6 +120,14 @@ struct SolvePlan {
   int32_t any_restitution;
   int32_t restitution_iterations;
   int32_t small_colour_items;     // runs of colours this small go to workgroup 0 (0: off)
+  // Warm start by body (every awake body dynamic): ws_table[c * n_awake + i] is the item of
+  // colour c (0..22) that touches awake body i, with the body's side s (0: a, 1: b): a
+  // contact slot as (slot << 1 | s), a joint as -((joint id << 1 | s) + 2), or -1. k_prepare
+  // fills it (the host sets it to -1 first). ws_delta holds each item's velocity change per
+  // side: contact slots first (2 * slot + s), then joints by id (2 * (n_constraints + id) + s).
+  // Null: warm start colour by colour.
+  int32_t* ws_table;
+  WarmDelta* ws_delta;
   GridBarrier* barrier;
   uint64_t* phase_log;            // null: no phase timing; else [2 * kMaxPhaseLog]: (class, wall clock)
 };

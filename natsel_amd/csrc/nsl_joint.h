@@ -145,6 +145,20 @@ __device__ inline void prepare_joint_item(const DeviceWorld& d, int32_t jid, con
   d.joints[jid] = j;
 }
 
+// A joint's warm-start velocity changes for its two bodies (warm_start_joint_item's
+// operations; the bodies' dq are not changed by warm starting).
+__device__ inline void warm_delta_joint(const DeviceWorld& d, const JointSim& j, WarmDelta* out) {
+  const BodyState sa = joint_state(d, j.index_a);
+  const BodyState sb = joint_state(d, j.index_b);
+  const Vec3 r_a = rotate(sa.dq, j.frame_a.p);
+  const Vec3 r_b = rotate(sb.dq, j.frame_b.p);
+  const float axial_impulse = ((j.spring_impulse + j.motor_impulse) + j.lower_impulse) - j.upper_impulse;
+  Vec3 angular_impulse = add(scale(j.perp_impulse.x, j.perp_axis_x), scale(j.perp_impulse.y, j.perp_axis_y));
+  angular_impulse = mul_add(angular_impulse, axial_impulse, j.rotation_axis_z);
+  out[0] = WarmDelta{scale(j.inv_mass_a, j.linear_impulse), mul_mv(j.inv_ia, add(cross(r_a, j.linear_impulse), angular_impulse))};
+  out[1] = WarmDelta{scale(j.inv_mass_b, j.linear_impulse), mul_mv(j.inv_ib, add(cross(r_b, j.linear_impulse), angular_impulse))};
+}
+
 __device__ inline void warm_start_joint_item(const DeviceWorld& d, int32_t jid) {
   const JointSim& j = d.joints[jid];
   BodyState sa = joint_state(d, j.index_a);

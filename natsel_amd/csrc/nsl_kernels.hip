@@ -307,15 +307,57 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     if (!grid_sync(bar)) return;
     clock.mark(kPIntegrateVelocities);
 
-    if (any_ovf) {
-      run_overflow_levels(plan, [&](int32_t jid) { warm_start_joint_item(d, jid); }, [&](int k) { overflow_warm_item(d, k); });
+    if (plan.ws_table != nullptr) {
+      // Warm starting only adds stored impulses to velocities. Every coloured item first
+      // computes the velocity change it gives each of its bodies (these read impulses and
+      // body rotations, which warm starting does not change, so this overlaps the overflow
+      // colour's levels); then each body applies its items' changes in colour order with
+      // the operations the colour-by-colour walk applies to it: the same bits in two phases.
+      if (any_ovf)
+        run_overflow_levels(plan, [&](int32_t jid) { warm_start_joint_item(d, jid); }, [&](int k) { overflow_warm_item(d, k); });
+      const int n_cj = plan.jr.begin[kOverflowColor];
+      WarmDelta* jd = plan.ws_delta + 2 * plan.n_constraints;
+      for (int k = tid; k < n_cj; k += nthreads) {
+        const int32_t jid = d.joint_ids[k];
+        warm_delta_joint(d, d.joints[jid], jd + 2 * jid);
+      }
+      for (int slot = tid; slot < plan.n_constraints; slot += nthreads)
+        warm_delta_contact(d.constraints[slot], plan.ws_delta + 2 * slot);
       if (!grid_sync(bar)) return;
-      clock.mark(kPOverflow);
+      clock.mark(kPWarmStart);
+      for (int i = tid; i < d.n_awake; i += nthreads) {
+        int32_t e[kOverflowColor];
+#pragma unroll
+        for (int c = 0; c < kOverflowColor; ++c) e[c] = plan.ws_table[c * d.n_awake + i];
+        BodyState s = d.states[i];
+#pragma unroll
+        for (int c = 0; c < kOverflowColor; ++c) {
+          if (e[c] == kNullIndex) continue;
+          const int32_t k = e[c] >= 0 ? e[c] : -e[c] - 2;  // (item << 1) | side
+          const WarmDelta w = e[c] >= 0 ? plan.ws_delta[k] : jd[k];
+          if (k & 1) {
+            s.w = add(s.w, w.dw);
+            s.v = add(s.v, w.dv);
+          } else {
+            s.w = nsl::sub(s.w, w.dw);  // `sub` is the substep counter here
+            s.v = nsl::sub(s.v, w.dv);
+          }
+        }
+        d.states[i] = s;
+      }
+      if (!grid_sync(bar)) return;
+      clock.mark(kPWarmStart);
+    } else {
+      if (any_ovf) {
+        run_overflow_levels(plan, [&](int32_t jid) { warm_start_joint_item(d, jid); }, [&](int k) { overflow_warm_item(d, k); });
+        if (!grid_sync(bar)) return;
+        clock.mark(kPOverflow);
+      }
+      if (!run_colours(d, colours, tid, nthreads, bar, clock, kPWarmStart, true,
+                       [&](int32_t jid) { warm_start_joint_item(d, jid); },
+                       [&](int slot, int32_t ia, int32_t ib) { warm_start_item(d, slot, ia, ib); }))
+        return;
     }
-    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPWarmStart, true,
-                     [&](int32_t jid) { warm_start_joint_item(d, jid); },
-                     [&](int slot, int32_t ia, int32_t ib) { warm_start_item(d, slot, ia, ib); }))
-      return;
 
     if (any_ovf) {
       run_overflow_levels(plan, [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
@@ -372,8 +414,29 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
     // step kernel has finished, the next starts after this kernel)
     __hip_atomic_store(&plan.barrier->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  for (int k = tid; k < plan.n_joints; k += nthreads) prepare_joint_item(d, d.joint_ids[k], p);
-  for (int slot = tid; slot < plan.n_constraints; slot += nthreads) points += prepare_item(d, slot, p);
+  for (int k = tid; k < plan.n_joints; k += nthreads) {
+    const int32_t jid = d.joint_ids[k];
+    prepare_joint_item(d, jid, p);
+    if (plan.ws_table != nullptr) {
+      int c = 0;
+      while (c < kOverflowColor && k >= plan.jr.end[c]) ++c;  // the overflow colour's joints are last
+      if (c < kOverflowColor) {
+        const JointSim& j = d.joints[jid];
+        if (j.index_a >= 0) plan.ws_table[c * d.n_awake + j.index_a] = -((jid << 1) + 2);
+        if (j.index_b >= 0) plan.ws_table[c * d.n_awake + j.index_b] = -((jid << 1 | 1) + 2);
+      }
+    }
+  }
+  for (int slot = tid; slot < plan.n_constraints; slot += nthreads) {
+    points += prepare_item(d, slot, p);
+    if (plan.ws_table != nullptr) {
+      int c = 0;
+      while (c < kOverflowColor - 1 && slot >= plan.cr.end[c]) ++c;  // slots are colour-major
+      const Constraint& cc = d.constraints[slot];
+      if (cc.index_a >= 0) plan.ws_table[c * d.n_awake + cc.index_a] = slot << 1;
+      if (cc.index_b >= 0) plan.ws_table[c * d.n_awake + cc.index_b] = slot << 1 | 1;
+    }
+  }
   for (int k = tid; k < plan.n_overflow; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);
   block_add(&d.counters[kCntPoints], points);
   block_add(&d.counters[kCntOverflowPoints], ovf_points);

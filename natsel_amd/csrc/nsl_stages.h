@@ -489,9 +489,11 @@ __device__ __forceinline__ void integrate_positions_item(const DeviceWorld& d, i
 // are read unconditionally (the slot always holds four) and used up to point_count.
 
 // contact_solver.rs:697-763
-__device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot, int32_t ia, int32_t ib) {
-  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  const Constraint& c = d.constraints[slot];
+// The impulse a contact constraint's warm start applies: linear, and angular for each body.
+struct WarmImpulse {
+  Vec3 linear, ang_a, ang_b;
+};
+__device__ __forceinline__ WarmImpulse warm_impulse(const Constraint& c) {
   const int pc = c.point_count;
   ConstraintPoint pt[4];
 #pragma unroll
@@ -523,12 +525,28 @@ __device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot, 
     ang_a = add(ang_a, a_a);
     ang_b = add(ang_b, a_b);
   }
-  ba.w = sub(ba.w, sym_mv(c.inv_ia, ang_a));
-  ba.v = sub(ba.v, scale(c.inv_mass_a, linear));
-  bb.w = add(bb.w, sym_mv(c.inv_ib, ang_b));
-  bb.v = add(bb.v, scale(c.inv_mass_b, linear));
+  return WarmImpulse{linear, ang_a, ang_b};
+}
+
+__device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot, int32_t ia, int32_t ib) {
+  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
+  const Constraint& c = d.constraints[slot];
+  const WarmImpulse wi = warm_impulse(c);
+  ba.w = sub(ba.w, sym_mv(c.inv_ia, wi.ang_a));
+  ba.v = sub(ba.v, scale(c.inv_mass_a, wi.linear));
+  bb.w = add(bb.w, sym_mv(c.inv_ib, wi.ang_b));
+  bb.v = add(bb.v, scale(c.inv_mass_b, wi.linear));
   store_state(d, ia, ba);
   store_state(d, ib, bb);
+}
+
+// Warm start by body: an item's velocity changes for each of its two bodies, computed with
+// the operations warm_start_item / warm_start_joint_item use and applied by the body in
+// colour order (side a subtracts, side b adds; contacts change w then v, joints v then w).
+__device__ __forceinline__ void warm_delta_contact(const Constraint& c, WarmDelta* out) {
+  const WarmImpulse wi = warm_impulse(c);
+  out[0] = WarmDelta{scale(c.inv_mass_a, wi.linear), sym_mv(c.inv_ia, wi.ang_a)};
+  out[1] = WarmDelta{scale(c.inv_mass_b, wi.linear), sym_mv(c.inv_ib, wi.ang_b)};
 }
 
 // contact_solver.rs:796-894

@@ -256,6 +256,10 @@ class GpuWorld : public HostScene {
   size_t cap_candidates_ = 0, cap_fast_ = 0;
   ContinuousArgs ccd_{};
   int32_t bullet_bodies_ = 0;  // dynamic bullet bodies (the bullet pass runs only if any)
+  int32_t kinematic_awake_ = 0;  // awake bodies that are not dynamic (warm start then runs by colour)
+  int32_t* d_ws_table_ = nullptr;  // warm start by body: the item of each colour per awake body
+  WarmDelta* d_ws_delta_ = nullptr;  // and each item's velocity change per side
+  size_t cap_ws_table_ = 0, cap_ws_delta_ = 0;
   // device pair finder and the pair-key set it checks (new contacts insert, disjoint ones erase)
   std::unique_ptr<PairFinder> finder_;
   Proxy* d_proxies_ = nullptr;

@@ -56,7 +56,7 @@ GpuWorld::~GpuWorld() {
                   d_.counters, d_init_ids_, d_init_records_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
                   d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
-                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active};
+                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active, d_ws_table_, d_ws_delta_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -172,8 +172,11 @@ void GpuWorld::upload_bodies() {
     ccd_ = ContinuousArgs{d_candidates_, per_type[kBodyStatic], per_type[kBodyKinematic], per_type[kBodyDynamic], d_proxies_};
   }
   bullet_bodies_ = 0;
-  for (const BodySim& s : awake_host_)
+  kinematic_awake_ = 0;
+  for (const BodySim& s : awake_host_) {
     if (s.body_type == kBodyDynamic && (s.flags & bflag::kIsBullet)) bullet_bodies_ += 1;
+    kinematic_awake_ += s.body_type != kBodyDynamic;
+  }
   bodies_dirty_ = false;
   device_newer_ = false;
 }
@@ -762,8 +765,11 @@ void GpuWorld::restore_state(const Snapshot& s) {
   split_island_ = s.split_island;
   d_.n_awake = (int32_t)s.n_awake;
   bullet_bodies_ = 0;
-  for (const BodySim& b : awake_host_)
+  kinematic_awake_ = 0;
+  for (const BodySim& b : awake_host_) {
     if (b.body_type == kBodyDynamic && (b.flags & bflag::kIsBullet)) bullet_bodies_ += 1;
+    kinematic_awake_ += b.body_type != kBodyDynamic;
+  }
   device_newer_ = false;
   joints_dirty_ = true;  // the colour lists of joints are rebuilt from colors_
   // the awake-set flags of every contact id, from the restored records
@@ -924,6 +930,17 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   plan.any_restitution = any_restitution ? 1 : 0;
   plan.restitution_iterations = options.restitution_iterations;
   plan.small_colour_items = options.small_colour_items;
+  // warm start by body needs every awake body dynamic (a kinematic body can sit in several
+  // items of one colour)
+  plan.ws_table = nullptr;
+  plan.ws_delta = nullptr;
+  if (kinematic_awake_ == 0) {
+    ensure(d_ws_table_, cap_ws_table_, (size_t)kOverflowColor * (size_t)d_.n_awake, false);
+    ensure(d_ws_delta_, cap_ws_delta_, 2 * (solve.size() + hjoints_.size()) + 2, false);
+    NSL_CHECK(hipMemsetAsync(d_ws_table_, 0xFF, (size_t)kOverflowColor * (size_t)d_.n_awake * sizeof(int32_t), stream_));
+    plan.ws_table = d_ws_table_;
+    plan.ws_delta = d_ws_delta_;
+  }
   plan.barrier = d_barrier_;
   plan.phase_log = phase_timing_ ? d_phase_log_ : nullptr;
   int widest = (d_.n_awake + 3) / 4;
@@ -1023,7 +1040,9 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     }
     phases += run >= 2 ? 1 : run;
     const int ovf_phase = plan.n_ovf_levels > 0 ? 1 : 0;
-    int64_t barriers = (int64_t)p.sub_steps * (2 + 3 * (phases + ovf_phase));
+    // warm start by body: the delta phase (with the overflow levels) and the body phase
+    const int ws_phases = plan.ws_table != nullptr ? 2 : phases + ovf_phase;
+    int64_t barriers = (int64_t)p.sub_steps * (2 + ws_phases + 2 * (phases + ovf_phase));
     if (any_restitution) barriers += (int64_t)options.restitution_iterations * (phases + (ovf.empty() ? 0 : 1));
     kstats_.barriers += barriers;
   }
