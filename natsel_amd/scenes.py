@@ -828,3 +828,18 @@ def pile_defaults(count: int = 10_000, seed: int = 0x5EED, **kw) -> Scene:
     sc.enable_continuous = True
     sc.bodies = [dataclasses.replace(b, enable_sleep=True) for b in sc.bodies]
     return sc
+
+
+def kinematic_platform(boxes: int = 6) -> Scene:
+    """A kinematic slab sliding and turning under a row of boxes, with a static ground below
+    (kinematic bodies move at their set velocities and push dynamic bodies; no dynamic body
+    pushes them back). Sleep off."""
+    sc = Scene(name="kinematic_platform", enable_sleep=False)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -2.0, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (20.0, 0.5, 20.0)))
+    k = sc.add_body(BodySpec(KINEMATIC, (0.0, 0.0, 0.0), linear_velocity=(0.5, 0.0, 0.0), angular_velocity=(0.0, 0.3, 0.0)))
+    sc.add_shape(ShapeSpec(k, BOX, (4.0, 0.25, 2.0)))
+    for i in range(boxes):
+        b = sc.add_body(BodySpec(DYNAMIC, (-3.0 + 1.2 * i, 0.76, 0.0)))
+        sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), friction=0.8))
+    return sc

@@ -201,3 +201,16 @@ def test_small_colour_groups_do_not_change_results(gpu_available):
         assert hashes == {o.world_state_hash()}, f"step {k}: {sorted(hashes)}"
     barriers = [w.kernel_stats()["barriers"] for w in worlds]
     assert barriers[0] > barriers[2] >= barriers[3], barriers
+
+
+def test_kinematic_platform_every_step_matches_oracle(gpu_available):
+    """A kinematic slab carrying boxes: the step keeps warm starting colour by colour when an
+    awake body is not dynamic (a kinematic body can sit in several items of one colour)."""
+    sc = scenes.kinematic_platform()
+    g, o = _pair(sc)
+    for k in range(300):
+        g.step(1.0 / 60.0, 4)
+        o.step(1.0 / 60.0, 4)
+        hg, ho = g.world_state_hash(), o.world_state_hash()
+        assert hg == ho, f"step {k}: {hg:x} != {ho:x}; {_first_difference(g, o)}"
+    assert abs(g.body_arrays()[0][1][0] - 2.5) < 1e-4  # the platform moved at its own speed
