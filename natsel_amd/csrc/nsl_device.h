@@ -128,6 +128,10 @@ struct SolvePlan {
   // Null: warm start colour by colour.
   int32_t* ws_table;
   WarmDelta* ws_delta;
+  // With warm start by body and an overflow colour: per awake body, -1 for a body outside the
+  // overflow colour (its velocity is integrated by any lane), else the last substep + 1 whose
+  // velocity workgroup 0 integrated (0: none yet). The host sets -1; k_prepare marks.
+  int32_t* ovf_done;
   GridBarrier* barrier;
   uint64_t* phase_log;            // null: no phase timing; else [2 * kMaxPhaseLog]: (class, wall clock)
 };

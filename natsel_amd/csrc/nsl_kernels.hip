@@ -303,9 +303,12 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   auto no_joint = [](int32_t) {};
 
   for (int sub = 0; sub < p.sub_steps; ++sub) {
-    for (int i = tid; i < d.n_awake; i += nthreads) integrate_velocities_item(d, i, p);
-    if (!grid_sync(bar)) return;
-    clock.mark(kPIntegrateVelocities);
+    // with warm start by body, velocities are integrated in its first phase
+    if (plan.ws_table == nullptr) {
+      for (int i = tid; i < d.n_awake; i += nthreads) integrate_velocities_item(d, i, p);
+      if (!grid_sync(bar)) return;
+      clock.mark(kPIntegrateVelocities);
+    }
 
     if (plan.ws_table != nullptr) {
       // Warm starting only adds stored impulses to velocities. Every coloured item first
@@ -313,8 +316,32 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       // body rotations, which warm starting does not change, so this overlaps the overflow
       // colour's levels); then each body applies its items' changes in colour order with
       // the operations the colour-by-colour walk applies to it: the same bits in two phases.
-      if (any_ovf)
-        run_overflow_levels(plan, [&](int32_t jid) { warm_start_joint_item(d, jid); }, [&](int k) { overflow_warm_item(d, k); });
+      // Velocities are integrated here too (it writes only v and w, which this phase does not
+      // otherwise read): workgroup 0 integrates a body of the overflow colour just before its
+      // first overflow item in this substep, every lane integrates the other bodies.
+      if (any_ovf) {
+        const int32_t stamp = sub + 1;
+        auto integrate_once = [&](int32_t b) {
+          if (b >= 0 && plan.ovf_done[b] != stamp) {
+            integrate_velocities_item(d, b, p);
+            plan.ovf_done[b] = stamp;
+          }
+        };
+        run_overflow_levels(
+            plan,
+            [&](int32_t jid) {
+              integrate_once(d.joints[jid].index_a);
+              integrate_once(d.joints[jid].index_b);
+              warm_start_joint_item(d, jid);
+            },
+            [&](int k) {
+              integrate_once(d.overflow[k].index_a);
+              integrate_once(d.overflow[k].index_b);
+              overflow_warm_item(d, k);
+            });
+      }
+      for (int i = tid; i < d.n_awake; i += nthreads)
+        if (!any_ovf || plan.ovf_done[i] < 0) integrate_velocities_item(d, i, p);
       const int n_cj = plan.jr.begin[kOverflowColor];
       WarmDelta* jd = plan.ws_delta + 2 * plan.n_constraints;
       for (int k = tid; k < n_cj; k += nthreads) {
@@ -420,6 +447,11 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
     if (plan.ws_table != nullptr) {
       int c = 0;
       while (c < kOverflowColor && k >= plan.jr.end[c]) ++c;  // the overflow colour's joints are last
+      if (c == kOverflowColor && plan.ovf_done != nullptr) {
+        const JointSim& j = d.joints[jid];
+        if (j.index_a >= 0) plan.ovf_done[j.index_a] = 0;
+        if (j.index_b >= 0) plan.ovf_done[j.index_b] = 0;
+      }
       if (c < kOverflowColor) {
         const JointSim& j = d.joints[jid];
         if (j.index_a >= 0) plan.ws_table[c * d.n_awake + j.index_a] = -((jid << 1) + 2);
@@ -437,7 +469,13 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
       if (cc.index_b >= 0) plan.ws_table[c * d.n_awake + cc.index_b] = slot << 1 | 1;
     }
   }
-  for (int k = tid; k < plan.n_overflow; k += nthreads) ovf_points += prepare_overflow_item(d, k, p);
+  for (int k = tid; k < plan.n_overflow; k += nthreads) {
+    ovf_points += prepare_overflow_item(d, k, p);
+    if (plan.ovf_done != nullptr) {  // bodies of the overflow colour: integrated by workgroup 0
+      if (d.overflow[k].index_a >= 0) plan.ovf_done[d.overflow[k].index_a] = 0;
+      if (d.overflow[k].index_b >= 0) plan.ovf_done[d.overflow[k].index_b] = 0;
+    }
+  }
   block_add(&d.counters[kCntPoints], points);
   block_add(&d.counters[kCntOverflowPoints], ovf_points);
 }

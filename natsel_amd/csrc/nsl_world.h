@@ -259,7 +259,8 @@ class GpuWorld : public HostScene {
   int32_t kinematic_awake_ = 0;  // awake bodies that are not dynamic (warm start then runs by colour)
   int32_t* d_ws_table_ = nullptr;  // warm start by body: the item of each colour per awake body
   WarmDelta* d_ws_delta_ = nullptr;  // and each item's velocity change per side
-  size_t cap_ws_table_ = 0, cap_ws_delta_ = 0;
+  int32_t* d_ovf_done_ = nullptr;  // overflow bodies' velocity integration marks
+  size_t cap_ws_table_ = 0, cap_ws_delta_ = 0, cap_ovf_done_ = 0;
   // device pair finder and the pair-key set it checks (new contacts insert, disjoint ones erase)
   std::unique_ptr<PairFinder> finder_;
   Proxy* d_proxies_ = nullptr;

@@ -56,7 +56,7 @@ GpuWorld::~GpuWorld() {
                   d_.counters, d_init_ids_, d_init_records_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
                   d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
-                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active, d_ws_table_, d_ws_delta_};
+                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active, d_ws_table_, d_ws_delta_, d_ovf_done_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -941,6 +941,12 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     plan.ws_table = d_ws_table_;
     plan.ws_delta = d_ws_delta_;
   }
+  plan.ovf_done = nullptr;
+  if (plan.ws_table != nullptr && plan.n_ovf_levels > 0) {
+    ensure(d_ovf_done_, cap_ovf_done_, (size_t)d_.n_awake, false);
+    NSL_CHECK(hipMemsetAsync(d_ovf_done_, 0xFF, (size_t)d_.n_awake * sizeof(int32_t), stream_));
+    plan.ovf_done = d_ovf_done_;
+  }
   plan.barrier = d_barrier_;
   plan.phase_log = phase_timing_ ? d_phase_log_ : nullptr;
   int widest = (d_.n_awake + 3) / 4;
@@ -1040,9 +1046,10 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     }
     phases += run >= 2 ? 1 : run;
     const int ovf_phase = plan.n_ovf_levels > 0 ? 1 : 0;
-    // warm start by body: the delta phase (with the overflow levels) and the body phase
-    const int ws_phases = plan.ws_table != nullptr ? 2 : phases + ovf_phase;
-    int64_t barriers = (int64_t)p.sub_steps * (2 + ws_phases + 2 * (phases + ovf_phase));
+    // warm start by body: the delta phase (with the overflow levels and the velocity
+    // integration) and the body phase
+    const int iv_ws_phases = plan.ws_table != nullptr ? 2 : 1 + phases + ovf_phase;
+    int64_t barriers = (int64_t)p.sub_steps * (1 + iv_ws_phases + 2 * (phases + ovf_phase));
     if (any_restitution) barriers += (int64_t)options.restitution_iterations * (phases + (ovf.empty() ? 0 : 1));
     kstats_.barriers += barriers;
   }
