@@ -25,7 +25,7 @@ def _check(batch, oracles, step):
 
 
 def test_batch_worlds_match_oracle_every_step(gpu_available):
-    """Piles and stacks of 8..40 bodies (the config-3 generator, scaled down), the
+    """Piles and stacks of 8..40 bodies (the 4096-world batch generator, scaled down), the
     210-box pyramid and the overflow hub in one batch."""
     from natsel_amd.batch import load_batch
 
