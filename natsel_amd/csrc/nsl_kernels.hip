@@ -311,11 +311,12 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     }
 
     if (plan.ws_table != nullptr) {
-      // Warm starting only adds stored impulses to velocities. Every coloured item first
-      // computes the velocity change it gives each of its bodies (these read impulses and
-      // body rotations, which warm starting does not change, so this overlaps the overflow
-      // colour's levels); then each body applies its items' changes in colour order with
-      // the operations the colour-by-colour walk applies to it: the same bits in two phases.
+      // Warm starting only adds stored impulses to velocities. In a first phase every
+      // coloured item computes the velocity change it gives each of its bodies (k_prepare
+      // has done so for the first substep; these read impulses and body rotations, which
+      // warm starting does not change), velocities are integrated and the overflow colour's
+      // levels run; then each body applies its items' changes in colour order with the
+      // operations the colour-by-colour walk applies to it: the same bits in two phases.
       // Velocities are integrated here too (it writes only v and w, which this phase does not
       // otherwise read): workgroup 0 integrates a body of the overflow colour just before its
       // first overflow item in this substep, every lane integrates the other bodies.
@@ -342,14 +343,16 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       }
       for (int i = tid; i < d.n_awake; i += nthreads)
         if (!any_ovf || plan.ovf_done[i] < 0) integrate_velocities_item(d, i, p);
-      const int n_cj = plan.jr.begin[kOverflowColor];
       WarmDelta* jd = plan.ws_delta + 2 * plan.n_constraints;
-      for (int k = tid; k < n_cj; k += nthreads) {
-        const int32_t jid = d.joint_ids[k];
-        warm_delta_joint(d, d.joints[jid], jd + 2 * jid);
+      if (sub > 0) {
+        const int n_cj = plan.jr.begin[kOverflowColor];
+        for (int k = tid; k < n_cj; k += nthreads) {
+          const int32_t jid = d.joint_ids[k];
+          warm_delta_joint(d, d.joints[jid], jd + 2 * jid);
+        }
+        for (int slot = tid; slot < plan.n_constraints; slot += nthreads)
+          warm_delta_contact(d.constraints[slot], plan.ws_delta + 2 * slot);
       }
-      for (int slot = tid; slot < plan.n_constraints; slot += nthreads)
-        warm_delta_contact(d.constraints[slot], plan.ws_delta + 2 * slot);
       if (!grid_sync(bar)) return;
       clock.mark(kPWarmStart);
       for (int i = tid; i < d.n_awake; i += nthreads) {
@@ -456,6 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
         const JointSim& j = d.joints[jid];
         if (j.index_a >= 0) plan.ws_table[c * d.n_awake + j.index_a] = -((jid << 1) + 2);
         if (j.index_b >= 0) plan.ws_table[c * d.n_awake + j.index_b] = -((jid << 1 | 1) + 2);
+        warm_delta_joint(d, j, plan.ws_delta + 2 * (plan.n_constraints + jid));  // the first substep's
       }
     }
   }
@@ -467,6 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
       const Constraint& cc = d.constraints[slot];
       if (cc.index_a >= 0) plan.ws_table[c * d.n_awake + cc.index_a] = slot << 1;
       if (cc.index_b >= 0) plan.ws_table[c * d.n_awake + cc.index_b] = slot << 1 | 1;
+      warm_delta_contact(cc, plan.ws_delta + 2 * slot);  // the first substep's
     }
   }
   for (int k = tid; k < plan.n_overflow; k += nthreads) {
