@@ -3,16 +3,18 @@
 Workload (one GPU): 10,000 dynamic bodies (1/3 boxes, 1/3 spheres, 1/3 capsules,
 sizes U[0.25, 0.75] m, density 1000, friction U[0.4, 0.8]) dropped from seeded
 lattice cells of a 20 x 60 x 20 m region into a static 5-wall container; dt = 1/60,
-4 substeps, f32, continuous collision and sleeping off. A "step" is one World::step
-(pair update, narrow phase, constraint preparation, 4 soft-step substeps, impulse
-storage, finalize). W warmup steps run untimed, then K steps are timed with a device
-synchronise on both sides, as the reference harness times steps 1..N.
+4 substeps, f32, continuous collision and sleeping off; the pile's run is 1000 steps. A
+"step" is one World::step (pair update, narrow phase, constraint preparation, 4 soft-step
+substeps, impulse storage, finalize). The timed K steps are the last K of the 1000-step run
+(the settled pile, ~48k contact points per step): the first 1000 - K steps (at least W)
+run untimed, then K steps are timed with a device synchronise on both sides.
 
 value = dynamic bodies x substeps x K / seconds (inputs resident in HBM: the scene is
-uploaded before the timed region). The roofline object reports the dominant kernel
-class: algorithmic bytes (the byte model below) / its summed launch time from HIP
-events. cpu_baseline is the CPU oracle (a restatement of the reference step) timed on a
-bounded sample of the same workload on this host, one thread.
+uploaded before the timed region). The roofline object reports the dominant kernel (the
+step kernel): algorithmic bytes (the byte model below) / its summed launch time from HIP
+events. cpu_baseline is the CPU oracle (a restatement of the reference step) run through
+the same first 1000 - K steps on this host and timed on the same K steps from the same
+state (minimum of 3 repeats), with its hashes compared with the GPU's.
 """
 from __future__ import annotations
 
@@ -90,41 +92,73 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def _oracle_rate(scene, threads: int, seconds: float, max_steps: int):
-    from oracle import pyoracle
+PILE_SCHEDULE = 1000  # the pile's run: 1000 steps from the drop
 
-    pyoracle.set_threads(threads)
-    w = pyoracle.load_scene(scene)
+
+def settle_steps(steps: int, warmup: int) -> int:
+    """Untimed steps before the timed window: the timed K steps are the last K of the pile's
+    1000-step run (the settled pile), and never fewer than the W warmup steps precede them."""
+    return max(warmup, PILE_SCHEDULE - steps)
+
+
+def _timed_steps(w, scene, k: int) -> float:
     t0 = time.perf_counter()
-    steps = 0
-    while steps < max_steps and time.perf_counter() - t0 < seconds:
+    for _ in range(k):
         w.step(scene.dt, scene.substeps)
-        steps += 1
-    dt = time.perf_counter() - t0
-    return scene.dynamic_count * scene.substeps * steps / dt, steps, dt, w.phase_ms()
+    return time.perf_counter() - t0
 
 
-def cpu_baseline(scene, seconds: float, max_steps: int) -> dict:
+def cpu_baseline(scene, settle: int, k: int, seconds: float, gpu_hashes: dict) -> dict:
     """The oracle (a C++ restatement of the reference step, OpenMP over the loops whose
-    items are independent) on this host: all threads for ~70% of the budget, then one
-    thread for the rest, each from step 1 of a fresh world."""
+    items are independent) on this host, timed on the same steps from the same state as the
+    GPU's timed window: it runs the pile's first `settle` steps untimed on all threads (its
+    hash there is compared with the GPU's), saves its state, then times the window's `k`
+    steps three times from that state (minimum reported), on all threads and on one thread.
+    The one-thread sample is cut to fewer steps when `k` would not fit in its share of
+    `seconds`."""
     from oracle import pyoracle
 
     threads = pyoracle.max_threads()
-    value, steps, dt, phases = _oracle_rate(scene, threads, 0.7 * seconds, max_steps)
-    value1, steps1, dt1, _ = _oracle_rate(scene, 1, 0.3 * seconds, max_steps)
     pyoracle.set_threads(threads)
+    w = pyoracle.load_scene(scene)
+    settle_s = _timed_steps(w, scene, settle)
+    settle_equal = w.world_state_hash() == gpu_hashes["settle"]
+    snap = w.save_state()
+    reps = []
+    for _ in range(3):
+        w.restore_state(snap)
+        p0 = w.phase_ms()
+        reps.append(_timed_steps(w, scene, k))
+    end_equal = w.world_state_hash() == gpu_hashes["end"]
+    phases = {key: v - p0[key] for key, v in w.phase_ms().items()}
+    pyoracle.set_threads(1)
+    w.restore_state(snap)
+    one = _timed_steps(w, scene, 1)
+    k1 = int(max(1, min(k, 0.3 * seconds / 3 / max(one, 1e-6))))
+    reps1 = []
+    for _ in range(3):
+        w.restore_state(snap)
+        reps1.append(_timed_steps(w, scene, k1))
+    pyoracle.set_threads(threads)
+    n = scene.dynamic_count * scene.substeps
     return {
-        "value": value,
+        "value": n * k / min(reps),
         "unit": "body-substeps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle (C++ restatement of the reference step, g++ -O3, OpenMP, {threads} threads), "
-                  f"steps 1..{steps} of the same pile",
-        "steps": steps,
-        "seconds": dt,
-        "phase_ms_per_step": {k: round(v / max(steps, 1), 3) for k, v in phases.items()},
-        "single_thread": {"value": value1, "steps": steps1, "seconds": dt1},
+        "sample": f"CPU oracle (C++ restatement of the reference step, g++ -O3, OpenMP, {threads} threads): steps "
+                  f"{settle + 1}..{settle + k} of the pile from the same settled state as the GPU's timed steps, "
+                  "minimum of 3 repeats",
+        "steps": k,
+        "seconds_min": min(reps),
+        "seconds_repeats": reps,
+        "phase_ms_per_step": {key: round(v / k, 3) for key, v in phases.items()},
+        "single_thread": {"value": n * k1 / min(reps1), "steps": k1, "seconds_min": min(reps1), "seconds_repeats": reps1,
+                          "sample": f"steps {settle + 1}..{settle + k1} from the same state, minimum of 3 repeats"},
+        "full_schedule_settle": {"steps": settle, "seconds": settle_s,
+                                 "value": n * settle / settle_s if settle_s > 0 else None},
+        "hash_equal_to_gpu_after_settle": settle_equal,
+        "hash_equal_to_gpu_after_window": end_equal,
     }
 
 
@@ -350,11 +384,51 @@ def adjoint_measurement(worlds: int, steps: int, fd_worlds: int = 16) -> dict:
     }
 
 
+def summarize_profile(path: str, kernel: str, window: str) -> dict:
+    """Summary of a rocprofv3 CSV (a kernel trace or a --pmc counter collection) over the
+    dispatches `window` ("a:b", 1-based, inclusive, counted over this kernel's dispatches in
+    dispatch order) of the kernels whose name starts with `kernel`: average duration (us) for
+    a trace, average value per counter for a counter collection, and the same over all of
+    the kernel's dispatches."""
+    import csv
+
+    a, b = (int(x) for x in window.split(":"))
+    rows = [r for r in csv.DictReader(open(path)) if r["Kernel_Name"].startswith(kernel)]
+    if not rows:
+        raise SystemExit(f"no dispatch of {kernel} in {path}")
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    out = {"file": os.path.basename(path), "kernel": kernel, "window": [a, b]}
+    if "Counter_Name" in rows[0]:
+        per_dispatch: dict = {}
+        for r in rows:
+            per_dispatch.setdefault(int(r["Dispatch_Id"]), {}).setdefault(r["Counter_Name"], 0.0)
+            per_dispatch[int(r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
+        ids = sorted(per_dispatch)
+        sel = ids[a - 1:b]
+        names = sorted(per_dispatch[ids[0]])
+        out["dispatches"] = len(ids)
+        out["window_dispatches"] = len(sel)
+        out["window_mean"] = {c: sum(per_dispatch[i][c] for i in sel) / len(sel) for c in names}
+        out["all_mean"] = {c: sum(per_dispatch[i][c] for i in ids) / len(ids) for c in names}
+    else:
+        dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
+        sel = dur[a - 1:b]
+        out["dispatches"] = len(dur)
+        out["window_dispatches"] = len(sel)
+        out["window_avg_us"] = sum(sel) / len(sel)
+        out["window_min_us"] = min(sel)
+        out["window_max_us"] = max(sel)
+        out["all_avg_us"] = sum(dur) / len(dur)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100,
+                    help="timed steps: the last K of the pile's 1000-step run")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed steps before the window (at least; the run's first 1000 - K steps are untimed)")
     ap.add_argument("--bodies", type=int, default=10_000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -370,7 +444,18 @@ def main() -> int:
                     help="steps of the pile with sleep and continuous collision on (0: skip it)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (reference kernel for rocprofv3 --pmc byte counters)")
+    ap.add_argument("--summarize-profile", default="",
+                    help="only summarise this rocprofv3 CSV (kernel trace or counter collection) over --kernel's "
+                         "dispatches --window a:b, then exit (no GPU used)")
+    ap.add_argument("--kernel", default="k_solve_step")
+    ap.add_argument("--window", default="", help="a:b, 1-based dispatches of --kernel (default: the timed window "
+                                                 "of a run with these --steps/--warmup)")
     args = ap.parse_args()
+    if args.summarize_profile:
+        settle = settle_steps(args.steps, args.warmup)
+        window = args.window or f"{settle + 1}:{settle + args.steps}"
+        print(json.dumps(summarize_profile(args.summarize_profile, args.kernel, window), indent=1))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)", file=sys.stderr)
         return 2
@@ -391,33 +476,41 @@ def main() -> int:
     from natsel_amd.world import load_scene
 
     scene = scenes.pile(args.bodies)
+    settle = settle_steps(args.steps, args.warmup)
     world = load_scene(scene, device=0, solver_blocks=args.blocks, small_colour_items=args.small_colours)
-    for _ in range(args.warmup):
+    # untimed: the pile's first `settle` steps (the drop and the settling), which include the
+    # W warmup steps; the timed window is the run's last K steps
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(settle):
         world.step(scene.dt, scene.substeps)
+    torch.cuda.synchronize()
+    settle_s = time.perf_counter() - s0
+    hashes = {"settle": world.world_state_hash()}
+    snap = world.save_state()
     world.reset_kernel_stats()
     world.set_kernel_timing(True, phases=False)  # launch events only: the step kernel's HIP-event time
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    marks = []
     for _ in range(args.steps):
         world.step(scene.dt, scene.substeps)  # returns after the step's device work
-        marks.append(time.perf_counter())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     world.set_kernel_timing(False)
     ks = world.kernel_stats()
-    world.close()  # release device memory now, not at interpreter exit
-    # the same steps again with the step kernel's phase clock on, for the per-phase breakdown
-    diag = load_scene(scene, device=0, solver_blocks=args.blocks, small_colour_items=args.small_colours)
-    for _ in range(args.warmup):
-        diag.step(scene.dt, scene.substeps)
-    diag.reset_kernel_stats()
-    diag.set_kernel_timing(True, phases=True)
+    hashes["end"] = world.world_state_hash()
+    # the same steps again from the saved state with the step kernel's phase clock on, for the
+    # per-phase breakdown; the restored run must end on the same hash
+    world.restore_state(snap)
+    world.reset_kernel_stats()
+    world.set_kernel_timing(True, phases=True)
     for _ in range(args.steps):
-        diag.step(scene.dt, scene.substeps)
-    diag.set_kernel_timing(False)
-    ks_phases = diag.kernel_stats()
-    diag.close()
+        world.step(scene.dt, scene.substeps)
+    world.set_kernel_timing(False)
+    ks_phases = world.kernel_stats()
+    rerun_equal = world.world_state_hash() == hashes["end"]
+    del snap
+    world.close()  # release device memory now, not at interpreter exit
     n = scene.dynamic_count
     value = n * scene.substeps * args.steps / elapsed
 
@@ -470,13 +563,20 @@ def main() -> int:
         "points_per_step": ks["points"] / max(ks["steps"], 1),
         "pair_tests_per_step": ks["pair_tests"] / max(ks["steps"], 1),
         "fast_bodies": ks["fast_bodies"],
+        "timed_window": {"first_step": settle + 1, "last_step": settle + args.steps, "untimed_steps_before": settle,
+                         "step_kernel_dispatches": [settle + 1, settle + args.steps],
+                         "note": "the step kernel runs once per step, so the timed window is its dispatches "
+                                 "first_step..last_step of this process (rocprofv3 summaries average over those)"},
+        "full_schedule": {"steps": settle + args.steps, "seconds": settle_s + elapsed,
+                          "value": n * scene.substeps * (settle + args.steps) / (settle_s + elapsed),
+                          "note": "the whole run from the drop (first step includes the upload), untimed steps included"},
+        "hash_after_settle": f"{hashes['settle']:016x}",
+        "hash_after_window": f"{hashes['end']:016x}",
+        "rerun_from_snapshot_same_hash": rerun_equal,
     }
     if not args.no_cpu:
-        cb = cpu_baseline(scene, args.cpu_seconds, args.steps)
-        k = min(cb["steps"], len(marks))
-        # GPU rate over the same leading steps as the CPU sample, for a like-for-like ratio
-        cb["gpu_value_same_steps"] = n * scene.substeps * k / (marks[k - 1] - t0)
-        cb["gpu_over_cpu_same_steps"] = cb["gpu_value_same_steps"] / cb["value"]
+        cb = cpu_baseline(scene, settle, args.steps, args.cpu_seconds, hashes)
+        cb["gpu_over_cpu"] = value / cb["value"]
         out["cpu_baseline"] = cb
     if args.batched_worlds > 0:
         out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps)
