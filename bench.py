@@ -81,15 +81,22 @@ def solve_bytes(ks: dict, substeps: int) -> float:
 TRAFFIC_PROFILE = os.path.join("profiles", "k_solve_step_pile10k_pmc_bytes.md")
 
 
-def measured_traffic():
-    """HBM-side bytes per step-kernel launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (scaled by a copy kernel of known size), or None."""
+BATCH_TRAFFIC_PROFILE = os.path.join("profiles", "k_batch_step_4096_trace_pmc.md")
+
+
+def measured_traffic(profile: str = TRAFFIC_PROFILE, key: str = "traffic_bytes_per_launch"):
+    """Fabric-side bytes (per step-kernel launch, or per batch step) from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a copy kernel of known size), or None."""
     try:
-        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+        text = open(os.path.join(ROOT, profile)).read()
     except OSError:
         return None
-    m = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
+    m = re.search(r"^" + key + r": (\d+)", text, re.M)
     return float(m.group(1)) if m else None
+
+
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 PILE_SCHEDULE = 1000  # the pile's run: 1000 steps from the drop
@@ -194,7 +201,9 @@ def batched_measurement(worlds: int, steps: int) -> dict:
                                "piles and stacks of boxes/spheres/capsules on ground boxes), dt 1/60, 4 substeps",
                    "worlds": worlds, "bodies": bodies},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_batch_pairs + k_batch_collide + k_batch_solve (one step)",
+                     "traffic": measured_traffic(BATCH_TRAFFIC_PROFILE, "traffic_bytes_per_step"),
+                     "traffic_unit": "bytes per step (the three launches)", "traffic_source": BATCH_TRAFFIC_PROFILE,
+                     "kernel": "k_batch_pairs + k_batch_collide + k_batch_solve (one step)",
                      "algorithmic_bytes_per_step": total / steps},
         "contacts_collided_per_step": counts["collided"] / steps,
         "constraints_per_step": counts["constraints"] / steps,
@@ -387,16 +396,18 @@ def adjoint_measurement(worlds: int, steps: int, fd_worlds: int = 16) -> dict:
 def summarize_profile(path: str, kernel: str, window: str) -> dict:
     """Summary of a rocprofv3 CSV (a kernel trace or a --pmc counter collection) over the
     dispatches `window` ("a:b", 1-based, inclusive, counted over this kernel's dispatches in
-    dispatch order) of the kernels whose name starts with `kernel`: average duration (us) for
+    dispatch order, or "all") of the kernel named `kernel` (namespace-qualified or not): average duration (us) for
     a trace, average value per counter for a counter collection, and the same over all of
     the kernel's dispatches."""
     import csv
 
-    a, b = (int(x) for x in window.split(":"))
-    rows = [r for r in csv.DictReader(open(path)) if r["Kernel_Name"].startswith(kernel)]
+    name = re.compile(r"(^|[\s:])" + re.escape(kernel) + r"($|[(<\s])")  # plain or namespace-qualified
+    rows = [r for r in csv.DictReader(open(path)) if name.search(r["Kernel_Name"])]
     if not rows:
         raise SystemExit(f"no dispatch of {kernel} in {path}")
     rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    n = len({r["Dispatch_Id"] for r in rows})
+    a, b = (1, n) if window == "all" else (int(x) for x in window.split(":"))
     out = {"file": os.path.basename(path), "kernel": kernel, "window": [a, b]}
     if "Counter_Name" in rows[0]:
         per_dispatch: dict = {}
@@ -408,16 +419,16 @@ def summarize_profile(path: str, kernel: str, window: str) -> dict:
         names = sorted(per_dispatch[ids[0]])
         out["dispatches"] = len(ids)
         out["window_dispatches"] = len(sel)
-        out["window_mean"] = {c: sum(per_dispatch[i][c] for i in sel) / len(sel) for c in names}
+        out["window_mean"] = {c: sum(per_dispatch[i][c] for i in sel) / len(sel) for c in names} if sel else None
         out["all_mean"] = {c: sum(per_dispatch[i][c] for i in ids) / len(ids) for c in names}
     else:
         dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
         sel = dur[a - 1:b]
         out["dispatches"] = len(dur)
         out["window_dispatches"] = len(sel)
-        out["window_avg_us"] = sum(sel) / len(sel)
-        out["window_min_us"] = min(sel)
-        out["window_max_us"] = max(sel)
+        out["window_avg_us"] = sum(sel) / len(sel) if sel else None
+        out["window_min_us"] = min(sel) if sel else None
+        out["window_max_us"] = max(sel) if sel else None
         out["all_avg_us"] = sum(dur) / len(dur)
     return out
 
@@ -447,14 +458,15 @@ def main() -> int:
     ap.add_argument("--summarize-profile", default="",
                     help="only summarise this rocprofv3 CSV (kernel trace or counter collection) over --kernel's "
                          "dispatches --window a:b, then exit (no GPU used)")
-    ap.add_argument("--kernel", default="k_solve_step")
+    ap.add_argument("--kernel", default="k_solve_step", help="kernel name prefix (several: comma-separated)")
     ap.add_argument("--window", default="", help="a:b, 1-based dispatches of --kernel (default: the timed window "
                                                  "of a run with these --steps/--warmup)")
     args = ap.parse_args()
     if args.summarize_profile:
         settle = settle_steps(args.steps, args.warmup)
         window = args.window or f"{settle + 1}:{settle + args.steps}"
-        print(json.dumps(summarize_profile(args.summarize_profile, args.kernel, window), indent=1))
+        res = [summarize_profile(args.summarize_profile, k, window) for k in args.kernel.split(",")]
+        print(json.dumps(res[0] if len(res) == 1 else res, indent=1))
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)", file=sys.stderr)
@@ -487,6 +499,7 @@ def main() -> int:
     torch.cuda.synchronize()
     settle_s = time.perf_counter() - s0
     hashes = {"settle": world.world_state_hash()}
+    _progress(f"pile: {settle} untimed steps in {settle_s:.2f} s")
     snap = world.save_state()
     world.reset_kernel_stats()
     world.set_kernel_timing(True, phases=False)  # launch events only: the step kernel's HIP-event time
@@ -509,6 +522,7 @@ def main() -> int:
     world.set_kernel_timing(False)
     ks_phases = world.kernel_stats()
     rerun_equal = world.world_state_hash() == hashes["end"]
+    _progress(f"pile: {args.steps} timed steps, {1e3 * elapsed / args.steps:.3f} ms per step")
     del snap
     world.close()  # release device memory now, not at interpreter exit
     n = scene.dynamic_count
@@ -578,14 +592,19 @@ def main() -> int:
         cb = cpu_baseline(scene, settle, args.steps, args.cpu_seconds, hashes)
         cb["gpu_over_cpu"] = value / cb["value"]
         out["cpu_baseline"] = cb
+        _progress(f"cpu baseline: {cb['value'] / 1e6:.2f} M body-substeps/s on {cb['cores']} threads")
     if args.batched_worlds > 0:
         out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps)
+        _progress(f"batched: {out['batched']['ms_per_step']:.3f} ms per step")
     if args.adjoint_worlds > 0:
         out["adjoint"] = adjoint_measurement(args.adjoint_worlds, args.adjoint_steps)
+        _progress("adjoint done")
     if args.large_world_steps > 0:
         out["large_world"] = large_world_measurement(args.large_world_steps)
+        _progress(f"large world: {out['large_world']['ms_per_step']:.3f} ms per step")
     if args.defaults_steps > 0:
         out["reference_defaults"] = defaults_measurement(args.defaults_steps, 0.4 * args.cpu_seconds)
+        _progress("reference defaults done")
     print(json.dumps(out))
     return 0
 
