@@ -282,3 +282,117 @@ def load_scene(scene) -> OracleWorld:
     for j in scene.joints:
         w.create_revolute_joint(dataclasses.replace(j, body_a=ids[j.body_a], body_b=ids[j.body_b]))
     return w
+
+
+# --- narrow phase -------------------------------------------------------------------------
+SHAPE_KINDS = {"box": 0, "sphere": 1, "capsule": 2}
+
+
+def shape_box(hx, hy, hz, p=(0.0, 0.0, 0.0), q=(0.0, 0.0, 0.0, 1.0)):
+    """A box hull (make_transformed_box_hull, hull.rs:754-852) placed by (p, q) in its shape frame."""
+    return ("box", np.array([hx, hy, hz, *p, *q], np.float32))
+
+
+def shape_sphere(center, radius):
+    return ("sphere", np.array([*center, radius], np.float32))
+
+
+def shape_capsule(c1, c2, radius):
+    return ("capsule", np.array([*c1, *c2, radius], np.float32))
+
+
+class NarrowCache:
+    """A SATCache (box-box) or SimplexCache (box-round) carried between calls, as the
+    reference's contact carries it. f = separation or metric; i = {type, index_a, index_b,
+    hit} or {count, index_a[4], index_b[4]}."""
+
+    def __init__(self):
+        self.f = np.zeros(1, np.float32)
+        self.i = np.zeros(9, np.int32)
+
+    @property
+    def sat(self) -> dict:
+        return {"separation": np.float32(self.f[0]), "type": int(self.i[0]), "index_a": int(self.i[1]),
+                "index_b": int(self.i[2]), "hit": int(self.i[3])}
+
+    @property
+    def simplex(self) -> dict:
+        return {"metric": np.float32(self.f[0]), "count": int(self.i[0]), "index_a": self.i[1:5].tolist(),
+                "index_b": self.i[5:9].tolist()}
+
+
+# SeparatingFeature (collision_types.rs:97-121)
+SAT_INVALID, SAT_BACKSIDE, SAT_FACE_A, SAT_FACE_B, SAT_EDGE = 0, 1, 2, 3, 4
+
+
+class Narrow:
+    """The oracle's narrow-phase routines (oracle/ref_narrow_capi.cpp) on plain arrays. `path`
+    selects the library: the oracle itself by default, or a build with one planted mistake."""
+
+    def __init__(self, path: str | None = None):
+        if path is None:
+            self.L = lib()
+        else:
+            self.L = ctypes.CDLL(path)
+        fp = ctypes.POINTER(ctypes.c_float)
+        ip = ctypes.POINTER(ctypes.c_int32)
+        self.L.nslo_collide_pair.argtypes = [ctypes.c_int, fp, ctypes.c_int, fp, fp, fp, ip, fp, ip]
+        self.L.nslo_separating_axis.argtypes = [fp, fp, fp, ctypes.c_int, fp, ip]
+        self.L.nslo_shape_distance.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, fp, ip, fp, ip]
+        self.L.nslo_box_hull.argtypes = [fp, fp, fp, ip, fp]
+        self._fp, self._ip = fp, ip
+
+    def collide(self, a, b, xf=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)), cache: NarrowCache | None = None) -> dict:
+        """The register table's routine for (a, b) with B placed in A's frame by xf = (p, q).
+        Returns normal, points [(point, separation, (owner1, index1, owner2, index2), feature id)]."""
+        cache = cache or NarrowCache()
+        t = np.array([*xf[0], *xf[1]], np.float32)
+        of = np.zeros(19, np.float32)
+        oi = np.zeros(21, np.int32)
+        fp, ip = self._fp, self._ip
+        rc = self.L.nslo_collide_pair(SHAPE_KINDS[a[0]], a[1].ctypes.data_as(fp), SHAPE_KINDS[b[0]], b[1].ctypes.data_as(fp),
+                                      t.ctypes.data_as(fp), cache.f.ctypes.data_as(fp), cache.i.ctypes.data_as(ip),
+                                      of.ctypes.data_as(fp), oi.ctypes.data_as(ip))
+        if rc != 0:
+            raise ValueError(f"no collide routine for ({a[0]}, {b[0]})")
+        n = int(oi[0])
+        pts = [(of[3 + 4 * k:6 + 4 * k].copy(), np.float32(of[6 + 4 * k]), tuple(int(x) for x in oi[1 + 4 * k:5 + 4 * k]),
+                int(np.uint32(oi[17 + k]))) for k in range(n)]
+        return {"normal": of[0:3].copy(), "points": pts, "cache": cache}
+
+    def separating_axis(self, a, b, xf, early_return: bool = True) -> dict:
+        """compute_separating_axis + get_best_axis for two box hulls."""
+        t = np.array([*xf[0], *xf[1]], np.float32)
+        of = np.zeros(7, np.float32)
+        oi = np.zeros(4, np.int32)
+        fp, ip = self._fp, self._ip
+        self.L.nslo_separating_axis(a[1].ctypes.data_as(fp), b[1].ctypes.data_as(fp), t.ctypes.data_as(fp), int(early_return),
+                                    of.ctypes.data_as(fp), oi.ctypes.data_as(ip))
+        return {"normal": of[0:3].copy(), "separation": np.float32(of[3]), "face_a": np.float32(of[4]),
+                "face_b": np.float32(of[5]), "edge": np.float32(of[6]), "kind": int(oi[0]), "index_a": int(oi[1]),
+                "index_b": int(oi[2]), "separated": int(oi[3])}
+
+    def distance(self, points_a, points_b, xf, cache: NarrowCache | None = None) -> dict:
+        """shape_distance (GJK, use_radii false) between two point clouds, B placed by xf."""
+        cache = cache or NarrowCache()
+        pa = np.ascontiguousarray(points_a, np.float32).reshape(-1, 3)
+        pb = np.ascontiguousarray(points_b, np.float32).reshape(-1, 3)
+        t = np.array([*xf[0], *xf[1]], np.float32)
+        of = np.zeros(10, np.float32)
+        oi = np.zeros(1, np.int32)
+        fp, ip = self._fp, self._ip
+        self.L.nslo_shape_distance(pa.ctypes.data_as(fp), len(pa), pb.ctypes.data_as(fp), len(pb), t.ctypes.data_as(fp),
+                                   cache.f.ctypes.data_as(fp), cache.i.ctypes.data_as(ip), of.ctypes.data_as(fp),
+                                   oi.ctypes.data_as(ip))
+        return {"point_a": of[0:3].copy(), "point_b": of[3:6].copy(), "normal": of[6:9].copy(), "distance": np.float32(of[9]),
+                "iterations": int(oi[0]), "cache": cache}
+
+    def box_hull(self, a) -> dict:
+        pts = np.zeros((8, 3), np.float32)
+        planes = np.zeros((6, 4), np.float32)
+        edges = np.zeros((24, 4), np.int32)
+        center = np.zeros(3, np.float32)
+        fp, ip = self._fp, self._ip
+        self.L.nslo_box_hull(a[1].ctypes.data_as(fp), pts.ctypes.data_as(fp), planes.ctypes.data_as(fp),
+                             edges.ctypes.data_as(ip), center.ctypes.data_as(fp))
+        return {"points": pts, "planes": planes, "edges": edges, "center": center}

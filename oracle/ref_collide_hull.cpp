@@ -4,6 +4,7 @@
 // Restates src/collision/convex_manifold.rs (separating-axis test with the cached
 // axis, face clipping, edge contacts, manifold reduction) and the clipping helpers of
 // src/collision/manifold.rs, operation for operation.
+#include "ref_plant.h"
 #include "ref_world.h"
 
 namespace nslref {
@@ -16,8 +17,6 @@ constexpr uint32_t kHullBitCount = 7;      // HULL_BIT_COUNT
 constexpr int kMaxClipPoints = 64;         // manifold.rs MAX_CLIP_POINTS
 
 struct ClipVertex { Vec3 position; float separation; FeaturePair pair; };
-struct SepAxis { Vec3 normal; float separation; int32_t index_a, index_b; uint8_t kind; };
-struct AxisQuery { SepAxis face_a, face_b, edge; uint8_t separated; };
 
 inline FeaturePair make_pair(int o1, int i1, int o2, int i2) {
   return FeaturePair{(uint8_t)o1, (uint8_t)i1, (uint8_t)o2, (uint8_t)i2};
@@ -273,7 +272,7 @@ AxisQuery compute_separating_axis(const Hull& ha, const Hull& hb, Xf xf_b, bool 
       if (adc * bdc >= eps || cba * bdc >= eps) continue;
       float max_cd = maxf(cba * cba, dba * dba);
       if (max_cd <= a_tol[i]) continue;
-      float t_ = -cba / (dba - cba);
+      float t_ = PLANTED(8) ? cba / (dba - cba) : -cba / (dba - cba);
       Vec3 n = mul_add(c, t_, sub(dn, c));
       float len2 = dot(n, n);
       float inv = 1.0f / std::sqrt(len2);
@@ -359,7 +358,7 @@ void reduce_manifold_points(LocalManifold& m, LocalPoint* points, int count) {
 
   best_score = tol_sqr;
   best_index = kNullIndex;
-  float sign = best_signed_area < 0.0f ? -1.0f : 1.0f;
+  float sign = (best_signed_area < 0.0f) != PLANTED(9) ? -1.0f : 1.0f;
   for (int i = 0; i < count; ++i) {
     Vec3 p = points[i].point;
     float u1 = sign * dot(normal, cross(sub(p, a), ba));
@@ -530,7 +529,7 @@ bool build_edge_contact(LocalManifold& m, const Hull& ha, const Hull& hb, Xf b_t
   SegDist r = line_distance(p_a, e_a, p_b, e_b);
   if (!within_segments(r)) { cache = SATCache{}; return false; }
   float separation = dot(q.normal, sub(r.p2, r.p1));
-  Vec3 point = scale(0.5f, add(r.p1, r.p2));
+  Vec3 point = PLANTED(7) ? r.p1 : scale(0.5f, add(r.p1, r.p2));
   m.normal = q.normal;
   m.point_count = 1;
   m.pts[0].point = point;
@@ -554,6 +553,10 @@ int find_support_vertex(const Hull& h, Vec3 d) {
 }
 
 }  // namespace
+
+AxisQuery separating_axis_query(const Hull& ha, const Hull& hb, Xf xf_b, bool early_return) {
+  return compute_separating_axis(ha, hb, xf_b, early_return);
+}
 
 uint32_t make_feature_id(FeaturePair p) {
   return ((uint32_t)p.owner1 << 24) | ((uint32_t)p.index1 << 16) | ((uint32_t)p.owner2 << 8) | (uint32_t)p.index2;
@@ -623,7 +626,11 @@ void collide_hulls(LocalManifold& m, const Hull& ha, const Hull& hb, Xf b_to_a, 
           SepAxis eq{neg(axis), 0.0f, cache.index_a, cache.index_b, kEdgePairAxis};
           SATCache local{};
           bool touching = build_edge_contact(m, ha, hb, b_to_a, eq, local);
-          if (touching && absf(cache.separation - local.separation) < ls) { cache.hit = 1; if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED); return; }
+          if (touching && (PLANTED(10) ? absf(cache.separation - local.separation) > ls : absf(cache.separation - local.separation) < ls)) {
+            cache.hit = 1;
+            if (sat_hits) __atomic_fetch_add(sat_hits, 1, __ATOMIC_RELAXED);
+            return;
+          }
         }
       }
       break;

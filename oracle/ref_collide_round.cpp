@@ -6,6 +6,7 @@
 // hull-capsule queries and builders, collide_hull_and_capsule),
 // src/collision/convex_manifold.rs:137-200 (clip_segment, clip_segment_to_hull_face)
 // and src/collision/distance.rs:11-709 (GJK shape_distance with its simplex cache).
+#include "ref_plant.h"
 #include "ref_world.h"
 
 namespace nslref {
@@ -168,6 +169,11 @@ void witness_points(const Simplex& s, Vec3* a, Vec3* b) {
   switch (s.count) {
     case 1: *a = v[0].wa; *b = v[0].wb; break;
     case 2:
+      if (PLANTED(4)) {
+        *a = blend2(v[1].a, v[0].wa, v[0].a, v[1].wa);
+        *b = blend2(v[1].a, v[0].wb, v[0].a, v[1].wb);
+        break;
+      }
       *a = blend2(v[0].a, v[0].wa, v[1].a, v[1].wa);
       *b = blend2(v[0].a, v[0].wb, v[1].a, v[1].wb);
       break;
@@ -350,7 +356,8 @@ void collide_spheres(LocalManifold& m, Sphere a, Sphere b, Xf b_to_a) {
   Vec3 normal = {0.0f, 1.0f, 0.0f};
   float d = std::sqrt(dsq);
   if (d * d > 1000.0f * FLT_MIN) normal = scale(1.0f / d, offset);
-  Vec3 point = scale(0.5f, mul_sub(add(mul_add(c1, a.radius, normal), c2), b.radius, normal));
+  Vec3 point = scale(0.5f, mul_sub(add(PLANTED(1) ? mul_sub(c1, a.radius, normal) : mul_add(c1, a.radius, normal), c2),
+                                    b.radius, normal));
   m.normal = normal;
   m.point_count = 1;
   set_point(m, 0, point, d - total, kPairSingle);
@@ -372,7 +379,7 @@ void collide_capsule_and_sphere(LocalManifold& m, Capsule a, Sphere b, Xf b_to_a
   Vec3 point = scale(0.5f, mul_add(add(mul_sub(center, b.radius, normal), closest), a.radius, normal));
   m.normal = normal;
   m.point_count = 1;
-  set_point(m, 0, point, d - total, kPairSingle);
+  set_point(m, 0, point, PLANTED(2) ? d - b.radius : d - total, kPairSingle);
 }
 
 // convex_manifold_shapes.rs:132-240
@@ -406,7 +413,7 @@ void collide_hull_and_sphere(LocalManifold& m, const Hull& a, Sphere b, Xf b_to_
     Vec3 c_b = mul_sub(center, radius_b, normal);
     m.normal = normal;
     m.point_count = 1;
-    set_point(m, 0, lerp(c_a, c_b, 0.5f), best_d - radius, kPairSingle);
+    set_point(m, 0, lerp(c_a, c_b, 0.5f), PLANTED(5) ? best_d + radius : best_d - radius, kPairSingle);
   }
 }
 
@@ -456,7 +463,7 @@ void collide_capsules(LocalManifold& m, Capsule a, Capsule b, Xf b_to_a) {
       float d1 = distance(cp1, vb[0].position), d2 = distance(cp2, vb[1].position);
       if (d1 <= radius && d2 <= radius) {
         if (d1 < min_d || d2 < min_d) return;
-        Vec3 n1 = scale(1.0f / d1, sub(vb[0].position, cp1));
+        Vec3 n1 = scale(1.0f / d1, PLANTED(3) ? sub(cp1, vb[0].position) : sub(vb[0].position, cp1));
         Vec3 n2 = scale(1.0f / d2, sub(vb[1].position, cp2));
         Vec3 normal = normalize(add(n1, n2));
         Vec3 p1 = scale(0.5f, mul_sub(add(mul_add(vb[0].position, a.radius, n1), cp1), b.radius, normal));
@@ -589,8 +596,10 @@ void collide_hull_and_capsule(LocalManifold& m, const Hull& h, Capsule c, Xf b_t
           Vec3 normal = ref_plane.n;
           m.normal = normal;
           m.point_count = 2;
-          set_point(m, 0, mul_sub(vb[0].position, 0.5f * (c.radius + d1), normal), d1 - c.radius, vb[0].pair);
-          set_point(m, 1, mul_sub(vb[1].position, 0.5f * (c.radius + d2), normal), d2 - c.radius, vb[1].pair);
+          set_point(m, 0, mul_sub(vb[0].position, 0.5f * (PLANTED(6) ? c.radius - d1 : c.radius + d1), normal), d1 - c.radius,
+                    vb[0].pair);
+          set_point(m, 1, mul_sub(vb[1].position, 0.5f * (PLANTED(6) ? c.radius - d2 : c.radius + d2), normal), d2 - c.radius,
+                    vb[1].pair);
           return;
         }
       }

@@ -11,6 +11,7 @@
 // Sources: src/math/functions.rs, src/math/types.rs, src/collision/internal_math.rs,
 // src/base/constants.rs.
 #pragma once
+#include "ref_plant.h"
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -382,7 +383,7 @@ inline SegDist segment_distance(Vec3 p1, Vec3 q1, Vec3 p2, Vec3 q2) {
     s1 = clampf(-c / a, 0.0f, 1.0f);
     s2 = 0.0f;
   } else if (s2 > 1.0f) {
-    s1 = clampf((b - c) / a, 0.0f, 1.0f);
+    s1 = clampf(PLANTED(11) ? (b + c) / a : (b - c) / a, 0.0f, 1.0f);
     s2 = 1.0f;
   }
   r.p1 = mul_add(p1, s1, d1); r.f1 = s1; r.p2 = mul_add(p2, s2, d2); r.f2 = s2;
@@ -395,7 +396,7 @@ inline Vec3 point_to_segment(Vec3 a, Vec3 b, Vec3 q) {
   float alpha = dot(ab, aq);
   if (alpha <= 0.0f) return a;
   float den = dot(ab, ab);
-  if (alpha > den) return b;
+  if (alpha > den) return PLANTED(12) ? a : b;
   alpha /= den;
   return mul_add(a, alpha, ab);
 }

@@ -536,6 +536,11 @@ class World {
 
 // narrow phase (src/collision/convex_manifold.rs, convex_manifold_shapes.rs)
 void collide_hulls(LocalManifold& m, const Hull& a, const Hull& b, Xf b_to_a, SATCache& cache, int64_t* sat_calls, int64_t* sat_hits);
+// compute_separating_axis (convex_manifold.rs:392-899): the best face axis of A, of B and
+// edge pair, B in A's frame (AxisQuery, manifold.rs)
+struct SepAxis { Vec3 normal; float separation; int32_t index_a, index_b; uint8_t kind; };
+struct AxisQuery { SepAxis face_a, face_b, edge; uint8_t separated; };
+AxisQuery separating_axis_query(const Hull& ha, const Hull& hb, Xf xf_b, bool early_return);
 void collide_spheres(LocalManifold& m, Sphere a, Sphere b, Xf b_to_a);
 void collide_capsule_and_sphere(LocalManifold& m, Capsule a, Sphere b, Xf b_to_a);
 void collide_capsules(LocalManifold& m, Capsule a, Capsule b, Xf b_to_a);
