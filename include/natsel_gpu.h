@@ -4,13 +4,15 @@
  * below stands in for one World method so the GPU stepper is a drop-in for the CPU
  * stepper (citations are to the reference sources):
  *
- *   nsl_world_create / destroy     World::new / Drop            src/simulation/world_runtime.rs:1162, :1799
+ *   nsl_world_create / destroy     World::new / drop of the World  src/simulation/world_runtime.rs:1162
  *   nsl_world_set_options          public World fields          src/simulation/world_runtime.rs:1243-1304
  *   nsl_create_body                World::create_body           src/simulation/world_runtime.rs:1506
+ *   nsl_default_body_def           BodyDef::default             src/types/body_def.rs:80-104
+ *   nsl_default_material           ShapeDef::default            src/types/shape_def.rs:114-135
  *   nsl_create_box                 create_hull_shape(make_box_hull)  src/simulation/shape_creation.rs:559,
  *                                                               src/collision/hull.rs:854
  *   nsl_create_sphere              World::create_sphere_shape   src/simulation/shape_creation.rs:504
- *   nsl_create_capsule             World::create_capsule_shape  src/simulation/shape_creation.rs:523
+ *   nsl_create_capsule             World::create_capsule_shape  src/simulation/shape_creation.rs:522
  *   nsl_world_step                 World::step(dt, substeps)    src/simulation/solver_integration.rs:807
  *   nsl_world_get_bodies           body_transform / body_linear_velocity / body_angular_velocity
  *                                                               src/simulation/body_api.rs:186-262
@@ -22,6 +24,10 @@
  *   nsl_world_island_summary       island pool and solver-set counts (island_split_tests.rs:46-117)
  *   nsl_world_check_islands        the reference's index validation helpers
  *   nsl_time_of_impact             time_of_impact               src/collision/time_of_impact.rs:406-598
+ *   nsl_collide                    collide_hulls / collide_spheres / collide_capsule_and_sphere /
+ *                                  collide_capsules / collide_hull_and_sphere / collide_hull_and_capsule
+ *                                                               src/collision/convex_manifold.rs:1419-1437,
+ *                                                               src/collision/convex_manifold_shapes.rs:21-797
  *   nsl_world_get_profile          World::profile               src/types/world_def.rs:158-184
  *   nsl_world_*kernel_*            per-kernel device timing (no counterpart: the reference times phases
  *                                  on the host, solver_stages.rs:551-586)
@@ -84,10 +90,36 @@ typedef struct nsl_body_def {
   int32_t is_bullet;                 /* swept against every body type, after the other fast bodies */
   float safety_factor;               /* 0.5: fast (swept) when a step moves it more than this times
                                         its smallest extent (body_def.rs:68, continuous.rs:121-151) */
+  float sleep_threshold;             /* 0.05 m/s: sleep velocity threshold (body_def.rs:67) */
+  int32_t motion_locks;              /* NSL_LOCK_* bits (MotionLocks, body_def.rs:24-33) */
+  int32_t is_awake;                  /* 1. A dynamic or kinematic body with is_awake 0 and enable_sleep 1
+                                        would start in a sleeping set of its own: refused (NSL_ERR_INVALID) */
+  int32_t is_enabled;                /* 1. 0 (the disabled set) is refused (NSL_ERR_INVALID) */
+  int32_t allow_fast_rotation;       /* 0: the angular speed clamp applies (solver_integration.rs:377-426) */
+  int32_t enable_contact_recycling;  /* 1: contacts of this body may skip the narrow phase when their
+                                        relative pose barely moved (collide_pass.rs:606-687) */
 } nsl_body_def;
 
+/* MotionLocks as body flag bits (world_runtime.rs:87-92). */
+#define NSL_LOCK_LINEAR_X 0x1
+#define NSL_LOCK_LINEAR_Y 0x2
+#define NSL_LOCK_LINEAR_Z 0x4
+#define NSL_LOCK_ANGULAR_X 0x8
+#define NSL_LOCK_ANGULAR_Y 0x10
+#define NSL_LOCK_ANGULAR_Z 0x20
+
+/* A shape's ShapeDef (src/types/shape_def.rs:90-135) with its base SurfaceMaterial. Fill it with
+   nsl_default_material before setting fields: a zero mask filters every pair out. */
 typedef struct nsl_material {
-  float density, friction, restitution, rolling_resistance;
+  float density, friction, restitution, rolling_resistance; /* 1000, 0.6, 0, 0 */
+  uint64_t filter_category_bits;      /* 1 (Filter, shape_def.rs:18-24; shape_runtime.rs:285-292) */
+  uint64_t filter_mask_bits;          /* every bit */
+  int32_t filter_group_index;         /* 0 */
+  int32_t is_sensor;                  /* 0. A sensor creates no contacts and takes no part in
+                                         continuous collision (contact_creation.rs:187,
+                                         continuous.rs:398-401); sensor events are not produced */
+  int32_t enable_speculative_contact; /* 1 */
+  int32_t enable_hit_events;          /* 0 (flag kept on the contact; events are not produced) */
 } nsl_material;
 
 typedef struct nsl_profile {
@@ -167,10 +199,67 @@ void nsl_default_options(nsl_options* out);
 void nsl_default_revolute_joint_def(nsl_revolute_joint_def* out);
 /* BodyDef::default (src/types/body_def.rs:80-104): a static body at the origin. */
 void nsl_default_body_def(nsl_body_def* out);
+/* ShapeDef::default (src/types/shape_def.rs:114-135) with SurfaceMaterial::default_material. */
+void nsl_default_material(nsl_material* out);
 
 /* time_of_impact (src/collision/time_of_impact.rs:406-598) for n queries on the device, one
    lane each; host arrays in and out. */
 int nsl_time_of_impact(int device, const nsl_toi_input* in, nsl_toi_output* out, int32_t n);
+
+/* ---------------------------------------------------------------------------------
+ * The narrow phase on its own. Each of the reference's collide routines takes two shapes,
+ * the transform of B in A's frame, a caller buffer for the manifold and the contact's cache
+ * (collide_hulls, convex_manifold.rs:1419-1437; collide_spheres, collide_capsule_and_sphere,
+ * collide_capsules, collide_hull_and_sphere, collide_hull_and_capsule,
+ * convex_manifold_shapes.rs:21-797). nsl_collide runs n such calls on the device, one lane
+ * each, in the pair order of the register table (contact_creation.rs:88-102): hull-hull,
+ * hull-sphere, hull-capsule, capsule-sphere, capsule-capsule, sphere-sphere. The results
+ * equal the step's own narrow phase bit for bit. */
+#define NSL_SHAPE_CAPSULE 0 /* ShapeType values (src/types/shape_def.rs:68-82) */
+#define NSL_SHAPE_HULL 3
+#define NSL_SHAPE_SPHERE 5
+
+typedef struct nsl_shape_geometry {
+  int32_t type;    /* NSL_SHAPE_* */
+  float params[7]; /* hull (a box): half extents hx, hy, hz; sphere: centre xyz, radius;
+                      capsule: centre1 xyz, centre2 xyz, radius */
+  float frame[7];  /* hull only: where the box sits in its shape frame, p xyz and q xyzs
+                      (make_transformed_box_hull, src/collision/hull.rs:754-852) */
+} nsl_shape_geometry;
+
+/* The cache a contact carries from call to call, zeroed when empty: SATCache for hull-hull
+   (src/types/collision_types.rs:97-121: separation, SeparatingFeature type, index_a,
+   index_b, hit), SimplexCache for hull-sphere and hull-capsule (src/types/queries.rs: metric,
+   count, index_a[0..count), index_b[0..count)); unused by the other pairs. */
+typedef struct nsl_collide_cache {
+  float value;                    /* SATCache.separation or SimplexCache.metric */
+  int32_t kind;                   /* SATCache.type or SimplexCache.count (0..3) */
+  uint8_t index_a[4], index_b[4]; /* SATCache: [0] only */
+  int32_t hit;                    /* SATCache.hit (set by the call) */
+} nsl_collide_cache;
+
+typedef struct nsl_collide_query {
+  nsl_shape_geometry a, b;
+  float transform_b_to_a[7]; /* p xyz, q xyzs */
+  nsl_collide_cache cache;
+} nsl_collide_query;
+
+/* LocalManifold with capacity 4 (MAX_MANIFOLD_POINTS): normal and points in A's frame,
+   separations, feature ids (src/collision/manifold.rs:58-94), and the cache after the call. */
+typedef struct nsl_local_manifold {
+  float normal[3];
+  int32_t point_count;
+  float point[4][3];
+  float separation[4];
+  uint32_t feature_id[4];
+  nsl_collide_cache cache;
+} nsl_local_manifold;
+
+/* n collide calls on the device; host arrays in and out. NSL_ERR_INVALID for a pair outside
+   the register table's order or a cache whose indices do not fit its shapes (SAT face index
+   above 5, SAT edge index odd or above 22, simplex count above 3 or a vertex index beyond the
+   shape's support points). */
+int nsl_collide(int device, const nsl_collide_query* queries, nsl_local_manifold* out, int32_t n);
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out);
 int nsl_world_destroy(nsl_world* world);

@@ -14,7 +14,8 @@ import numpy as np
 
 from . import capi
 from .scenes import BOX, SPHERE, Scene
-from .world import BodyDef, BoxHull, Capsule, ShapeDef, Sphere, WorldOptions, _material, c_options, make_box_hull
+from .world import (BodyDef, BoxHull, Capsule, ShapeDef, Sphere, WorldOptions, _material, body_def_of, c_body_def, c_options,
+                    make_box_hull, shape_def_of)
 
 
 class BatchWorld:
@@ -25,16 +26,7 @@ class BatchWorld:
         self.index = index
 
     def create_body(self, d: BodyDef) -> int:
-        c = capi.BodyDef()
-        c.type = d.type
-        c.position[:] = d.position
-        c.rotation[:] = d.rotation
-        c.linear_velocity[:] = d.linear_velocity
-        c.angular_velocity[:] = d.angular_velocity
-        c.linear_damping, c.angular_damping, c.gravity_scale = d.linear_damping, d.angular_damping, d.gravity_scale
-        c.enable_sleep = int(d.enable_sleep)
-        c.is_bullet = int(d.is_bullet)
-        c.safety_factor = d.safety_factor
+        c = c_body_def(d)
         out = ctypes.c_int32()
         capi.check(self._b._lib.nsl_batch_create_body(self._b._h, self.index, ctypes.byref(c), ctypes.byref(out)))
         return out.value
@@ -202,11 +194,9 @@ def load_batch(scene_list: list[Scene], device: int = 0) -> Batch:
                                    enable_restitution_propagation=first.restitution_propagation))
     for scene in scene_list:
         w = b.add_world()
-        ids = [w.create_body(BodyDef(s.body_type, s.position, s.rotation, s.linear_velocity, s.angular_velocity, s.linear_damping,
-                                     s.angular_damping, s.gravity_scale, s.enable_sleep, s.is_bullet, s.safety_factor))
-               for s in scene.bodies]
+        ids = [w.create_body(body_def_of(s)) for s in scene.bodies]
         for s in scene.shapes:
-            d = ShapeDef(s.density, s.friction, s.restitution)
+            d = shape_def_of(s)
             if s.kind == BOX:
                 w.create_hull_shape(ids[s.body], d, make_box_hull(*s.params))
             elif s.kind == SPHERE:

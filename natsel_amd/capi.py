@@ -44,7 +44,17 @@ class BodyDef(ctypes.Structure):
         ("enable_sleep", ctypes.c_int32),
         ("is_bullet", ctypes.c_int32),
         ("safety_factor", ctypes.c_float),
+        ("sleep_threshold", ctypes.c_float),
+        ("motion_locks", ctypes.c_int32),
+        ("is_awake", ctypes.c_int32),
+        ("is_enabled", ctypes.c_int32),
+        ("allow_fast_rotation", ctypes.c_int32),
+        ("enable_contact_recycling", ctypes.c_int32),
     ]
+
+
+LOCK_LINEAR_X, LOCK_LINEAR_Y, LOCK_LINEAR_Z = 0x1, 0x2, 0x4  # MotionLocks bits (world_runtime.rs:87-92)
+LOCK_ANGULAR_X, LOCK_ANGULAR_Y, LOCK_ANGULAR_Z = 0x8, 0x10, 0x20
 
 
 class Material(ctypes.Structure):
@@ -53,6 +63,12 @@ class Material(ctypes.Structure):
         ("friction", ctypes.c_float),
         ("restitution", ctypes.c_float),
         ("rolling_resistance", ctypes.c_float),
+        ("filter_category_bits", ctypes.c_uint64),
+        ("filter_mask_bits", ctypes.c_uint64),
+        ("filter_group_index", ctypes.c_int32),
+        ("is_sensor", ctypes.c_int32),
+        ("enable_speculative_contact", ctypes.c_int32),
+        ("enable_hit_events", ctypes.c_int32),
     ]
 
 
@@ -132,6 +148,28 @@ class ToiOutput(ctypes.Structure):
     ]
 
 
+SHAPE_CAPSULE, SHAPE_HULL, SHAPE_SPHERE = 0, 3, 5  # ShapeType (shape_def.rs:68-82)
+
+
+class ShapeGeometry(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("params", ctypes.c_float * 7), ("frame", ctypes.c_float * 7)]
+
+
+class CollideCache(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_float), ("kind", ctypes.c_int32), ("index_a", ctypes.c_uint8 * 4),
+                ("index_b", ctypes.c_uint8 * 4), ("hit", ctypes.c_int32)]
+
+
+class CollideQuery(ctypes.Structure):
+    _fields_ = [("a", ShapeGeometry), ("b", ShapeGeometry), ("transform_b_to_a", ctypes.c_float * 7),
+                ("cache", CollideCache)]
+
+
+class LocalManifold(ctypes.Structure):
+    _fields_ = [("normal", ctypes.c_float * 3), ("point_count", ctypes.c_int32), ("point", (ctypes.c_float * 3) * 4),
+                ("separation", ctypes.c_float * 4), ("feature_id", ctypes.c_uint32 * 4), ("cache", CollideCache)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [
         ("last_step_ms", ctypes.c_double),
@@ -166,7 +204,9 @@ SIGNATURES = {
     ),
     "nsl_default_revolute_joint_def": (None, [ctypes.POINTER(RevoluteJointDef)]),
     "nsl_default_body_def": (None, [ctypes.POINTER(BodyDef)]),
+    "nsl_default_material": (None, [ctypes.POINTER(Material)]),
     "nsl_time_of_impact": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ToiInput), ctypes.POINTER(ToiOutput), _I]),
+    "nsl_collide": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CollideQuery), ctypes.POINTER(LocalManifold), _I]),
     "nsl_create_revolute_joint": (ctypes.c_int, [_P, ctypes.POINTER(RevoluteJointDef), ctypes.POINTER(_I)]),
     "nsl_world_get_joint": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_I)]),
     "nsl_world_joint_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),

@@ -164,7 +164,7 @@ __device__ void create_contact(const WorldRefs& r, int32_t a, int32_t b, bool en
   c.enc_a = ba.enc;
   c.enc_b = bb.enc;
   if (ba.type == kBodyStatic || bb.type == kBodyStatic) c.flags |= cflag::kStatic;
-  if (enable_speculative && pa.pad && pb.pad) c.flags |= cflag::kSpeculative;
+  if (enable_speculative && (pa.flags & kProxySpeculative) && (pb.flags & kProxySpeculative)) c.flags |= cflag::kSpeculative;
   if ((ba.flags & bflag::kEnableRecycling) && (bb.flags & bflag::kEnableRecycling) && (c.flags & cflag::kSpeculative))
     c.flags |= cflag::kRecycle;
   c.rolling_resistance = maxf(sa.rolling_resistance, sb.rolling_resistance) * maxf(sa.radius, sb.radius);
@@ -278,6 +278,7 @@ __device__ __forceinline__ bool new_pair(const WorldRefs& r, int m, const Proxy&
   const Proxy pj = r.proxy[j];
   if (pm.body_type != kBodyDynamic && pj.body_type != kBodyDynamic) return false;
   if (pm.body == pj.body) return false;
+  if ((pm.flags | pj.flags) & kProxySensor) return false;  // contact_creation.rs:187
   if (!aabb_overlaps(fm, r.fat[j])) return false;
   if (pm.group == pj.group && pm.group != 0) {
     if (pm.group < 0) return false;
@@ -783,7 +784,8 @@ void BatchStepper::upload() {
       fat.push_back(hs.fat);
       aabbs.push_back(hs.aabb);
       moved.push_back(hs.proxy_moved ? 1 : 0);
-      proxies.push_back(Proxy{hs.body, sc.bodies_[hs.body].type, hs.group, hs.speculative ? 1 : 0, hs.category, hs.mask});
+      proxies.push_back(Proxy{hs.body, sc.bodies_[hs.body].type, hs.group,
+                              (hs.speculative ? kProxySpeculative : 0) | (hs.sensor ? kProxySensor : 0), hs.category, hs.mask});
     }
     hulls.insert(hulls.end(), sc.hulls_.begin(), sc.hulls_.end());
     // capacities: a pile keeps a few contacts per shape; the overflow colour is rare

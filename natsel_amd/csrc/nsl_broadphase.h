@@ -21,10 +21,12 @@ struct Proxy {
   int32_t body;
   int32_t body_type;
   int32_t group;
-  int32_t pad;
+  int32_t flags;  // kProxy*
   uint64_t category;
   uint64_t mask;
 };
+constexpr int32_t kProxySpeculative = 0x1;  // the shape's enable_speculative_contact
+constexpr int32_t kProxySensor = 0x2;       // a sensor: no contacts, no continuous collision
 
 class PairFinder {
  public:

@@ -90,6 +90,7 @@ __global__ void k_bp_test(int n, const int64_t* offsets, const int32_t* ids, con
     const Proxy pa = proxies[a], pb = proxies[b];
     if (pa.body_type != kBodyDynamic && pb.body_type != kBodyDynamic) continue;
     if (pa.body == pb.body) continue;
+    if ((pa.flags | pb.flags) & kProxySensor) continue;  // contact_creation.rs:187
     if (!aabb_overlaps(fat[a], fat[b])) continue;
     if (pa.group == pb.group && pa.group != 0) {  // shape_runtime.rs:285-292
       if (pa.group < 0) continue;

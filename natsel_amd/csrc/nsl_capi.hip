@@ -9,6 +9,7 @@
 
 #include "../../include/natsel_gpu.h"
 #include "nsl_batch.h"
+#include "nsl_narrow.h"
 #include "nsl_world.h"
 
 struct nsl_world {
@@ -67,6 +68,12 @@ nsl::Material material(const nsl_material* m) {
     out.friction = m->friction;
     out.restitution = m->restitution;
     out.rolling_resistance = m->rolling_resistance;
+    out.category_bits = m->filter_category_bits;
+    out.mask_bits = m->filter_mask_bits;
+    out.group_index = m->filter_group_index;
+    out.is_sensor = m->is_sensor != 0;
+    out.enable_speculative = m->enable_speculative_contact != 0;
+    out.enable_hit_events = m->enable_hit_events != 0;
   }
   return out;
 }
@@ -86,6 +93,15 @@ nsl::BodyDesc body_desc(const nsl_body_def* def) {
   d.enable_sleep = def->enable_sleep != 0;
   d.is_bullet = def->is_bullet != 0;
   d.safety_factor = def->safety_factor;
+  d.sleep_threshold = def->sleep_threshold;
+  d.lock_flags = (uint32_t)def->motion_locks & 0x3Fu;
+  d.allow_fast_rotation = def->allow_fast_rotation != 0;
+  d.enable_contact_recycling = def->enable_contact_recycling != 0;
+  // world_runtime.rs:1517-1540: a body that is not enabled goes to the disabled set, one that
+  // is not awake (and may sleep) to a sleeping set of its own; neither is built here
+  if (!def->is_enabled) throw std::invalid_argument("a body must be enabled (is_enabled = 0 is not supported)");
+  if (def->type != NSL_BODY_STATIC && !def->is_awake && def->enable_sleep)
+    throw std::invalid_argument("a body must start awake (is_awake = 0 with enable_sleep is not supported)");
   return d;
 }
 
@@ -201,6 +217,28 @@ void nsl_default_body_def(nsl_body_def* out) {
   out->enable_sleep = d.enable_sleep ? 1 : 0;
   out->is_bullet = d.is_bullet ? 1 : 0;
   out->safety_factor = d.safety_factor;
+  out->sleep_threshold = d.sleep_threshold;
+  out->motion_locks = (int32_t)d.lock_flags;
+  out->is_awake = 1;
+  out->is_enabled = 1;
+  out->allow_fast_rotation = d.allow_fast_rotation ? 1 : 0;
+  out->enable_contact_recycling = d.enable_contact_recycling ? 1 : 0;
+}
+
+void nsl_default_material(nsl_material* out) {
+  if (!out) return;
+  const nsl::Material m;
+  std::memset(out, 0, sizeof(*out));
+  out->density = m.density;
+  out->friction = m.friction;
+  out->restitution = m.restitution;
+  out->rolling_resistance = m.rolling_resistance;
+  out->filter_category_bits = m.category_bits;
+  out->filter_mask_bits = m.mask_bits;
+  out->filter_group_index = m.group_index;
+  out->is_sensor = m.is_sensor ? 1 : 0;
+  out->enable_speculative_contact = m.enable_speculative ? 1 : 0;
+  out->enable_hit_events = m.enable_hit_events ? 1 : 0;
 }
 
 static_assert(sizeof(nsl_toi_input) == sizeof(nsl::ToiQuery), "toi input layout");
@@ -208,6 +246,35 @@ static_assert(offsetof(nsl_toi_input, sweep_a) == offsetof(nsl::ToiQuery, sweep_
 static_assert(offsetof(nsl_toi_input, max_fraction) == offsetof(nsl::ToiQuery, max_fraction), "toi input layout");
 static_assert(sizeof(nsl_toi_output) == sizeof(nsl::ToiOutput), "toi output layout");
 static_assert(offsetof(nsl_toi_output, root_iterations) == offsetof(nsl::ToiOutput, root_iterations), "toi output layout");
+
+int nsl_collide(int device, const nsl_collide_query* queries, nsl_local_manifold* out, int32_t n) {
+  if (n < 0 || (n > 0 && (!queries || !out))) return fail(NSL_ERR_INVALID, "null argument or negative count");
+  int32_t bad = -1;
+  if (const char* why = nsl::validate_collide_queries(queries, n, &bad)) {
+    g_error = "query " + std::to_string(bad) + ": " + why;
+    return NSL_ERR_INVALID;
+  }
+  if (n == 0) return NSL_OK;
+  return guarded([&] {
+    auto check = [](hipError_t e) {
+      if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+    };
+    check(hipSetDevice(device));
+    nsl_collide_query* dq = nullptr;
+    nsl_local_manifold* dout = nullptr;
+    check(hipMalloc(&dq, (size_t)n * sizeof(nsl_collide_query)));
+    hipError_t e = hipMalloc(&dout, (size_t)n * sizeof(nsl_local_manifold));
+    if (e == hipSuccess) e = hipMemcpy(dq, queries, (size_t)n * sizeof(nsl_collide_query), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      nsl::launch_collide_queries(dq, dout, n, nullptr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)n * sizeof(nsl_local_manifold), hipMemcpyDeviceToHost);
+    (void)hipFree(dq);
+    if (dout) (void)hipFree(dout);
+    check(e);
+  });
+}
 
 int nsl_time_of_impact(int device, const nsl_toi_input* in, nsl_toi_output* out, int32_t n) {
   if (n < 0 || (n > 0 && (!in || !out))) return fail(NSL_ERR_INVALID, "null argument or negative count");

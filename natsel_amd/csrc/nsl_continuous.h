@@ -368,18 +368,21 @@ __device__ bool solve_continuous_body(const DeviceWorld& d, const ContinuousArgs
     const Aabb box1 = d.shape_aabbs[fast_id];
     const Aabb box2 = offset_aabb(shape_aabb_at(d, fs, xf2), base);
     d.shape_aabbs[fast_id] = box2;
+    const Proxy fp = a.proxies[fast_id];
+    if (fp.flags & kProxySensor) continue;  // no continuous collision for sensors (continuous.rs:398-401)
     if (!is_bullet && !is_shape_fast(d, fs, xf_point(xf1, lc), xf_point(xf2, lc), rotation_chord, fast.safety_factor))
       continue;
     const Aabb swept = aabb_union(box1, box2);
     Vec3 fast_store[2];
     const ToiProxy fast_proxy = shape_toi_proxy(d, fs, fast_store);
-    const Proxy fp = a.proxies[fast_id];
     for (int k = 0; k < n_candidates; ++k) {
       const int32_t cid = a.candidates[k];
       if (!aabb_overlaps(d.fat_aabbs[cid], swept)) continue;
       // continuous_query_callback_shapes (continuous.rs:218-298)
       const Proxy cp = a.proxies[cid];
       if (cid == fast_id || cp.body == fp.body) continue;
+      // a sensor candidate reports only sensor events, which are not produced (continuous.rs:239-246)
+      if (cp.flags & kProxySensor) continue;
       if (cp.group == fp.group && cp.group != 0) {  // shape_runtime.rs:285-292
         if (!(fp.group > 0)) continue;
       } else if (!((fp.mask & cp.category) != 0 && (fp.category & cp.mask) != 0)) {

@@ -24,6 +24,7 @@ struct BodyDesc {
 struct Material {
   float density = 1000.0f, friction = 0.6f, restitution = 0.0f, rolling_resistance = 0.0f;
   bool enable_speculative = true, enable_hit_events = false;
+  bool is_sensor = false;  // no contacts, no continuous collision (contact_creation.rs:187, continuous.rs:398-401)
   uint64_t category_bits = 1, mask_bits = ~0ull;
   int32_t group_index = 0;
 };
@@ -89,6 +90,7 @@ class HostScene {
     int32_t group = 0;
     float density = 0, restitution = 0, rolling_resistance = 0, radius = 0;
     bool speculative = true;
+    bool sensor = false;
     bool proxy_moved = false;
     Aabb fat{};
     Aabb aabb{};  // the shape's bounds at its body's last finalize or sweep

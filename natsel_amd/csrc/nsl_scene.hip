@@ -136,6 +136,7 @@ int32_t HostScene::add_shape(int32_t body, Shape shape, const Material& m, float
   hs.rolling_resistance = m.rolling_resistance;
   hs.radius = radius;
   hs.speculative = m.enable_speculative;
+  hs.sensor = m.is_sensor;
   hs.aabb = aabb;
   hs.fat = aabb_inflate(aabb, b.type == kBodyStatic ? kSpeculativeDistance : shape.aabb_margin);
   hs.proxy_moved = true;  // dynamic proxies always; static ones via invoke_contact_creation

@@ -153,7 +153,8 @@ void GpuWorld::upload_bodies() {
       aabb_host_[i] = hs.aabb;
       moved_host_[i] = hs.proxy_moved ? 1 : 0;
       pairs_pending_ = pairs_pending_ || hs.proxy_moved;
-      proxies_host_[i] = Proxy{hs.body, bodies_[hs.body].type, hs.group, 0, hs.category, hs.mask};
+      proxies_host_[i] = Proxy{hs.body, bodies_[hs.body].type, hs.group,
+                               (hs.speculative ? kProxySpeculative : 0) | (hs.sensor ? kProxySensor : 0), hs.category, hs.mask};
     }
     if (nsh) {
       NSL_CHECK(hipMemcpyAsync(d_.shapes, shapes_.data(), nsh * sizeof(Shape), hipMemcpyHostToDevice, stream_));

@@ -37,6 +37,10 @@ class BodySpec:
     gravity_scale: float = 1.0
     is_bullet: bool = False
     safety_factor: float = 0.5
+    sleep_threshold: float = 0.05
+    motion_locks: int = 0  # MotionLocks bits: linear x, y, z = 1, 2, 4; angular x, y, z = 8, 16, 32
+    allow_fast_rotation: bool = False
+    enable_contact_recycling: bool = True
 
 
 @dataclass
@@ -47,6 +51,13 @@ class ShapeSpec:
     density: float = 1000.0
     friction: float = 0.6
     restitution: float = 0.0
+    rolling_resistance: float = 0.0
+    category_bits: int = 1
+    mask_bits: int = 0xFFFF_FFFF_FFFF_FFFF
+    group_index: int = 0
+    is_sensor: bool = False
+    enable_speculative_contact: bool = True
+    enable_hit_events: bool = False
 
 
 @dataclass
@@ -86,6 +97,7 @@ class Scene:
     restitution_iterations: int = 2
     restitution_threshold: float = 1.0
     restitution_propagation: bool = False
+    tags: dict = field(default_factory=dict)  # name -> body index, for scenes that name their bodies
 
     def add_body(self, spec: BodySpec) -> int:
         self.bodies.append(spec)
@@ -842,4 +854,71 @@ def kinematic_platform(boxes: int = 6) -> Scene:
     for i in range(boxes):
         b = sc.add_body(BodySpec(DYNAMIC, (-3.0 + 1.2 * i, 0.76, 0.0)))
         sc.add_shape(ShapeSpec(b, BOX, (0.5, 0.5, 0.5), friction=0.8))
+    return sc
+
+
+LOCK_LINEAR_Z, LOCK_ANGULAR_X, LOCK_ANGULAR_Y = 0x4, 0x8, 0x10
+LOCK_ROTATION = 0x8 | 0x10 | 0x20
+
+
+def def_features(enable_sleep: bool = True, enable_continuous: bool = True) -> Scene:
+    """Every BodyDef and ShapeDef field the C ABI carries, in one scene on a static ground
+    (body_def.rs:58-104, shape_def.rs:90-135). Named bodies (Scene.tags):
+      roll_sphere / slide_sphere   spheres launched at 3 m/s, rolling resistance 0.3 and 0
+      roll_capsule                 a capsule rolling across x, rolling resistance 0.2
+      planar_*                     tilted boxes locked to the x-y plane (linear z, angular x, y)
+      fixed_rotation               a box with every rotation locked, launched spinning
+      group_a / group_b            boxes of filter group -1 dropped onto each other (never touch)
+      mask_a / mask_b              category 2 boxes whose masks exclude category 2
+      sensor_drop                  a ball falling through a static sensor box
+      bullet_ball                  a ball at 60 m/s (continuous) through a thin static sensor wall
+      fast_spin / clamped_spin     spin 300 rad/s above the clamp (0.25 pi per substep), with and
+                                   without allow_fast_rotation
+      no_recycle                   contact recycling off
+      sleepy                       sleep threshold 0.5 m/s
+      no_speculative / hit_events  shape flags
+    """
+    sc = Scene(name="def_features", enable_sleep=enable_sleep, enable_continuous=enable_continuous)
+    g = sc.add_body(BodySpec(STATIC, (0.0, -0.5, 0.0)))
+    sc.add_shape(ShapeSpec(g, BOX, (30.0, 0.5, 30.0)))
+
+    def body(tag, pos, **kw):
+        shape = kw.pop("shape")
+        b = sc.add_body(BodySpec(DYNAMIC, pos, **kw))
+        kind, params, skw = shape
+        sc.add_shape(ShapeSpec(b, kind, params, **skw))
+        sc.tags[tag] = b
+        return b
+
+    body("roll_sphere", (0.0, 0.3, -6.0), linear_velocity=(3.0, 0.0, 0.0),
+         shape=(SPHERE, (0.0, 0.0, 0.0, 0.3), {"rolling_resistance": 0.3}))
+    body("slide_sphere", (0.0, 0.3, -4.0), linear_velocity=(3.0, 0.0, 0.0), shape=(SPHERE, (0.0, 0.0, 0.0, 0.3), {}))
+    body("roll_capsule", (0.0, 0.2, -2.0), linear_velocity=(2.0, 0.0, 0.0),
+         shape=(CAPSULE, (0.0, 0.0, -0.4, 0.0, 0.0, 0.4, 0.2), {"rolling_resistance": 0.2}))
+    tilt = (0.0, 0.0, float(np.sin(0.15)), float(np.cos(0.15)))
+    for k in range(3):
+        body(f"planar_{k}", (-6.0 + 0.3 * k, 0.6 + 1.1 * k, 0.0), rotation=tilt, motion_locks=LOCK_LINEAR_Z | LOCK_ANGULAR_X | LOCK_ANGULAR_Y,
+             angular_velocity=(1.0, 2.0, 0.5), shape=(BOX, (0.5, 0.5, 0.5), {}))
+    body("fixed_rotation", (-3.0, 0.5, 3.0), linear_velocity=(2.0, 0.0, 0.0), angular_velocity=(0.0, 3.0, 0.0),
+         motion_locks=LOCK_ROTATION, shape=(BOX, (0.5, 0.5, 0.5), {}))
+    body("group_a", (3.0, 0.5, 3.0), shape=(BOX, (0.5, 0.5, 0.5), {"group_index": -1}))
+    body("group_b", (3.2, 1.6, 3.1), shape=(BOX, (0.5, 0.5, 0.5), {"group_index": -1}))
+    body("mask_a", (6.0, 0.5, 3.0), shape=(BOX, (0.5, 0.5, 0.5), {"category_bits": 2, "mask_bits": 0xFFFF_FFFF_FFFF_FFFD}))
+    body("mask_b", (6.1, 1.6, 3.0), shape=(BOX, (0.5, 0.5, 0.5), {"category_bits": 2, "mask_bits": 0xFFFF_FFFF_FFFF_FFFD}))
+    s = sc.add_body(BodySpec(STATIC, (9.0, 2.0, 3.0)))
+    sc.add_shape(ShapeSpec(s, BOX, (1.0, 0.5, 1.0), is_sensor=True))
+    body("sensor_drop", (9.0, 4.0, 3.0), shape=(SPHERE, (0.0, 0.0, 0.0, 0.25), {}))
+    wall = sc.add_body(BodySpec(STATIC, (14.0, 1.0, -6.0)))
+    sc.add_shape(ShapeSpec(wall, BOX, (0.05, 1.0, 1.0), is_sensor=True))
+    body("bullet_ball", (8.0, 1.0, -6.0), linear_velocity=(60.0, 0.0, 0.0), gravity_scale=0.0,
+         shape=(SPHERE, (0.0, 0.0, 0.0, 0.1), {}))
+    body("fast_spin", (0.0, 0.6, 6.0), angular_velocity=(0.0, 300.0, 0.0), allow_fast_rotation=True,
+         shape=(BOX, (0.5, 0.5, 0.5), {}))
+    body("clamped_spin", (3.0, 0.6, 6.0), angular_velocity=(0.0, 300.0, 0.0), shape=(BOX, (0.5, 0.5, 0.5), {}))
+    body("no_recycle", (3.0, 0.5, -3.0), linear_velocity=(0.5, 0.0, 0.0), enable_contact_recycling=False,
+         shape=(BOX, (0.5, 0.5, 0.5), {}))
+    body("sleepy", (6.0, 0.5, -3.0), sleep_threshold=0.5, shape=(BOX, (0.5, 0.5, 0.5), {}))
+    body("no_speculative", (-6.0, 0.5, -3.0), linear_velocity=(0.0, -1.0, 0.0),
+         shape=(BOX, (0.5, 0.5, 0.5), {"enable_speculative_contact": False}))
+    body("hit_events", (-3.0, 0.5, -3.0), shape=(SPHERE, (0.0, 0.0, 0.0, 0.5), {"enable_hit_events": True}))
     return sc

@@ -20,7 +20,7 @@ from .scenes import BOX, CAPSULE, DYNAMIC, KINEMATIC, SPHERE, STATIC, Scene  # n
 
 
 @dataclass
-class BodyDef:  # src/types/body_def.rs:80-101 defaults
+class BodyDef:  # src/types/body_def.rs:58-104 (fields and defaults)
     type: int = STATIC
     position: tuple = (0.0, 0.0, 0.0)
     rotation: tuple = (0.0, 0.0, 0.0, 1.0)
@@ -32,14 +32,26 @@ class BodyDef:  # src/types/body_def.rs:80-101 defaults
     enable_sleep: bool = True
     is_bullet: bool = False
     safety_factor: float = 0.5
+    sleep_threshold: float = 0.05
+    motion_locks: int = 0  # capi.LOCK_* bits (MotionLocks)
+    is_awake: bool = True
+    is_enabled: bool = True
+    allow_fast_rotation: bool = False
+    enable_contact_recycling: bool = True
 
 
 @dataclass
-class ShapeDef:  # src/types/shape_def.rs:114-135 defaults
+class ShapeDef:  # src/types/shape_def.rs:90-135 (fields and defaults; the base material's friction etc.)
     density: float = 1000.0
     friction: float = 0.6
     restitution: float = 0.0
     rolling_resistance: float = 0.0
+    category_bits: int = 1  # Filter (shape_def.rs:18-24)
+    mask_bits: int = 0xFFFF_FFFF_FFFF_FFFF
+    group_index: int = 0
+    is_sensor: bool = False
+    enable_speculative_contact: bool = True
+    enable_hit_events: bool = False
 
 
 @dataclass
@@ -86,7 +98,28 @@ class WorldOptions:  # world_runtime.rs:1243-1304
 
 
 def _material(d: ShapeDef) -> capi.Material:
-    return capi.Material(d.density, d.friction, d.restitution, d.rolling_resistance)
+    return capi.Material(d.density, d.friction, d.restitution, d.rolling_resistance, d.category_bits, d.mask_bits,
+                         d.group_index, int(d.is_sensor), int(d.enable_speculative_contact), int(d.enable_hit_events))
+
+
+def c_body_def(d: BodyDef) -> capi.BodyDef:
+    """BodyDef as the C ABI's nsl_body_def."""
+    c = capi.BodyDef()
+    c.type = d.type
+    c.position[:] = d.position
+    c.rotation[:] = d.rotation
+    c.linear_velocity[:] = d.linear_velocity
+    c.angular_velocity[:] = d.angular_velocity
+    c.linear_damping, c.angular_damping, c.gravity_scale = d.linear_damping, d.angular_damping, d.gravity_scale
+    c.enable_sleep = int(d.enable_sleep)
+    c.is_bullet = int(d.is_bullet)
+    c.safety_factor = d.safety_factor
+    c.sleep_threshold = d.sleep_threshold
+    c.motion_locks = int(d.motion_locks)
+    c.is_awake, c.is_enabled = int(d.is_awake), int(d.is_enabled)
+    c.allow_fast_rotation = int(d.allow_fast_rotation)
+    c.enable_contact_recycling = int(d.enable_contact_recycling)
+    return c
 
 
 def c_options(o: WorldOptions) -> capi.Options:
@@ -154,16 +187,7 @@ class World:
 
     # --- creation -------------------------------------------------------------------
     def create_body(self, d: BodyDef) -> int:
-        c = capi.BodyDef()
-        c.type = d.type
-        c.position[:] = d.position
-        c.rotation[:] = d.rotation
-        c.linear_velocity[:] = d.linear_velocity
-        c.angular_velocity[:] = d.angular_velocity
-        c.linear_damping, c.angular_damping, c.gravity_scale = d.linear_damping, d.angular_damping, d.gravity_scale
-        c.enable_sleep = int(d.enable_sleep)
-        c.is_bullet = int(d.is_bullet)
-        c.safety_factor = d.safety_factor
+        c = c_body_def(d)
         out = ctypes.c_int32()
         capi.check(self._lib.nsl_create_body(self._h, ctypes.byref(c), ctypes.byref(out)))
         return out.value
@@ -315,6 +339,20 @@ class World:
         return {name: getattr(p, name) for name, _ in capi.Profile._fields_}
 
 
+def body_def_of(b) -> BodyDef:
+    """The BodyDef of a scene's BodySpec."""
+    return BodyDef(b.body_type, b.position, b.rotation, b.linear_velocity, b.angular_velocity, b.linear_damping,
+                   b.angular_damping, b.gravity_scale, b.enable_sleep, b.is_bullet, b.safety_factor,
+                   sleep_threshold=b.sleep_threshold, motion_locks=b.motion_locks, allow_fast_rotation=b.allow_fast_rotation,
+                   enable_contact_recycling=b.enable_contact_recycling)
+
+
+def shape_def_of(s) -> ShapeDef:
+    """The ShapeDef of a scene's ShapeSpec."""
+    return ShapeDef(s.density, s.friction, s.restitution, s.rolling_resistance, s.category_bits, s.mask_bits, s.group_index,
+                    s.is_sensor, s.enable_speculative_contact, s.enable_hit_events)
+
+
 def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0, small_colour_items: int = 128) -> World:
     """Build a World from a seeded scene description (natsel_amd.scenes)."""
     opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous, solver_blocks=solver_blocks,
@@ -323,13 +361,9 @@ def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0, small_colo
                         restitution_threshold=scene.restitution_threshold,
                         enable_restitution_propagation=scene.restitution_propagation)
     w = World(device, opts)
-    ids = []
-    for b in scene.bodies:
-        ids.append(w.create_body(BodyDef(b.body_type, b.position, b.rotation, b.linear_velocity, b.angular_velocity,
-                                         b.linear_damping, b.angular_damping, b.gravity_scale, b.enable_sleep, b.is_bullet,
-                                         b.safety_factor)))
+    ids = [w.create_body(body_def_of(b)) for b in scene.bodies]
     for s in scene.shapes:
-        d = ShapeDef(s.density, s.friction, s.restitution)
+        d = shape_def_of(s)
         if s.kind == BOX:
             w.create_hull_shape(ids[s.body], d, make_box_hull(*s.params))
         elif s.kind == SPHERE:

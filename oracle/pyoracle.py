@@ -39,10 +39,11 @@ def lib():
         L.nslo_set_options.argtypes = [P, I, I, I, I]
         L.nslo_set_gravity.argtypes = [P, F, F, F]
         L.nslo_set_restitution.argtypes = [P, I, F, I]
-        L.nslo_create_body.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), fp, fp, fp, I, F, F, F, I, F]
-        L.nslo_create_box.argtypes = [P, I, F, F, F, F, F, F]
-        L.nslo_create_sphere.argtypes = [P, I, fp, F, F, F, F]
-        L.nslo_create_capsule.argtypes = [P, I, fp, fp, F, F, F, F]
+        L.nslo_create_body.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), fp, fp, fp, I, F, F, F, I, F, F, I, I, I]
+        lp = ctypes.POINTER(ctypes.c_int64)
+        L.nslo_create_box.argtypes = [P, I, F, F, F, fp, lp]
+        L.nslo_create_sphere.argtypes = [P, I, fp, F, fp, lp]
+        L.nslo_create_capsule.argtypes = [P, I, fp, fp, F, fp, lp]
         L.nslo_step.argtypes = [P, F, I]
         L.nslo_state_hash.argtypes = [P]
         L.nslo_state_hash.restype = ctypes.c_uint64
@@ -121,6 +122,17 @@ class _OracleSnapshot:
             pass
 
 
+def _shape_options(kw: dict) -> dict:
+    """ShapeDef fields beyond density, friction and restitution, with their defaults."""
+    out = {"rolling_resistance": 0.0, "category_bits": 1, "mask_bits": 0xFFFF_FFFF_FFFF_FFFF, "group_index": 0,
+           "is_sensor": False, "enable_speculative_contact": True, "enable_hit_events": False}
+    unknown = set(kw) - set(out)
+    if unknown:
+        raise TypeError(f"unknown shape fields {sorted(unknown)}")
+    out.update(kw)
+    return out
+
+
 class OracleWorld:
     """The oracle's World: create bodies/shapes from a scene, step, hash."""
 
@@ -141,23 +153,38 @@ class OracleWorld:
             pass
 
     def create_body(self, body_type, position, rotation=(0, 0, 0, 1), v=(0, 0, 0), w=(0, 0, 0), enable_sleep=True,
-                    linear_damping=0.0, angular_damping=0.0, gravity_scale=1.0, is_bullet=False, safety_factor=0.5) -> int:
+                    linear_damping=0.0, angular_damping=0.0, gravity_scale=1.0, is_bullet=False, safety_factor=0.5,
+                    sleep_threshold=0.05, motion_locks=0, allow_fast_rotation=False, enable_contact_recycling=True) -> int:
         p = (ctypes.c_double * 3)(*position)
         q = (ctypes.c_float * 4)(*rotation)
         lv = (ctypes.c_float * 3)(*v)
         av = (ctypes.c_float * 3)(*w)
         return self.L.nslo_create_body(self.h, body_type, p, q, lv, av, int(enable_sleep), linear_damping, angular_damping,
-                                       gravity_scale, int(is_bullet), float(safety_factor))
+                                       gravity_scale, int(is_bullet), float(safety_factor), float(sleep_threshold),
+                                       int(motion_locks), int(allow_fast_rotation), int(enable_contact_recycling))
 
-    def create_box(self, body, hx, hy, hz, density=1000.0, friction=0.6, restitution=0.0) -> int:
-        return self.L.nslo_create_box(self.h, body, hx, hy, hz, density, friction, restitution)
+    @staticmethod
+    def _shape_def(density, friction, restitution, rolling_resistance, category_bits, mask_bits, group_index, is_sensor,
+                   enable_speculative_contact, enable_hit_events):
+        mat = (ctypes.c_float * 4)(density, friction, restitution, rolling_resistance)
+        def s64(u):  # the u64 filter bits as the int64 the C array holds
+            u &= 0xFFFF_FFFF_FFFF_FFFF
+            return u - (1 << 64) if u >= (1 << 63) else u
+        opt = (ctypes.c_int64 * 6)(s64(category_bits), s64(mask_bits), group_index, int(is_sensor),
+                                   int(enable_speculative_contact), int(enable_hit_events))
+        return mat, opt
 
-    def create_sphere(self, body, center, radius, density=1000.0, friction=0.6, restitution=0.0) -> int:
-        return self.L.nslo_create_sphere(self.h, body, (ctypes.c_float * 3)(*center), radius, density, friction, restitution)
+    def create_box(self, body, hx, hy, hz, density=1000.0, friction=0.6, restitution=0.0, **shape) -> int:
+        mat, opt = self._shape_def(density, friction, restitution, **_shape_options(shape))
+        return self.L.nslo_create_box(self.h, body, hx, hy, hz, mat, opt)
 
-    def create_capsule(self, body, c1, c2, radius, density=1000.0, friction=0.6, restitution=0.0) -> int:
-        return self.L.nslo_create_capsule(self.h, body, (ctypes.c_float * 3)(*c1), (ctypes.c_float * 3)(*c2), radius, density,
-                                          friction, restitution)
+    def create_sphere(self, body, center, radius, density=1000.0, friction=0.6, restitution=0.0, **shape) -> int:
+        mat, opt = self._shape_def(density, friction, restitution, **_shape_options(shape))
+        return self.L.nslo_create_sphere(self.h, body, (ctypes.c_float * 3)(*center), radius, mat, opt)
+
+    def create_capsule(self, body, c1, c2, radius, density=1000.0, friction=0.6, restitution=0.0, **shape) -> int:
+        mat, opt = self._shape_def(density, friction, restitution, **_shape_options(shape))
+        return self.L.nslo_create_capsule(self.h, body, (ctypes.c_float * 3)(*c1), (ctypes.c_float * 3)(*c2), radius, mat, opt)
 
     def create_revolute_joint(self, j) -> int:
         """j: natsel_amd.scenes.JointSpec."""
@@ -270,15 +297,20 @@ def load_scene(scene) -> OracleWorld:
     w.L.nslo_set_restitution(w.h, int(scene.restitution_iterations), float(scene.restitution_threshold),
                              int(scene.restitution_propagation))
     ids = [w.create_body(b.body_type, b.position, b.rotation, b.linear_velocity, b.angular_velocity, b.enable_sleep,
-                         b.linear_damping, b.angular_damping, b.gravity_scale, b.is_bullet, b.safety_factor)
+                         b.linear_damping, b.angular_damping, b.gravity_scale, b.is_bullet, b.safety_factor,
+                         b.sleep_threshold, b.motion_locks, b.allow_fast_rotation, b.enable_contact_recycling)
            for b in scene.bodies]
     for s in scene.shapes:
+        extra = {"rolling_resistance": s.rolling_resistance, "category_bits": s.category_bits, "mask_bits": s.mask_bits,
+                 "group_index": s.group_index, "is_sensor": s.is_sensor,
+                 "enable_speculative_contact": s.enable_speculative_contact, "enable_hit_events": s.enable_hit_events}
         if s.kind == BOX:
-            w.create_box(ids[s.body], *s.params, s.density, s.friction, s.restitution)
+            w.create_box(ids[s.body], *s.params, s.density, s.friction, s.restitution, **extra)
         elif s.kind == SPHERE:
-            w.create_sphere(ids[s.body], s.params[:3], s.params[3], s.density, s.friction, s.restitution)
+            w.create_sphere(ids[s.body], s.params[:3], s.params[3], s.density, s.friction, s.restitution, **extra)
         else:
-            w.create_capsule(ids[s.body], s.params[:3], s.params[3:6], s.params[6], s.density, s.friction, s.restitution)
+            w.create_capsule(ids[s.body], s.params[:3], s.params[3:6], s.params[6], s.density, s.friction, s.restitution,
+                             **extra)
     for j in scene.joints:
         w.create_revolute_joint(dataclasses.replace(j, body_a=ids[j.body_a], body_b=ids[j.body_b]))
     return w
@@ -341,6 +373,9 @@ class Narrow:
         self.L.nslo_shape_distance.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, fp, ip, fp, ip]
         self.L.nslo_box_hull.argtypes = [fp, fp, fp, ip, fp]
         self._fp, self._ip = fp, ip
+
+    def cache(self) -> NarrowCache:
+        return NarrowCache()
 
     def collide(self, a, b, xf=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)), cache: NarrowCache | None = None) -> dict:
         """The register table's routine for (a, b) with B placed in A's frame by xf = (p, q).

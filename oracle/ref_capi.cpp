@@ -36,10 +36,15 @@ void nslo_set_restitution(void* wp, int iterations, float threshold, int propaga
 
 int nslo_create_body(void* wp, int type, const double* pos, const float* rot, const float* v, const float* av,
                      int enable_sleep, float linear_damping, float angular_damping, float gravity_scale,
-                     int is_bullet, float safety_factor) {
+                     int is_bullet, float safety_factor, float sleep_threshold, int lock_flags, int allow_fast_rotation,
+                     int enable_contact_recycling) {
   BodyDef d;
   d.is_bullet = is_bullet != 0;
   d.safety_factor = safety_factor;
+  d.sleep_threshold = sleep_threshold;
+  d.lock_flags = (uint32_t)lock_flags & BF::FIXED_ROTATION_AND_LINEAR;
+  d.allow_fast_rotation = allow_fast_rotation != 0;
+  d.enable_contact_recycling = enable_contact_recycling != 0;
   d.type = (BodyType)type;
   d.position = Pos{pos[0], pos[1], pos[2]};
   d.rotation = Quat{{rot[0], rot[1], rot[2]}, rot[3]};
@@ -52,23 +57,31 @@ int nslo_create_body(void* wp, int type, const double* pos, const float* rot, co
   return static_cast<World*>(wp)->create_body(d);
 }
 
-static ShapeDef shape_def(float density, float friction, float restitution) {
+// ShapeDef (shape_def.rs:90-135): mat = density, friction, restitution, rolling resistance;
+// opt = filter category, mask and group, is_sensor, enable_speculative_contact, enable_hit_events.
+static ShapeDef shape_def(const float* mat, const int64_t* opt) {
   ShapeDef s;
-  s.density = density;
-  s.friction = friction;
-  s.restitution = restitution;
+  s.density = mat[0];
+  s.friction = mat[1];
+  s.restitution = mat[2];
+  s.rolling_resistance = mat[3];
+  s.filter.category = (uint64_t)opt[0];
+  s.filter.mask = (uint64_t)opt[1];
+  s.filter.group = (int32_t)opt[2];
+  s.is_sensor = opt[3] != 0;
+  s.enable_speculative = opt[4] != 0;
+  s.enable_hit_events = opt[5] != 0;
   return s;
 }
 
-int nslo_create_box(void* wp, int body, float hx, float hy, float hz, float density, float friction, float restitution) {
-  return static_cast<World*>(wp)->create_hull_shape(body, shape_def(density, friction, restitution), make_box_hull(hx, hy, hz));
+int nslo_create_box(void* wp, int body, float hx, float hy, float hz, const float* mat, const int64_t* opt) {
+  return static_cast<World*>(wp)->create_hull_shape(body, shape_def(mat, opt), make_box_hull(hx, hy, hz));
 }
-int nslo_create_sphere(void* wp, int body, const float* c, float r, float density, float friction, float restitution) {
-  return static_cast<World*>(wp)->create_sphere_shape(body, shape_def(density, friction, restitution), Sphere{{c[0], c[1], c[2]}, r});
+int nslo_create_sphere(void* wp, int body, const float* c, float r, const float* mat, const int64_t* opt) {
+  return static_cast<World*>(wp)->create_sphere_shape(body, shape_def(mat, opt), Sphere{{c[0], c[1], c[2]}, r});
 }
-int nslo_create_capsule(void* wp, int body, const float* c1, const float* c2, float r, float density, float friction,
-                        float restitution) {
-  return static_cast<World*>(wp)->create_capsule_shape(body, shape_def(density, friction, restitution),
+int nslo_create_capsule(void* wp, int body, const float* c1, const float* c2, float r, const float* mat, const int64_t* opt) {
+  return static_cast<World*>(wp)->create_capsule_shape(body, shape_def(mat, opt),
                                                        Capsule{{c1[0], c1[1], c1[2]}, {c2[0], c2[1], c2[2]}, r});
 }
 

@@ -403,6 +403,7 @@ void World::solve_continuous(int32_t sim_index, float dt) {
     const AABB box1 = fs.aabb;
     const AABB box2 = offset_aabb(compute_shape_aabb(fs, xf2), base);
     fs.aabb = box2;
+    if (fs.is_sensor) continue;  // no continuous collision for sensors (continuous.rs:398-401)
     if (!is_bullet && !is_shape_fast(fs, centroid1, centroid2, rotation_chord, safety_factor)) continue;
     const AABB swept = aabb_union(box1, box2);
     const ToiProxy fast_proxy = shape_proxy(fs);
@@ -414,6 +415,8 @@ void World::solve_continuous(int32_t sim_index, float dt) {
         if (!aabb_overlaps(fat_aabbs[s.id], swept)) continue;
         // continuous_query_callback_shapes (continuous.rs:218-298)
         if (s.id == fs.id || s.body_id == fs.body_id) continue;
+        // a sensor candidate reports only sensor events, which are not produced (continuous.rs:239-246)
+        if (s.is_sensor) continue;
         if (s.filter.group == fs.filter.group && s.filter.group != 0) {
           if (!(fs.filter.group > 0)) continue;
         } else if (!((fs.filter.mask & s.filter.category) != 0 && (fs.filter.category & s.filter.mask) != 0)) {

@@ -166,6 +166,7 @@ int32_t World::create_shape_internal(int32_t body_id, const ShapeDef& def, Shape
   shape.filter = def.filter;
   shape.enable_speculative = def.enable_speculative;
   shape.enable_hit_events = def.enable_hit_events;
+  shape.is_sensor = def.is_sensor;
   shape.aabb_margin = compute_shape_margin(shape);
   // create_shape_proxy / update_shape_aabbs (shape_creation.rs:237-283)
   WXf xf = sim_of(b).transform;
@@ -412,6 +413,7 @@ bool World::should_create_pair(int32_t ia, int32_t ib) const {
   const Shape& a = shapes[ia];
   const Shape& b = shapes[ib];
   if (a.body_id == b.body_id) return false;
+  if (a.is_sensor || b.is_sensor) return false;  // contact_creation.rs:187
   // shape_runtime.rs:285-292
   if (a.filter.group == b.filter.group && a.filter.group != 0) {
     if (!(a.filter.group > 0)) return false;

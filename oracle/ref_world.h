@@ -38,6 +38,7 @@ constexpr uint32_t ALLOW_FAST_ROTATION = 0x400, ENLARGE_BULLET_BOUNDS = 0x800;
 constexpr uint32_t DYNAMIC_FLAG = 0x1000, ENABLE_SLEEP = 0x2000, ENABLE_CONTACT_RECYCLING = 0x4000;
 constexpr uint32_t DIRTY_MASS = 0x8000, ENABLE_GYROSCOPIC_TORQUE = 0x10000;
 constexpr uint32_t FIXED_ROTATION = LOCK_ANGULAR_X | LOCK_ANGULAR_Y | LOCK_ANGULAR_Z;
+constexpr uint32_t FIXED_ROTATION_AND_LINEAR = FIXED_ROTATION | LOCK_LINEAR_X | LOCK_LINEAR_Y | LOCK_LINEAR_Z;
 constexpr uint32_t TRANSIENT = IS_SPEED_CAPPED | HAD_TOI;
 }  // namespace BF
 
@@ -122,6 +123,7 @@ struct Shape {
   Vec3 tangent_velocity = kZero3;
   Filter filter;
   bool enable_speculative = true, enable_hit_events = false;
+  bool is_sensor = false;  // sensor_index != NULL: no contacts, no continuous collision
   AABB aabb{};
   float aabb_margin = 0;
   int32_t next_shape_id = kNullIndex, prev_shape_id = kNullIndex;
@@ -337,6 +339,7 @@ struct ShapeDef {
   float density = 1000.f, friction = 0.6f, restitution = 0.f, rolling_resistance = 0.f;
   Filter filter;
   bool enable_speculative = true, invoke_contact_creation = true, enable_hit_events = false;
+  bool is_sensor = false;
 };
 
 // GJK (src/collision/distance.rs:483-709) with use_radii false, in frame A

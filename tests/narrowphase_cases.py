@@ -28,8 +28,7 @@ import math
 
 import numpy as np
 
-from oracle.pyoracle import (SAT_EDGE, SAT_FACE_A, SAT_FACE_B, SAT_INVALID, NarrowCache, shape_box, shape_capsule,
-                             shape_sphere)
+from oracle.pyoracle import SAT_EDGE, SAT_FACE_A, SAT_FACE_B, shape_box, shape_capsule, shape_sphere
 
 F32 = np.float32
 PI = F32(3.14159265359)           # constants.rs:6
@@ -151,20 +150,20 @@ def ka_capsule_sphere(nar):
 
 
 def ka_hull_sphere(nar):
-    c = NarrowCache()
+    c = nar.cache()
     m = nar.collide(shape_box(1, 1, 1), shape_sphere((0, 0, 0), 1.0), ((1.2, 0.5, 0), (0, 0, 0, 1)), c)
     return [] if len(m["points"]) == 1 else [f"point count {len(m['points'])}"]
 
 
 def ka_hull_capsule(nar):
-    c = NarrowCache()
+    c = nar.cache()
     m = nar.collide(shape_box(1, 1, 1), shape_capsule((0, 0, 0), (0, 0, 0), 0.25), ((1.1, 0, 0), (0, 0, 0, 1)), c)
     return [] if len(m["points"]) == 2 else [f"point count {len(m['points'])}"]
 
 
 def ka_face_contact(nar):
     f = []
-    c = NarrowCache()
+    c = nar.cache()
     xf = ((1.5, 0, 0), (0, 0, 0, 1))
     nar.collide(shape_box(1, 1, 1), shape_box(1, 1, 1), xf, c)
     s = c.sat
@@ -191,14 +190,14 @@ def ka_rotated_contact(nar):
     f = []
     q = _make_quat_from_axis_angle((0.0, 1.0, 0.0), 0.785398163)
     xf = ((1.2, 0.7, 0.3), q)
-    c = NarrowCache()
+    c = nar.cache()
     m = nar.collide(shape_box(1, 1, 1), shape_box(1, 1, 1), xf, c)
     if bits(m["normal"][1]) != 0 or bits(m["normal"][2]) != 0:
         f.append(f"normal y/z bits not zero: {m['normal']}")
     s = c.sat
     if s["type"] != SAT_FACE_A or s["index_a"] != 1 or s["index_b"] != 5:
         f.append(f"cache {s}")
-    m2 = nar.collide(shape_box(1, 1, 1), shape_box(1, 1, 1), xf, NarrowCache())
+    m2 = nar.collide(shape_box(1, 1, 1), shape_box(1, 1, 1), xf, nar.cache())
     if len(m2["points"]) != 4:
         f.append(f"point count {len(m2['points'])}")
     return f
@@ -216,7 +215,7 @@ def rb_overlap_never_empty(nar):
         axis = r.next_direction()
         angle = r.next_float(-0.01, 0.01) if (i & 1) else r.next_float(-PI, PI)
         offset = F32(0.4) * r.next_direction()
-        m = nar.collide(a, b, (tuple(offset), exact_quat(axis, angle)), NarrowCache())
+        m = nar.collide(a, b, (tuple(offset), exact_quat(axis, angle)), nar.cache())
         if not m["points"]:
             f.append(f"sample {i}: no points")
         elif abs(float(np.linalg.norm(m["normal"].astype(np.float64))) - 1.0) >= 1e-5:
@@ -230,7 +229,7 @@ def rb_overlap_never_empty(nar):
 
 def rb_max_point_count(nar):
     """:145-161 (the portable build's capacity is four points)."""
-    m = nar.collide(shape_box(0.5, 0.5, 0.5), shape_box(0.5, 0.5, 0.5), ((0, 0.99, 0), (0, 0, 0, 1)), NarrowCache())
+    m = nar.collide(shape_box(0.5, 0.5, 0.5), shape_box(0.5, 0.5, 0.5), ((0, 0.99, 0), (0, 0, 0, 1)), nar.cache())
     return [] if 1 <= len(m["points"]) <= MAX_POINTS else [f"point count {len(m['points'])}"]
 
 
@@ -243,7 +242,7 @@ def rb_deep_overlap(nar):
         axis = r.next_direction()
         angle = r.next_float(-PI, PI)
         offset = F32(0.2) * r.next_direction()
-        m = nar.collide(a, b, (tuple(offset), exact_quat(axis, angle)), NarrowCache())
+        m = nar.collide(a, b, (tuple(offset), exact_quat(axis, angle)), nar.cache())
         if not m["points"] or min_sep(m["points"]) >= 0.0:
             f.append(f"sample {i}: points {len(m['points'])} min separation {min_sep(m['points'])}")
             break
@@ -262,7 +261,7 @@ def rb_crossed_edge(nar):
     root2 = F32(np.sqrt(F32(2.0)))
     for d in (F32(1.42), root2, F32(1.41), F32(1.3)):
         expected = float(d - root2)
-        c = NarrowCache()
+        c = nar.cache()
         m = nar.collide(a, b, ((float(d), 0, 0), (0, 0, 0, 1)), c)
         if len(m["points"]) != 1 or c.sat["type"] != SAT_EDGE:
             f.append(f"d {d}: count {len(m['points'])} cache {c.sat}")
@@ -306,7 +305,7 @@ def rb_ridge_crossing(nar):
 
     def run(overlap, angle):
         lift = float(F32(2.0) * F32(ridge_y) - F32(overlap))
-        c = NarrowCache()
+        c = nar.cache()
         m = nar.collide(a, b, ((0.0, lift, 0.0), exact_quat(AXIS_Y, angle)), c)
         return m, c
 
@@ -389,7 +388,7 @@ def rb_edge_axis_oracle(nar, hulls):
         for yaw in (-0.35, -0.15, 0.0, 0.15, 0.35):
             for dist in (1.38, 1.40, root2, 1.44):
                 xf = ((dist, 0.0, 0.0), exact_quat(AXIS_Y, yaw))
-                c = NarrowCache()
+                c = nar.cache()
                 m = nar.collide(a, b, xf, c)
                 if c.sat["type"] != SAT_EDGE or len(m["points"]) != 1:
                     continue
@@ -421,7 +420,7 @@ def rb_hull_capsule_edge_deep(nar):
                 mid = np.float32(edge_point) + F32(-depth) * outward
                 c1 = mid + F32(-0.5) * cdir
                 c2 = mid + F32(0.5) * cdir
-                m = nar.collide(hull, shape_capsule(tuple(c1), tuple(c2), radius), IDENTITY, NarrowCache())
+                m = nar.collide(hull, shape_capsule(tuple(c1), tuple(c2), radius), IDENTITY, nar.cache())
                 label = f"d {depth} r {radius} tilt {tilt}"
                 if len(m["points"]) != 1:
                     f.append(f"{label}: point count {len(m['points'])}")
@@ -445,7 +444,7 @@ def rb_edge_cache(nar):
     f = []
     a, b = crossed_edge_hulls(0.5)
     root2 = F32(np.sqrt(F32(2.0)))
-    c = NarrowCache()
+    c = nar.cache()
 
     def run(d):
         return nar.collide(a, b, ((d, 0, 0), (0, 0, 0, 1)), c)
@@ -473,7 +472,7 @@ def rb_edge_endpoint(nar):
     """:657-699 (the automatic part): past the end of edge A the true gap is a vertex-edge
     distance beyond the speculative distance: no points and a positive cached separation."""
     a, b = crossed_edge_hulls(0.5)
-    c = NarrowCache()
+    c = nar.cache()
     m = nar.collide(a, b, ((1.41, 0.55, 0.0), (0, 0, 0, 1)), c)
     if len(m["points"]) != 0 or not c.sat["separation"] > 0:
         return [f"count {len(m['points'])} cache {c.sat}"]
@@ -490,7 +489,7 @@ def rb_parallel_edge(nar):
     a = shape_box(0.5, 0.5, 0.5)
     for axis in axes:
         for angle in angles:
-            c = NarrowCache()
+            c = nar.cache()
             m = nar.collide(a, a, ((0.0, float(F32(1.0) - F32(overlap)), 0.0), exact_quat(axis, angle)), c)
             label = f"axis {axis} angle {angle}"
             if not 4 <= len(m["points"]) <= MAX_POINTS:
@@ -520,7 +519,7 @@ def _seam(nar, make, analytic_all):
     prev, shallow, deep = 0.0, 0, 0
     for i in range(steps + 1):
         y = y_start - F32(i) * dy
-        m = nar.collide(shape_box(0.5, 0.5, 0.5), make(y, radius), IDENTITY, NarrowCache())
+        m = nar.collide(shape_box(0.5, 0.5, 0.5), make(y, radius), IDENTITY, nar.cache())
         if not m["points"]:
             return f + [f"step {i}: no points"]
         expected = float((y - F32(0.5)) - radius)
@@ -573,7 +572,7 @@ def rb_speculative_margin(nar):
                  ("capsule-capsule", cap_x, cap_z, up(0.5)), ("hull-sphere", box, sphere, up(0.75)),
                  ("hull-capsule", box, cap_x, up(0.75)), ("hull-hull", box, box, up(1.0))]
         for label, a, b, xf in cases:
-            m = nar.collide(a, b, xf, NarrowCache())
+            m = nar.collide(a, b, xf, nar.cache())
             if not m["points"]:
                 f.append(f"{label} gap {gap}: no points")
             elif abs(float(min_sep(m["points"])) - float(gap)) > 1e-4:
@@ -742,7 +741,7 @@ def cf_hull_capsule_face(nar):
     separation = height - radius."""
     f = []
     for x1, x2, h, r in ((-0.3, 0.3, 0.62, 0.1), (-0.8, 0.2, 0.58, 0.1), (-0.2, 0.9, 0.55, 0.08), (-0.4, 0.1, 0.6, 0.15)):
-        c = NarrowCache()
+        c = nar.cache()
         m = nar.collide(shape_box(0.5, 0.5, 0.5), shape_capsule((x1, h, 0.1), (x2, h, 0.1), r), IDENTITY, c)
         lo, hi = max(-0.5, x1), min(0.5, x2)
         if len(m["points"]) != 2:
@@ -795,7 +794,7 @@ def cf_reduced_face_manifold(nar):
     h = 0.5
     for ang in (0.3, 0.5, 0.785398, 1.1):
         q = exact_quat(AXIS_Y, ang)
-        m = nar.collide(shape_box(h, h, h), shape_box(h, h, h), ((0.0, 2 * h - 0.01, 0.0), q), NarrowCache())
+        m = nar.collide(shape_box(h, h, h), shape_box(h, h, h), ((0.0, 2 * h - 0.01, 0.0), q), nar.cache())
         sq = [np.array(v, np.float64) for v in ((h, h), (-h, h), (-h, -h), (h, -h))]  # (x, z) CCW about +y seen from below
         sq = [np.array([v[0], v[1]]) for v in sq]
         sq_ccw = sq if (sq[1][0] - sq[0][0]) * (sq[2][1] - sq[0][1]) - (sq[1][1] - sq[0][1]) * (sq[2][0] - sq[0][0]) > 0 else sq[::-1]
