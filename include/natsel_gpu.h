@@ -261,6 +261,15 @@ typedef struct nsl_local_manifold {
    shape's support points). */
 int nsl_collide(int device, const nsl_collide_query* queries, nsl_local_manifold* out, int32_t n);
 
+/* nsl_collide without any allocation. No counterpart in the reference: for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving loops). The
+   queries and manifolds are staged in a fixed device array of the library (about 2.2 MiB on
+   each device it runs on, held until the process ends) instead of buffers allocated per
+   call; results equal nsl_collide's. At most 8192 queries per call (NSL_ERR_INVALID above).
+   Every call on one device shares that device's array: calls that overlap on one device
+   (from several host threads) race. */
+int nsl_collide_noalloc(int device, const nsl_collide_query* queries, nsl_local_manifold* out, int32_t n);
+
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out);
 int nsl_world_destroy(nsl_world* world);
 int nsl_world_set_options(nsl_world* world, const nsl_options* options);

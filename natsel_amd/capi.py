@@ -207,6 +207,7 @@ SIGNATURES = {
     "nsl_default_material": (None, [ctypes.POINTER(Material)]),
     "nsl_time_of_impact": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ToiInput), ctypes.POINTER(ToiOutput), _I]),
     "nsl_collide": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CollideQuery), ctypes.POINTER(LocalManifold), _I]),
+    "nsl_collide_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CollideQuery), ctypes.POINTER(LocalManifold), _I]),
     "nsl_create_revolute_joint": (ctypes.c_int, [_P, ctypes.POINTER(RevoluteJointDef), ctypes.POINTER(_I)]),
     "nsl_world_get_joint": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_I)]),
     "nsl_world_joint_count": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),

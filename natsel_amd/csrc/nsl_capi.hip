@@ -276,6 +276,17 @@ int nsl_collide(int device, const nsl_collide_query* queries, nsl_local_manifold
   });
 }
 
+int nsl_collide_noalloc(int device, const nsl_collide_query* queries, nsl_local_manifold* out, int32_t n) {
+  if (n < 0 || (n > 0 && (!queries || !out))) return fail(NSL_ERR_INVALID, "null argument or negative count");
+  if (n > nsl::kNoAllocMaxQueries) return fail(NSL_ERR_INVALID, "more queries than nsl_collide_noalloc stages (8192)");
+  int32_t bad = -1;
+  if (const char* why = nsl::validate_collide_queries(queries, n, &bad)) return fail(NSL_ERR_INVALID, why);
+  if (n == 0) return NSL_OK;
+  if (hipSetDevice(device) != hipSuccess) return fail(NSL_ERR_DEVICE, "hipSetDevice failed");
+  if (const char* why = nsl::collide_queries_noalloc(device, queries, out, n)) return fail(NSL_ERR_DEVICE, why);
+  return NSL_OK;
+}
+
 int nsl_time_of_impact(int device, const nsl_toi_input* in, nsl_toi_output* out, int32_t n) {
   if (n < 0 || (n > 0 && (!in || !out))) return fail(NSL_ERR_INVALID, "null argument or negative count");
   for (int32_t k = 0; k < n; ++k)

@@ -35,4 +35,10 @@ NSL_HD BoxHull make_box_hull(Vec3 half, Xf t) {
 const char* validate_collide_queries(const nsl_collide_query* q, int32_t n, int32_t* bad);
 void launch_collide_queries(const nsl_collide_query* q, nsl_local_manifold* out, int n, hipStream_t s);
 
+// nsl_collide_noalloc: at most this many queries per call, staged in a fixed device array
+// (about 2.2 MiB per device, held until the process ends).
+constexpr int32_t kNoAllocMaxQueries = 8192;
+// One call through the fixed array on `device` (current). nullptr on success, else the reason.
+const char* collide_queries_noalloc(int device, const nsl_collide_query* queries, nsl_local_manifold* out, int32_t n);
+
 }  // namespace nsl

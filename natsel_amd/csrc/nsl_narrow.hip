@@ -1,6 +1,8 @@
 // The narrow phase on its own (nsl_collide): one collide call per lane, with the same
 // device routines the step's collide kernel uses, so a query's manifold and cache equal the
 // step's bit for bit.
+#include <mutex>
+
 #include "../../include/natsel_gpu.h"
 #include "nsl_box_collide.h"
 #include "nsl_narrow.h"
@@ -117,6 +119,51 @@ const char* validate_collide_queries(const nsl_collide_query* q, int32_t n, int3
 void launch_collide_queries(const nsl_collide_query* q, nsl_local_manifold* out, int n, hipStream_t s) {
   if (n > 0)
     hipLaunchKernelGGL(k_collide_queries, dim3((n + kNarrowBlock - 1) / kNarrowBlock), dim3(kNarrowBlock), 0, s, q, out, n);
+}
+
+// ---------------------------------------------------------------------------------
+// The staging memory of nsl_collide_noalloc: the queries and the manifolds of one call, in
+// a fixed device array instead of buffers allocated per call. Each device holds its own
+// copy of the array; its address on a device comes from hipGetSymbolAddress, once per
+// device (the array's name in host code is not a device address).
+namespace {
+
+constexpr size_t kQueryBytes = ((size_t)kNoAllocMaxQueries * sizeof(nsl_collide_query) + 255) & ~(size_t)255;
+constexpr size_t kScratchBytes = kQueryBytes + (size_t)kNoAllocMaxQueries * sizeof(nsl_local_manifold);
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_lock;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+const char* no_alloc_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return "device ordinal out of range";
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if (g_scratch_ptr[device] == nullptr) {
+    // with `device` current (the caller set it): this device's copy of the array
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return "hipGetSymbolAddress failed";
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return nullptr;
+}
+
+const char* collide_queries_noalloc(int device, const nsl_collide_query* queries, nsl_local_manifold* out, int32_t n) {
+  void* base = nullptr;
+  if (const char* why = no_alloc_scratch(device, &base)) return why;
+  nsl_collide_query* dq = static_cast<nsl_collide_query*>(base);
+  nsl_local_manifold* dout = reinterpret_cast<nsl_local_manifold*>(static_cast<unsigned char*>(base) + kQueryBytes);
+  if (hipMemcpy(dq, queries, (size_t)n * sizeof(nsl_collide_query), hipMemcpyHostToDevice) != hipSuccess)
+    return "copy of the queries failed";
+  launch_collide_queries(dq, dout, n, nullptr);
+  if (hipGetLastError() != hipSuccess) return "collide launch failed";
+  if (hipMemcpy(out, dout, (size_t)n * sizeof(nsl_local_manifold), hipMemcpyDeviceToHost) != hipSuccess)
+    return "copy of the manifolds failed";
+  return nullptr;
 }
 
 }  // namespace nsl
