@@ -552,11 +552,15 @@ void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const St
 
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
                        KernelTimer* t) {
-  // An ordinary launch: `blocks` is clamped by the caller to what fits on the device at
-  // once, so every workgroup is resident while the step runs on its own stream (a
-  // workgroup that never arrives at a barrier would end the launch with the timeout
-  // error, not a hang).
-  NSL_TIMED(t, kKSolve, s, hipLaunchKernelGGL(k_solve_step, dim3(blocks), dim3(kSolveBlock), 0, s, d, plan, p));
+  // A cooperative launch: the runtime starts the grid only when every workgroup can be
+  // resident at once, which the grid barriers need (`blocks` is clamped by the caller to the
+  // occupancy limit); the barriers' bounded wait still turns a stalled grid into an error.
+  void* args[] = {const_cast<DeviceWorld*>(&d), const_cast<SolvePlan*>(&plan), const_cast<StepParams*>(&p)};
+  hipError_t launch = hipSuccess;
+  NSL_TIMED(t, kKSolve, s,
+            launch = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_solve_step), dim3(blocks), dim3(kSolveBlock),
+                                                args, 0, s));
+  if (launch != hipSuccess) throw std::runtime_error(std::string("step kernel launch: ") + hipGetErrorString(launch));
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) throw std::runtime_error(std::string("step kernel launch: ") + hipGetErrorString(err));
 }

@@ -201,6 +201,9 @@ class GpuWorld : public HostScene {
   std::vector<uint64_t> phase_log_host_;
   GridBarrier barrier_host_{};
   int max_blocks_ = 0;
+  // set when a grid barrier wait timed out mid-step; cleared by restore_state
+  bool poisoned_ = false;
+  void check_usable() const;
   double wall_clock_khz_ = 0.0;
   KernelStats kstats_{};
   int32_t counters_host_[kCounterCount] = {};

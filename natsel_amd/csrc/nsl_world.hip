@@ -647,6 +647,7 @@ struct GpuWorld::Snapshot {
 void GpuWorld::destroy_snapshot(Snapshot* s) { delete s; }
 
 GpuWorld::Snapshot* GpuWorld::save_state() {
+  check_usable();
   NSL_CHECK(hipSetDevice(device_));
   flush_locations();
   if (device_newer_) pull_bodies();
@@ -789,12 +790,20 @@ void GpuWorld::restore_state(const Snapshot& s) {
   body_moved_.assign(bodies_.size(), 0);
   for (size_t b = 0; b < bodies_.size(); ++b) mark_body_moved((int32_t)b);
   flush_locations();
+  poisoned_ = false;
 }
 
 // ---------------------------------------------------------------------------------
 // one step: solver_integration.rs:807-1015
 
+void GpuWorld::check_usable() const {
+  if (poisoned_)
+    throw std::runtime_error("the world was left part-way through a step by a grid barrier timeout: restore a snapshot "
+                             "or create the world again");
+}
+
 void GpuWorld::step(float dt, int32_t sub_steps) {
+  check_usable();
   NSL_CHECK(hipSetDevice(device_));
   const double t0 = now_ms();
   if (bodies_dirty_ || shapes_dirty_) upload_bodies();
@@ -982,8 +991,11 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   std::memcpy(counters_host_, readback_->counters, sizeof(counters_host_));
   barrier_host_ = readback_->barrier;
   if (barrier_host_.timed_out) {
+    // the step kernel stopped part-way and store/finalize ran on its partial results: every
+    // later call on this world fails until a snapshot is restored
     barrier_host_ = GridBarrier{};
     NSL_CHECK(hipMemset(d_barrier_, 0, sizeof(GridBarrier)));
+    poisoned_ = true;
     throw std::runtime_error("step kernel: grid barrier wait exceeded its limit (workgroups not co-resident?)");
   }
   // solver_integration.rs:1003-1005
@@ -1106,11 +1118,13 @@ void GpuWorld::reset_kernel_stats() {
 }
 
 void GpuWorld::get_bodies(double* pos, float* rot, float* v, float* w) {
+  check_usable();
   if (device_newer_) pull_bodies();
   HostScene::get_bodies(pos, rot, v, w);
 }
 
 uint64_t GpuWorld::state_hash() {
+  check_usable();
   if (device_newer_) pull_bodies();
   return HostScene::state_hash();
 }
