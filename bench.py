@@ -446,6 +446,8 @@ def main() -> int:
     ap.add_argument("--blocks", type=int, default=0, help="workgroups of the step kernel (0: sized from the largest phase)")
     ap.add_argument("--small-colours", type=int, default=128,
                     help="colours of at most this many items run on one workgroup (0: a grid barrier after every colour)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shape of the step kernel instead of autotuning it after the settle")
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched measurement (0: skip it)")
     ap.add_argument("--batched-steps", type=int, default=240)
     ap.add_argument("--large-world-steps", type=int, default=500, help="steps of the large-world measurement (0: skip it)")
@@ -498,6 +500,9 @@ def main() -> int:
         world.step(scene.dt, scene.substeps)
     torch.cuda.synchronize()
     settle_s = time.perf_counter() - s0
+    # the step kernel's launch shape for the settled pile, by timing (results do not change;
+    # the world is restored to the settled state)
+    tuning = world.autotune(scene.dt, scene.substeps, 5) if args.autotune else None
     hashes = {"settle": world.world_state_hash()}
     _progress(f"pile: {settle} untimed steps in {settle_s:.2f} s")
     snap = world.save_state()
@@ -522,6 +527,7 @@ def main() -> int:
     world.set_kernel_timing(False)
     ks_phases = world.kernel_stats()
     rerun_equal = world.world_state_hash() == hashes["end"]
+    world_opts = {"small_colour_items": world.options.small_colour_items, "stage_records": int(world.options.stage_records)}
     _progress(f"pile: {args.steps} timed steps, {1e3 * elapsed / args.steps:.3f} ms per step")
     del snap
     world.close()  # release device memory now, not at interpreter exit
@@ -547,8 +553,10 @@ def main() -> int:
         "avg_launch_us": 1e3 * ms["solve"] / launches,
         "algorithmic_bytes_per_launch": dom_bytes / launches,
         "grid_barriers_per_launch": ks["barriers"] / launches,
-        "small_colour_items": args.small_colours,
+        "small_colour_items": world_opts["small_colour_items"],
+        "stage_records": world_opts["stage_records"],
         "workgroups": ks["solver_blocks"],
+        "autotune": tuning,
     }
     out = {
         "metric": "body-substeps/s (10k pile)",

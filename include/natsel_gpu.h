@@ -29,6 +29,8 @@
  *                                                               src/collision/convex_manifold.rs:1419-1437,
  *                                                               src/collision/convex_manifold_shapes.rs:21-797
  *   nsl_world_get_profile          World::profile               src/types/world_def.rs:158-184
+ *   nsl_world_autotune             the step kernel's launch shape by timing (no counterpart: the reference
+ *                                  sizes its stage blocks by fixed rules, solver_stages.rs:490-525)
  *   nsl_world_*kernel_*            per-kernel device timing (no counterpart: the reference times phases
  *                                  on the host, solver_stages.rs:551-586)
  *   nsl_batch_*                    many Worlds at once: create_body / create_*_shape / step /
@@ -77,6 +79,8 @@ typedef struct nsl_options {
   int32_t small_colour_items;        /* 128: consecutive colours of at most this many items run on one
                                         workgroup with workgroup barriers, one grid barrier after the
                                         run (0: every colour behind a grid barrier); results do not change */
+  int32_t stage_records;             /* 0: 1 = the step kernel copies each lane's constraint record of the
+                                        next colour to LDS while the current colour runs; results do not change */
 } nsl_options;
 
 typedef struct nsl_body_def {
@@ -330,6 +334,22 @@ int nsl_world_get_profile(nsl_world* world, nsl_profile* out);
    and a download of the marks after each step (phase_ms / phase_count). Results do not
    change. */
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on);
+
+/* The launch shape the autotuner picked (nsl_world_autotune). */
+typedef struct nsl_tuning {
+  int32_t solver_blocks, small_colour_items, stage_records; /* now set in the world's options */
+  double step_kernel_us;    /* average step-kernel time of that shape over the trial steps */
+  double previous_us;       /* the same for the options the world had before the call */
+  int32_t candidates;       /* shapes tried */
+} nsl_tuning;
+/* Picks the step kernel's launch shape for this world as it is now: from a snapshot, runs
+   `steps` steps of (dt, sub_steps) under each candidate (workgroups around the size the
+   widest colour needs, the small-colour threshold 0 / 128 / 256, record staging off / on),
+   times the step kernel with events, checks that every candidate ends on the same state
+   hash (the shape never changes results; NSL_ERR_INTERNAL otherwise), restores the snapshot
+   and keeps the fastest shape in the world's options. No reference counterpart: the
+   reference sizes its stage blocks by fixed rules (solver_stages.rs:490-525). */
+int nsl_world_autotune(nsl_world* world, float dt, int32_t sub_steps, int32_t steps, nsl_tuning* out);
 int nsl_world_reset_kernel_stats(nsl_world* world);
 int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out);
 

@@ -28,6 +28,7 @@ class Options(ctypes.Structure):
         ("solver_blocks", ctypes.c_int32),
         ("enable_restitution_propagation", ctypes.c_int32),
         ("small_colour_items", ctypes.c_int32),
+        ("stage_records", ctypes.c_int32),
     ]
 
 
@@ -170,6 +171,11 @@ class LocalManifold(ctypes.Structure):
                 ("separation", ctypes.c_float * 4), ("feature_id", ctypes.c_uint32 * 4), ("cache", CollideCache)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("solver_blocks", ctypes.c_int32), ("small_colour_items", ctypes.c_int32), ("stage_records", ctypes.c_int32),
+                ("step_kernel_us", ctypes.c_double), ("previous_us", ctypes.c_double), ("candidates", ctypes.c_int32)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [
         ("last_step_ms", ctypes.c_double),
@@ -231,6 +237,7 @@ SIGNATURES = {
     "nsl_world_check_islands": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "nsl_world_get_profile": (ctypes.c_int, [_P, ctypes.POINTER(Profile)]),
     "nsl_world_set_kernel_timing": (ctypes.c_int, [_P, _I]),
+    "nsl_world_autotune": (ctypes.c_int, [_P, _F, _I, _I, ctypes.POINTER(Tuning)]),
     "nsl_world_reset_kernel_stats": (ctypes.c_int, [_P]),
     "nsl_world_get_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStats)]),
     "nsl_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Options), ctypes.POINTER(_P)]),

@@ -59,6 +59,7 @@ void apply(nsl::WorldOptions& o, const nsl_options& in) {
   o.solver_blocks = in.solver_blocks;
   o.enable_restitution_propagation = in.enable_restitution_propagation != 0;
   o.small_colour_items = in.small_colour_items;
+  o.stage_records = in.stage_records != 0 ? 1 : 0;
 }
 
 nsl::Material material(const nsl_material* m) {
@@ -137,6 +138,7 @@ void nsl_default_options(nsl_options* out) {
   out->solver_blocks = o.solver_blocks;
   out->enable_restitution_propagation = o.enable_restitution_propagation;
   out->small_colour_items = o.small_colour_items;
+  out->stage_records = o.stage_records;
 }
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out) {
@@ -463,6 +465,20 @@ int nsl_world_get_profile(nsl_world* world, nsl_profile* out) {
 int nsl_world_set_kernel_timing(nsl_world* world, int32_t on) {
   if (!world) return fail(NSL_ERR_INVALID, "null world");
   return guarded([&] { world->impl->set_kernel_timing((on & 1) != 0, (on & 2) != 0); });
+}
+
+int nsl_world_autotune(nsl_world* world, float dt, int32_t sub_steps, int32_t steps, nsl_tuning* out) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  if (steps < 1 || steps > 1000 || sub_steps < 1 || !(dt > 0.0f)) return fail(NSL_ERR_INVALID, "bad time step or step count");
+  return guarded([&] {
+    const nsl::GpuWorld::Tuning t = world->impl->autotune(dt, sub_steps, steps);
+    out->solver_blocks = t.blocks;
+    out->small_colour_items = t.small_colour_items;
+    out->stage_records = t.stage_records;
+    out->step_kernel_us = t.best_us;
+    out->previous_us = t.previous_us;
+    out->candidates = t.candidates;
+  });
 }
 
 int nsl_world_reset_kernel_stats(nsl_world* world) {

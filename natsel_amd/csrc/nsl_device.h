@@ -120,6 +120,7 @@ struct SolvePlan {
   int32_t any_restitution;
   int32_t restitution_iterations;
   int32_t small_colour_items;     // runs of colours this small go to workgroup 0 (0: off)
+  int32_t stage_records;          // push and relax read their records from LDS copies made a colour ahead
   // Warm start by body (every awake body dynamic): ws_table[c * n_awake + i] is the item of
   // colour c (0..22) that touches awake body i, with the body's side s (0: a, 1: b): a
   // contact slot as (slot << 1 | s), a joint as -((joint id << 1 | s) + 2), or -1. k_prepare
@@ -128,6 +129,9 @@ struct SolvePlan {
   // Null: warm start colour by colour.
   int32_t* ws_table;
   WarmDelta* ws_delta;
+  // one zero WarmDelta after the items' (k_prepare clears it): the body phase loads it for a
+  // colour the body has no item in, so all of a body's loads issue together
+  WarmDelta* ws_zero;
   // With warm start by body and an overflow colour: per awake body, -1 for a body outside the
   // overflow colour (its velocity is integrated by any lane), else the last substep + 1 whose
   // velocity workgroup 0 integrated (0: none yet). The host sets -1; k_prepare marks.

@@ -102,7 +102,7 @@ constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 // make every store issued before the barrier visible to every load after it, across the
 // XCDs' L2 caches. A wait that exceeds the spin limit raises `timed_out` and every
 // workgroup leaves the kernel at its next barrier; returns false then.
-__device__ bool grid_sync(GridBarrier* b) {
+__device__ __forceinline__ bool grid_sync(GridBarrier* b) {
   __shared__ int s_ok;
   __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached L2 before lane 0 releases them
   __syncthreads();
@@ -145,7 +145,7 @@ __device__ void block_add(int32_t* dst, int v) {
 struct PhaseClock {
   uint64_t* log;
   int n;
-  __device__ void mark(int cls) {
+  __device__ __forceinline__ void mark(int cls) {
     if (log != nullptr && blockIdx.x == 0 && threadIdx.x == 0 && n < kMaxPhaseLog) {
       log[2 * n] = (uint64_t)cls;
       log[2 * n + 1] = wall_clock64();
@@ -198,6 +198,18 @@ __device__ __forceinline__ int2 first_indices(const DeviceWorld& d, const Colour
   return make_int2(d.constraints[slot].index_a, d.constraints[slot].index_b);
 }
 
+// Copies constraint record `src` into `rec` (kRecPieces pieces of 16 bytes, piece j of this
+// lane at rec[j * kSolveBlock + threadIdx.x]) with global_load_lds: the copy goes from HBM to
+// LDS without registers and completes by the next grid barrier (its wait for all memory
+// operations), so it overlaps the rest of the phase.
+__device__ __forceinline__ void stage_record(const Constraint* src, float4* rec) {
+  const int wave_base = (int)threadIdx.x & ~63;
+  const char* g = reinterpret_cast<const char*>(src);
+#pragma unroll
+  for (int j = 0; j < kRecPieces; ++j)
+    __builtin_amdgcn_global_load_lds(g + 16 * j, rec + j * kSolveBlock + wave_base, 16, 0, 0);
+}
+
 // One stage over the active colours in order, a grid barrier after each colour or group
 // of small colours. A lane's body indices for its first contact slot in the next colour are
 // read before the barrier, so each phase opens with its body-state loads. `joint` runs the
@@ -205,12 +217,19 @@ __device__ __forceinline__ int2 first_indices(const DeviceWorld& d, const Colour
 // rows). A group of small colours runs on workgroup 0 alone, in colour order with a
 // workgroup barrier between colours (the workgroup shares one L1, so its stores are visible
 // to its loads after the barrier): the same items in the same order, fewer grid barriers.
+// With `rec` (two LDS buffers of kRecPieces * kSolveBlock pieces), a lane's first contact
+// item of a colour reads its record from LDS: the lane copied it there during the previous
+// colour's phase (a lane owns the same slot all launch and is the only writer of its
+// record, so the copy is current). `item(slot, ia, ib, staged, hook)`: `staged` is the LDS
+// copy or null, `hook` issues the next colour's copy after the item's body-state loads.
 template <class JointItem, class Item>
-__device__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid, int nthreads, GridBarrier* bar, PhaseClock& clock,
-                            int cls, bool with_joints, JointItem joint, Item item) {
+__device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid, int nthreads, GridBarrier* bar, PhaseClock& clock,
+                            int cls, bool with_joints, JointItem joint, Item item, float4* rec = nullptr) {
   const int n_active = t.n;
   if (n_active == 0) return true;
   int2 pf = first_indices(d, t, 0, tid, with_joints);
+  int cur = 0;             // rec buffer of this colour's copy
+  bool staged = false;     // the lane's first contact item of colour a is in rec[cur]
   for (int a = 0; a < n_active;) {
     const int g = t.group_end[a];
     if (g > a + 1) {
@@ -224,28 +243,49 @@ __device__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid,
               joint(d.joint_ids[jb + k]);
             } else {
               const int slot = cb + k;
-              item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
+              item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b, nullptr, NoHook{});
             }
           }
           if (c + 1 < g) __syncthreads();
         }
       }
       a = g;
+      staged = false;
     } else {
       const int nj = joints_in(t, a, with_joints);
       const int jb = t.jbegin[a], cb = t.begin[a] - nj;
       const int n_items = nj + (t.end[a] - t.begin[a]);
+      // the lane's first contact slot of the next colour when that colour runs on its own
+      int next_slot = -1;
+      if (rec != nullptr && a + 1 < n_active && t.group_end[a + 1] == a + 2) {
+        const int nj2 = joints_in(t, a + 1, with_joints);
+        const int s2 = t.begin[a + 1] + tid - nj2;
+        if (tid >= nj2 && s2 < t.end[a + 1]) next_slot = s2;
+      }
+      float4* next_rec = rec != nullptr ? rec + (cur ^ 1) * (kRecPieces * kSolveBlock) : nullptr;
+      auto hook = [&]() {
+        if (next_slot >= 0) stage_record(d.constraints + next_slot, next_rec);
+      };
+      const float4* mine = staged ? rec + cur * (kRecPieces * kSolveBlock) : nullptr;
       int k = tid;
-      if (k < nj) joint(d.joint_ids[jb + k]);
-      else if (k < n_items) item(cb + k, pf.x, pf.y);
+      if (k < nj) {
+        joint(d.joint_ids[jb + k]);
+        hook();
+      } else if (k < n_items) {
+        item(cb + k, pf.x, pf.y, mine, hook);
+      } else {
+        hook();
+      }
       for (k += nthreads; k < n_items; k += nthreads) {
         if (k < nj) {
           joint(d.joint_ids[jb + k]);
         } else {
           const int slot = cb + k;
-          item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b);
+          item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b, nullptr, NoHook{});
         }
       }
+      staged = next_slot >= 0;
+      cur ^= 1;
       a += 1;
     }
     if (a < n_active) pf = first_indices(d, t, a, tid, with_joints);
@@ -258,7 +298,7 @@ __device__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid,
 // The overflow colour's levels, run by workgroup 0 (one workgroup barrier per level; the
 // workgroup shares one L1, so its stores are visible to its loads after the barrier).
 template <class JointItem, class Item>
-__device__ void run_overflow_levels(const SolvePlan& plan, JointItem joint, Item item) {
+__device__ __forceinline__ void run_overflow_levels(const SolvePlan& plan, JointItem joint, Item item) {
   if (blockIdx.x != 0) return;
   for (int l = 0; l < plan.n_ovf_levels; ++l) {
     const int b = plan.ovf_level_begin[l], e = plan.ovf_level_begin[l + 1];
@@ -285,6 +325,9 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   clock.mark(-1);
 
   __shared__ ColourTable colours;
+  // the staged constraint records of push and relax (two buffers, one per colour in flight)
+  __shared__ float4 s_rec[2 * kRecPieces * kSolveBlock];
+  float4* rec = plan.stage_records ? s_rec : nullptr;
   if (threadIdx.x == 0) {
     int n = 0;
     for (int ci = 0; ci < kOverflowColor; ++ci)
@@ -350,8 +393,18 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
           const int32_t jid = d.joint_ids[k];
           warm_delta_joint(d, d.joints[jid], jd + 2 * jid);
         }
-        for (int slot = tid; slot < plan.n_constraints; slot += nthreads)
-          warm_delta_contact(d.constraints[slot], plan.ws_delta + 2 * slot);
+        // four constraints per lane at a time, their records loaded before any is computed
+        // (the phase is a chain of memory round trips otherwise)
+        constexpr int kU = 4;
+        int slot = tid;
+        for (; slot + (kU - 1) * nthreads < plan.n_constraints; slot += kU * nthreads) {
+          Constraint c[kU];
+#pragma unroll
+          for (int u = 0; u < kU; ++u) c[u] = d.constraints[slot + u * nthreads];
+#pragma unroll
+          for (int u = 0; u < kU; ++u) warm_delta_contact(c[u], plan.ws_delta + 2 * (slot + u * nthreads));
+        }
+        for (; slot < plan.n_constraints; slot += nthreads) warm_delta_contact(d.constraints[slot], plan.ws_delta + 2 * slot);
       }
       if (!grid_sync(bar)) return;
       clock.mark(kPWarmStart);
@@ -360,11 +413,19 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
 #pragma unroll
         for (int c = 0; c < kOverflowColor; ++c) e[c] = plan.ws_table[c * d.n_awake + i];
         BodyState s = d.states[i];
+        // every colour's delta loaded at once (the zero record for a colour without an item)
+        WarmDelta wd[kOverflowColor];
+#pragma unroll
+        for (int c = 0; c < kOverflowColor; ++c) {
+          const int32_t k = e[c] >= 0 ? e[c] : -e[c] - 2;  // (item << 1) | side
+          const WarmDelta* src = e[c] == kNullIndex ? plan.ws_zero : (e[c] >= 0 ? plan.ws_delta + k : jd + k);
+          wd[c] = *src;
+        }
 #pragma unroll
         for (int c = 0; c < kOverflowColor; ++c) {
           if (e[c] == kNullIndex) continue;
-          const int32_t k = e[c] >= 0 ? e[c] : -e[c] - 2;  // (item << 1) | side
-          const WarmDelta w = e[c] >= 0 ? plan.ws_delta[k] : jd[k];
+          const int32_t k = e[c] >= 0 ? e[c] : -e[c] - 2;
+          const WarmDelta w = wd[c];
           if (k & 1) {
             s.w = add(s.w, w.dw);
             s.v = add(s.v, w.dv);
@@ -385,7 +446,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       }
       if (!run_colours(d, colours, tid, nthreads, bar, clock, kPWarmStart, true,
                        [&](int32_t jid) { warm_start_joint_item(d, jid); },
-                       [&](int slot, int32_t ia, int32_t ib) { warm_start_item(d, slot, ia, ib); }))
+                       [&](int slot, int32_t ia, int32_t ib, const float4*, auto) { warm_start_item(d, slot, ia, ib); }))
         return;
     }
 
@@ -397,7 +458,10 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     }
     if (!run_colours(d, colours, tid, nthreads, bar, clock, kPPush, true,
                      [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
-                     [&](int slot, int32_t ia, int32_t ib) { push_item(d, slot, p, ia, ib); }))
+                     [&](int slot, int32_t ia, int32_t ib, const float4* staged, auto hook) {
+                       push_item(d, slot, p, ia, ib, staged, kSolveBlock, hook);
+                     },
+                     rec))
       return;
 
     for (int i = tid; i < d.n_awake; i += nthreads) integrate_positions_item(d, i, p);
@@ -412,7 +476,10 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     }
     if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRelax, true,
                      [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
-                     [&](int slot, int32_t ia, int32_t ib) { relax_item(d, slot, p, ia, ib); }))
+                     [&](int slot, int32_t ia, int32_t ib, const float4* staged, auto hook) {
+                       relax_item(d, slot, p, ia, ib, staged, kSolveBlock, hook);
+                     },
+                     rec))
       return;
   }
 
@@ -424,7 +491,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         clock.mark(kPOverflow);
       }
       if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRestitution, false, no_joint,
-                       [&](int slot, int32_t ia, int32_t ib) { restitution_item(d, slot, p, ia, ib); }))
+                       [&](int slot, int32_t ia, int32_t ib, const float4*, auto) { restitution_item(d, slot, p, ia, ib); }))
         return;
     }
   }
@@ -443,6 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
     // the step kernel's barrier counts its arrivals from zero (stream order: the previous
     // step kernel has finished, the next starts after this kernel)
     __hip_atomic_store(&plan.barrier->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (plan.ws_zero != nullptr) *plan.ws_zero = WarmDelta{zero3(), zero3()};
   }
   for (int k = tid; k < plan.n_joints; k += nthreads) {
     const int32_t jid = d.joint_ids[k];

@@ -58,6 +58,9 @@ struct WorldOptions {
   // consecutive colours of at most this many items (joints and contacts) run on one
   // workgroup with workgroup barriers between them; 0 = every colour behind a grid barrier
   int32_t small_colour_items = 128;
+  // 1: the step kernel copies each lane's constraint record of the next colour to LDS while
+  // the current colour runs (results do not change)
+  int32_t stage_records = 0;
 };
 
 

@@ -549,10 +549,25 @@ __device__ __forceinline__ void warm_delta_contact(const Constraint& c, WarmDelt
   out[1] = WarmDelta{scale(c.inv_mass_b, wi.linear), sym_mv(c.inv_ib, wi.ang_b)};
 }
 
-// contact_solver.rs:796-894
-__device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib) {
-  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  Constraint& c = d.constraints[slot];
+// A constraint record staged in LDS by the step kernel: 16-byte piece j of the record of
+// lane `lane` at rec[j * lanes + lane] (written by global_load_lds, one piece per lane per
+// instruction, so consecutive lanes hold consecutive pieces).
+constexpr int kRecPieces = (int)(sizeof(Constraint) / 16);
+static_assert(sizeof(Constraint) % 16 == 0, "constraint records move in 16-byte pieces");
+
+__device__ __forceinline__ Constraint staged_record(const float4* rec, int lanes, int lane) {
+  float4 piece[kRecPieces];
+#pragma unroll
+  for (int j = 0; j < kRecPieces; ++j) piece[j] = rec[j * lanes + lane];
+  Constraint c;
+  __builtin_memcpy(&c, piece, sizeof(Constraint));
+  return c;
+}
+
+// contact_solver.rs:796-894. Reads the record from `c` and writes its impulses to `out`
+// (the same record, or `c` a staged copy of it).
+__device__ __forceinline__ void push_core(const Constraint& c, Constraint& out, const StepParams& p, BodyState& ba,
+                                          BodyState& bb) {
   const int pc = c.point_count;
   ConstraintPoint pt[4];
 #pragma unroll
@@ -583,7 +598,7 @@ __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const 
       float v_n = lnv + (dot(u_b, bb.w) - dot(u_a, ba.w));
       float neg_impulse = cp.normal_mass * (pm * v_n + velocity_bias) + pi * cp.normal_impulse;
       float impulse = maxf(-neg_impulse, -cp.normal_impulse);
-      c.points[i].normal_impulse = cp.normal_impulse + impulse;
+      out.points[i].normal_impulse = cp.normal_impulse + impulse;
       lnv += inv_mass * impulse;
       manifold_impulse += impulse;
       ba.w = sub(ba.w, scale(impulse, g_a));
@@ -593,15 +608,36 @@ __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const 
     ba.v = sub(ba.v, scale(c.inv_mass_a, pp));
     bb.v = add(bb.v, scale(c.inv_mass_b, pp));
   }
+}
+
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
+// `staged`: the record's copy in LDS (rec, lanes, lane) or null to read it from HBM. `hook`
+// runs right after the body-state loads are issued (the step kernel issues the next colour's
+// record copy there, so that copy overlaps this item's work).
+template <class Hook = NoHook>
+__device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib,
+                                          const float4* staged = nullptr, int lanes = 0, Hook hook = Hook{}) {
+  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
+  hook();
+  Constraint& out = d.constraints[slot];
+  if (staged != nullptr) {
+    const Constraint c = staged_record(staged, lanes, (int)threadIdx.x);
+    push_core(c, out, p, ba, bb);
+  } else {
+    push_core(out, out, p, ba, bb);
+  }
   store_state(d, ia, ba);
   store_state(d, ib, bb);
 }
 
-// contact_solver.rs:927-1097
-__device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib) {
-  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  Constraint& c = d.constraints[slot];
+// contact_solver.rs:927-1097 (reads `c`, writes `out` as push_core)
+__device__ __forceinline__ void relax_core(const Constraint& c, Constraint& out, const StepParams& p, BodyState& ba,
+                                           BodyState& bb) {
   const int pc = c.point_count;
+  Vec3 twist_a, twist_b;
   ConstraintPoint pt[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) pt[i] = c.points[i];
@@ -629,8 +665,8 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       float neg_impulse = cp.normal_mass * (v_n + velocity_bias);
       float impulse = maxf(-neg_impulse, -cp.normal_impulse);
       float new_impulse = cp.normal_impulse + impulse;
-      c.points[i].normal_impulse = new_impulse;
-      c.points[i].total_normal_impulse = cp.total_normal_impulse + new_impulse;
+      out.points[i].normal_impulse = new_impulse;
+      out.points[i].total_normal_impulse = cp.total_normal_impulse + new_impulse;
       total_normal += new_impulse;
       total_twist_limit += cp.lever_arm * new_impulse;
       lnv += inv_mass * impulse;
@@ -647,7 +683,7 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       bool mask = lsq > eps + max_impulse * max_impulse;
       float nrm = max_impulse / (sqrtf_(lsq) + eps);
       ri = scale(mask ? nrm : 1.0f, ri);
-      c.rolling_impulse = ri;
+      out.rolling_impulse = ri;
       delta = sub(ri, old);
       ba.w = sub(ba.w, sym_mv(c.inv_ia, delta));
       bb.w = add(bb.w, sym_mv(c.inv_ib, delta));
@@ -658,7 +694,7 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       float old = c.twist_impulse;
       float impulse = -(c.twist_mass * twist_speed);
       impulse = maxf(-max_lambda - old, minf(impulse, max_lambda - old));
-      c.twist_impulse = old + impulse;
+      out.twist_impulse = old + impulse;
       ba.w = sub(ba.w, scale(impulse, sym_mv(c.inv_ia, normal)));
       bb.w = add(bb.w, scale(impulse, sym_mv(c.inv_ib, normal)));
     }
@@ -677,7 +713,7 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       float sc = mask ? nrm : 1.0f;
       ni = Vec2{sc * ni.x, sc * ni.y};
       di = Vec2{ni.x - fi_old.x, ni.y - fi_old.y};
-      c.friction_impulse = ni;
+      out.friction_impulse = ni;
       Vec3 pp = scale(di.x, t1);
       pp = add(pp, scale(di.y, t2));
       ba.w = sub(ba.w, sym_mv(c.inv_ia, cross(r_a, pp)));
@@ -686,6 +722,20 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
       ba.v = sub(ba.v, scale(c.inv_mass_a, pp));
       bb.v = add(bb.v, scale(c.inv_mass_b, pp));
     }
+  }
+}
+
+template <class Hook = NoHook>
+__device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib,
+                                           const float4* staged = nullptr, int lanes = 0, Hook hook = Hook{}) {
+  BodyState ba = load_state(d, ia), bb = load_state(d, ib);
+  hook();
+  Constraint& out = d.constraints[slot];
+  if (staged != nullptr) {
+    const Constraint c = staged_record(staged, lanes, (int)threadIdx.x);
+    relax_core(c, out, p, ba, bb);
+  } else {
+    relax_core(out, out, p, ba, bb);
   }
   store_state(d, ia, ba);
   store_state(d, ib, bb);

@@ -126,8 +126,10 @@ struct ConstraintPoint {  // contact_solver.rs:118-147
   float normal_mass, lever_arm;
 };
 
-// ContactConstraint + its single ManifoldConstraint (contact_solver.rs:149-240).
-struct Constraint {
+// ContactConstraint + its single ManifoldConstraint (contact_solver.rs:149-240). 16-byte
+// aligned with a size that is a multiple of 16: the step kernel copies a record to LDS in
+// 16-byte pieces.
+struct alignas(16) Constraint {
   int32_t contact_id, index_a, index_b, point_count;
   int32_t manifold_count;
   uint32_t is_static;

@@ -91,6 +91,13 @@ class GpuWorld : public HostScene {
   // bits, the work counters. Device arrays are copied device to device.
   struct Snapshot;
   Snapshot* save_state();  // owned by the caller; free with destroy_snapshot
+  struct Tuning {
+    int blocks = 0, small_colour_items = 0, stage_records = 0, candidates = 0;
+    double best_us = 0.0, previous_us = 0.0;
+  };
+  // Times candidate launch shapes of the step kernel from a snapshot and keeps the fastest
+  // in `options`; the world's state is unchanged.
+  Tuning autotune(float dt, int32_t sub_steps, int32_t steps);
   static void destroy_snapshot(Snapshot* s);
   // world_state.rs:439-560 restore_state. The world must hold the bodies, shapes and joints
   // it held when the snapshot was taken.

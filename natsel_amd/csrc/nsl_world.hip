@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 
 namespace nsl {
@@ -646,6 +647,72 @@ struct GpuWorld::Snapshot {
 
 void GpuWorld::destroy_snapshot(Snapshot* s) { delete s; }
 
+GpuWorld::Tuning GpuWorld::autotune(float dt, int32_t sub_steps, int32_t steps) {
+  check_usable();
+  std::unique_ptr<Snapshot> snap(save_state());
+  const WorldOptions saved = options;
+  const bool was_timing = timing_, was_phase = phase_timing_;
+  // steps of one candidate from the snapshot: average step-kernel time, end hash, workgroups
+  auto trial = [&](const WorldOptions& o, uint64_t* hash, int* blocks) {
+    options = o;
+    restore_state(*snap);
+    set_kernel_timing(true, false);
+    kstats_ = KernelStats{};
+    for (int k = 0; k < steps; ++k) step(dt, sub_steps);
+    *hash = state_hash();
+    *blocks = (int)kstats_.solver_blocks;
+    return 1e3 * kstats_.ms[kKSolve] / (double)std::max<int64_t>(kstats_.launches[kKSolve], 1);
+  };
+  Tuning t;
+  uint64_t h0 = 0;
+  int d0 = 0;
+  try {
+    t.previous_us = trial(saved, &h0, &d0);
+    const int cap = std::max(1, std::min(max_blocks_, 96));
+    std::vector<int> sizes;
+    for (int num : {2, 3, 4, 5, 6}) {  // d0 x 2/4 .. 6/4
+      const int b = std::max(1, std::min(cap, (d0 * num + 2) / 4));
+      if (std::find(sizes.begin(), sizes.end(), b) == sizes.end()) sizes.push_back(b);
+    }
+    t.best_us = t.previous_us;
+    t.blocks = d0;
+    t.small_colour_items = saved.small_colour_items;
+    t.stage_records = saved.stage_records;
+    for (int b : sizes)
+      for (int small : {0, 128, 256})
+        for (int stage : {0, 1}) {
+          WorldOptions o = saved;
+          o.solver_blocks = b;
+          o.small_colour_items = small;
+          o.stage_records = stage;
+          uint64_t h = 0;
+          int used = 0;
+          const double us = trial(o, &h, &used);
+          t.candidates += 1;
+          if (h != h0) throw std::runtime_error("autotune: a launch shape changed the results (it must not)");
+          if (us < t.best_us) {
+            t.best_us = us;
+            t.blocks = b;
+            t.small_colour_items = small;
+            t.stage_records = stage;
+          }
+        }
+  } catch (...) {
+    options = saved;
+    restore_state(*snap);
+    set_kernel_timing(was_timing, was_phase);
+    throw;
+  }
+  options = saved;
+  options.solver_blocks = t.blocks;
+  options.small_colour_items = t.small_colour_items;
+  options.stage_records = t.stage_records;
+  restore_state(*snap);
+  set_kernel_timing(was_timing, was_phase);
+  reset_kernel_stats();
+  return t;
+}
+
 GpuWorld::Snapshot* GpuWorld::save_state() {
   check_usable();
   NSL_CHECK(hipSetDevice(device_));
@@ -940,16 +1007,19 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   plan.any_restitution = any_restitution ? 1 : 0;
   plan.restitution_iterations = options.restitution_iterations;
   plan.small_colour_items = options.small_colour_items;
+  plan.stage_records = options.stage_records;
   // warm start by body needs every awake body dynamic (a kinematic body can sit in several
   // items of one colour)
   plan.ws_table = nullptr;
   plan.ws_delta = nullptr;
+  plan.ws_zero = nullptr;
   if (kinematic_awake_ == 0) {
     ensure(d_ws_table_, cap_ws_table_, (size_t)kOverflowColor * (size_t)d_.n_awake, false);
     ensure(d_ws_delta_, cap_ws_delta_, 2 * (solve.size() + hjoints_.size()) + 2, false);
     NSL_CHECK(hipMemsetAsync(d_ws_table_, 0xFF, (size_t)kOverflowColor * (size_t)d_.n_awake * sizeof(int32_t), stream_));
     plan.ws_table = d_ws_table_;
     plan.ws_delta = d_ws_delta_;
+    plan.ws_zero = d_ws_delta_ + 2 * (solve.size() + hjoints_.size());
   }
   plan.ovf_done = nullptr;
   if (plan.ws_table != nullptr && plan.n_ovf_levels > 0) {

@@ -95,6 +95,7 @@ class WorldOptions:  # world_runtime.rs:1243-1304
     solver_blocks: int = 0  # workgroups of the step kernel; 0 = sized from the largest phase
     enable_restitution_propagation: bool = False
     small_colour_items: int = 128  # colours this small run on one workgroup (0: off); results do not change
+    stage_records: bool = False  # the step kernel copies next-colour constraint records to LDS; results do not change
 
 
 def _material(d: ShapeDef) -> capi.Material:
@@ -134,6 +135,7 @@ def c_options(o: WorldOptions) -> capi.Options:
     c.solver_blocks = int(o.solver_blocks)
     c.enable_restitution_propagation = int(o.enable_restitution_propagation)
     c.small_colour_items = int(o.small_colour_items)
+    c.stage_records = int(o.stage_records)
     return c
 
 
@@ -318,6 +320,17 @@ class World:
     def set_kernel_timing(self, on: bool, phases: bool = True) -> None:
         """Launch events when `on`; the step kernel's per-phase clock as well when `phases`."""
         capi.check(self._lib.nsl_world_set_kernel_timing(self._h, (1 | (2 if phases else 0)) if on else 0))
+
+    def autotune(self, time_step: float, sub_step_count: int, steps: int = 5) -> dict:
+        """Times candidate launch shapes of the step kernel on `steps` steps from the current
+        state (restored afterwards), keeps the fastest in the options and returns the choice;
+        kernel statistics restart at zero."""
+        t = capi.Tuning()
+        capi.check(self._lib.nsl_world_autotune(self._h, time_step, sub_step_count, steps, ctypes.byref(t)))
+        self._opts.solver_blocks = t.solver_blocks
+        self._opts.small_colour_items = t.small_colour_items
+        self._opts.stage_records = bool(t.stage_records)
+        return {n: getattr(t, n) for n, _ in capi.Tuning._fields_}
 
     def reset_kernel_stats(self) -> None:
         capi.check(self._lib.nsl_world_reset_kernel_stats(self._h))
