@@ -26,6 +26,8 @@ namespace nsl {
 constexpr int kBatchMaxAwake = 256;    // dynamic bodies per world (their states sit in LDS)
 constexpr int kBatchMaxShapes = 4096;  // shapes per world (pair keys pack two 16-bit ids)
 constexpr int kBatchMaxNewPairs = 2048;  // new pairs per world and step (sorted in LDS)
+constexpr int kBatchPairsBlock = 256;    // pair update: four wavefronts per world
+constexpr int kPairsLdsShapes = 512;     // shapes per world whose fat AABBs the pair update keeps in LDS
 constexpr int kBatchLanes = 64;        // one wavefront per world (pair update, solve)
 constexpr int kBatchCollideBlock = 128;  // collide pass: one workgroup per world
 
@@ -66,7 +68,13 @@ struct ContactMeta {  // per contact id: shapes, graph colour and index, livenes
   int32_t shape_a, shape_b, color, local;
   int32_t alive;
   float restitution;
+  int32_t pair_type;  // the collide routine: see kPairTypes
 };
+
+// Collide routines by primary shape (hull > capsule > sphere, contact_creation.rs:88-102):
+// 0 hull-hull, 1 hull-sphere, 2 hull-capsule, 3 capsule-sphere, 4 capsule-capsule,
+// 5 sphere-sphere. The collide launch takes each routine's contacts together.
+constexpr int kPairTypes = 6;
 
 // Recording of a batched rollout for its adjoint (nsl_adjoint.hip): per step and world
 // the prepared constraints in solve order (warm-start impulses included), their points'
@@ -120,6 +128,9 @@ struct BatchDevice {
   BatchWorldDesc* worlds;
   BatchWorldState* wstate;
   int32_t* order;     // worlds in launch order (most bodies first)
+  int2* collide_items;     // this step's live contacts as (world, id): kPairTypes lists of collide_type_cap
+  int32_t* collide_counts; // [kPairTypes] list lengths (written by the pair launch, cleared by the solve launch)
+  int32_t collide_type_cap;
   int32_t* counters;  // as DeviceWorld::counters
   unsigned long long* work;  // [0] contacts collided, [1] constraints prepared, [2] their points, [3] overflow constraints
   int32_t n_worlds;
@@ -127,6 +138,7 @@ struct BatchDevice {
   int32_t restitution_iterations;
   int32_t enable_continuous;
   int32_t any_bullets;  // a dynamic bullet body in some world: the bullet pass runs
+  int32_t collide_blocks;  // workgroups of the collide launch (as many as are resident at once)
   BatchTape tape;
   int32_t tape_step;  // step of the tape to record this launch (-1: not recording)
 };

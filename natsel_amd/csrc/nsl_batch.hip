@@ -124,24 +124,14 @@ __device__ __forceinline__ void colour_set(const WorldRefs& r, int c, int body, 
   else w &= ~(1u << (body & 31));
 }
 
-// contact_creation.rs:219-425 (one lane): the primary shape is hull > capsule > sphere.
-__device__ void create_contact(const WorldRefs& r, int32_t a, int32_t b, bool enable_speculative) {
+// contact_creation.rs:219-425 for the contact of shapes a, b that takes id `id` and entry
+// `local` of the non-touching list; the primary shape is hull > capsule > sphere.
+__device__ void create_contact_at(const WorldRefs& r, int32_t a, int32_t b, int32_t id, int32_t local, bool enable_speculative) {
   auto rank = [](int32_t kind) { return kind == kShapeHull ? 2 : (kind == kShapeCapsule ? 1 : 0); };
   if (rank(r.shapes[a].kind) < rank(r.shapes[b].kind)) {
     const int32_t t = a;
     a = b;
     b = t;
-  }
-  BatchWorldState* ws = r.ws;
-  int32_t id;
-  if (ws->n_free > 0) {
-    id = r.free_ids[--ws->n_free];
-  } else {
-    if (ws->next_id >= r.contact_cap) {
-      ws->error = kBatchContactsFull;
-      return;
-    }
-    id = ws->next_id++;
   }
   const Shape& sa = r.shapes[a];
   const Shape& sb = r.shapes[b];
@@ -153,11 +143,13 @@ __device__ void create_contact(const WorldRefs& r, int32_t a, int32_t b, bool en
   m.shape_a = a;
   m.shape_b = b;
   m.color = kNullIndex;
-  m.local = ws->n_non_touching;
+  m.local = local;
   m.alive = 1;
   m.restitution = maxf(sa.restitution, sb.restitution);
+  m.pair_type = sa.kind == kShapeHull ? (sb.kind == kShapeHull ? 0 : (sb.kind == kShapeSphere ? 1 : 2))
+                                      : (sa.kind == kShapeCapsule ? (sb.kind == kShapeSphere ? 3 : 4) : 5);
   r.meta[id] = m;
-  r.non_touching[ws->n_non_touching++] = id;
+  r.non_touching[local] = id;
   Contact c{};
   c.shape_a = a;
   c.shape_b = b;
@@ -274,12 +266,12 @@ __device__ void apply_change(const WorldRefs& r, int32_t id, uint8_t ch) {
 }
 
 // Pair test of the update (contact_creation.rs:901-1042, shape_runtime.rs:285-292).
-__device__ __forceinline__ bool new_pair(const WorldRefs& r, int m, const Proxy& pm, const Aabb& fm, int j) {
+__device__ __forceinline__ bool new_pair(const WorldRefs& r, int m, const Proxy& pm, const Aabb& fm, int j, const Aabb& fj) {
+  if (!aabb_overlaps(fm, fj)) return false;
   const Proxy pj = r.proxy[j];
   if (pm.body_type != kBodyDynamic && pj.body_type != kBodyDynamic) return false;
   if (pm.body == pj.body) return false;
   if ((pm.flags | pj.flags) & kProxySensor) return false;  // contact_creation.rs:187
-  if (!aabb_overlaps(fm, r.fat[j])) return false;
   if (pm.group == pj.group && pm.group != 0) {
     if (pm.group < 0) return false;
   } else if (!((pm.mask & pj.category) != 0 && (pm.category & pj.mask) != 0)) {
@@ -288,16 +280,17 @@ __device__ __forceinline__ bool new_pair(const WorldRefs& r, int m, const Proxy&
   return !pair_bit(r, m < j ? m : j, m < j ? j : m);
 }
 
-// Ascending sort of n keys in LDS by one wavefront (bitonic over the next power of two;
+// Ascending sort of n keys in LDS by the workgroup (bitonic over the next power of two;
 // the padding holds all-ones keys).
-__device__ void sort_keys(uint32_t* keys, int n, int lane) {
+__device__ void sort_keys(uint32_t* keys, int n) {
+  const int tid = threadIdx.x, nt = blockDim.x;
   int p = 1;
   while (p < n) p <<= 1;
-  for (int i = n + lane; i < p; i += kBatchLanes) keys[i] = ~0u;
-  wave_sync();
+  for (int i = n + tid; i < p; i += nt) keys[i] = ~0u;
+  __syncthreads();
   for (int k = 2; k <= p; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < p; i += kBatchLanes) {
+      for (int i = tid; i < p; i += nt) {
         const int x = i ^ j;
         if (x > i) {
           const uint32_t a = keys[i], b = keys[x];
@@ -308,7 +301,7 @@ __device__ void sort_keys(uint32_t* keys, int n, int lane) {
           }
         }
       }
-      wave_sync();
+      __syncthreads();
     }
   }
 }
@@ -319,82 +312,151 @@ struct Colours {
   int end[kOverflowColor];
 };
 
-// Pair update of one world (contact_creation.rs:901-1042): every moved proxy against
-// every shape, the new keys sorted, contacts created in key order by one lane.
-__global__ __launch_bounds__(kBatchLanes) void k_batch_pairs(BatchDevice b) {
+// Pair update of one world (contact_creation.rs:901-1042) by a workgroup of four
+// wavefronts: every moved proxy against every shape (each wavefront takes every fourth
+// 64-shape chunk of moved proxies; the fat AABBs and moved flags sit in LDS), the new keys
+// sorted, and the contacts created in key order: the k-th key takes the k-th id the
+// reference's pool hands out (the free list from its top, then fresh ids) and the k-th
+// free entry of the non-touching list, so all keys are created at once.
+__global__ __launch_bounds__(kBatchPairsBlock) void k_batch_pairs(BatchDevice b) {
   __shared__ uint32_t s_keys[kBatchMaxNewPairs];
-  __shared__ int s_count;
-  const int lane = threadIdx.x;
-  if (blockIdx.x == 0 && lane == 0) b.counters[kCntFastList] = 0;  // the solve launch lists this step's fast bodies
+  __shared__ float4 s_lo[kPairsLdsShapes], s_hi[kPairsLdsShapes];  // fat AABB; s_lo.w: 1 if the proxy moved
+  __shared__ int s_count, s_type_count[kPairTypes], s_type_next[kPairTypes];
+  const int tid = threadIdx.x, lane = tid % kBatchLanes, wave = tid / kBatchLanes;
+  constexpr int kWaves = kBatchPairsBlock / kBatchLanes;
+  if (blockIdx.x == 0 && tid == 0) b.counters[kCntFastList] = 0;  // the solve launch lists this step's fast bodies
   const int world = b.order[blockIdx.x];
   const BatchWorldDesc wd = b.worlds[world];
   const WorldRefs r = world_refs(b, wd, world);
   BatchWorldState* ws = r.ws;
   if (ws->error != kBatchOk) return;
-  if (lane == 0) s_count = 0;
-  wave_sync();
-  for (int base = 0; base < r.n_shapes; base += kBatchLanes) {
-    const bool mv = base + lane < r.n_shapes && r.moved[base + lane] != 0;
-    uint64_t mset = wave_ballot(mv);
+  const int n_shapes = r.n_shapes;
+  if (tid == 0) s_count = 0;
+  if (tid < kPairTypes) s_type_count[tid] = 0;
+  for (int j = tid; j < n_shapes && j < kPairsLdsShapes; j += kBatchPairsBlock) {
+    const Aabb f = r.fat[j];
+    s_lo[j] = make_float4(f.lo.x, f.lo.y, f.lo.z, r.moved[j] != 0 ? 1.0f : 0.0f);
+    s_hi[j] = make_float4(f.hi.x, f.hi.y, f.hi.z, 0.0f);
+  }
+  __syncthreads();
+  auto fat_of = [&](int j) {
+    if (j < kPairsLdsShapes) {
+      const float4 lo = s_lo[j], hi = s_hi[j];
+      return Aabb{Vec3{lo.x, lo.y, lo.z}, Vec3{hi.x, hi.y, hi.z}};
+    }
+    return r.fat[j];
+  };
+  auto moved_of = [&](int j) { return j < kPairsLdsShapes ? s_lo[j].w != 0.0f : r.moved[j] != 0; };
+  for (int base = wave * kBatchLanes; base < n_shapes; base += kWaves * kBatchLanes) {
+    uint64_t mset = wave_ballot(base + lane < n_shapes && moved_of(base + lane));
     while (mset) {
       const int m = base + (__ffsll((unsigned long long)mset) - 1);
       mset &= mset - 1;
       const Proxy pm = r.proxy[m];
-      const Aabb fm = r.fat[m];
-      for (int j0 = 0; j0 < r.n_shapes; j0 += kBatchLanes) {
+      const Aabb fm = fat_of(m);
+      for (int j0 = 0; j0 < n_shapes; j0 += kBatchLanes) {
         const int j = j0 + lane;
         // a pair of two moved proxies is found from the lower one
-        const bool hit = j < r.n_shapes && j != m && (j > m || r.moved[j] == 0) && new_pair(r, m, pm, fm, j);
+        const bool hit = j < n_shapes && j != m && (j > m || !moved_of(j)) && new_pair(r, m, pm, fm, j, fat_of(j));
         const uint64_t hits = wave_ballot(hit);
         if (hits == 0) continue;
-        const int base_count = s_count;
-        const int total = __popcll(hits);
-        if (base_count + total > kBatchMaxNewPairs) {
-          if (lane == 0) ws->error = kBatchPairsFull;
-          return;
-        }
-        if (hit) {
+        int at = 0;
+        if (lane == 0) at = atomicAdd(&s_count, __popcll(hits));
+        at = __shfl(at, 0, kBatchLanes);
+        const int k = at + lanes_below(hits, lane);
+        if (hit && k < kBatchMaxNewPairs) {
           const int lo = m < j ? m : j, hi = m < j ? j : m;
-          s_keys[base_count + lanes_below(hits, lane)] = ((uint32_t)lo << 16) | (uint32_t)hi;
+          s_keys[k] = ((uint32_t)lo << 16) | (uint32_t)hi;
         }
-        wave_sync();
-        if (lane == 0) s_count = base_count + total;
-        wave_sync();
       }
     }
+  }
+  __syncthreads();
+  if (s_count > kBatchMaxNewPairs) {
+    if (tid == 0) ws->error = kBatchPairsFull;
+    return;
   }
   const int n_keys = s_count;
   if (n_keys > 0) {
-    sort_keys(s_keys, n_keys, lane);
-    if (lane == 0)
-      for (int k = 0; k < n_keys && ws->error == kBatchOk; ++k)
-        create_contact(r, (int32_t)(s_keys[k] >> 16), (int32_t)(s_keys[k] & 0xffffu), b.enable_speculative != 0);
+    sort_keys(s_keys, n_keys);
+    const int n_free = ws->n_free, next_id = ws->next_id, n_nt = ws->n_non_touching;
+    // keys past the pool's capacity are not created and the world stops (contact_creation.rs:219-425)
+    const int fresh = mini(maxi(n_keys - n_free, 0), maxi(r.contact_cap - next_id, 0));
+    const int n_create = mini(n_keys, n_free) + fresh;
+    for (int k = tid; k < n_create; k += kBatchPairsBlock) {
+      const int32_t id = k < n_free ? r.free_ids[n_free - 1 - k] : next_id + (k - n_free);
+      create_contact_at(r, (int32_t)(s_keys[k] >> 16), (int32_t)(s_keys[k] & 0xffffu), id, n_nt + k, b.enable_speculative != 0);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      ws->n_free = maxi(n_free - n_keys, 0);
+      ws->next_id = next_id + fresh;
+      ws->n_non_touching = n_nt + n_create;
+      if (n_create < n_keys) ws->error = kBatchContactsFull;
+    }
   }
-  for (int s = lane; s < r.n_shapes; s += kBatchLanes) r.moved[s] = 0;
+  for (int s = tid; s < n_shapes; s += kBatchPairsBlock) r.moved[s] = 0;
+  __syncthreads();
+  if (ws->error != kBatchOk) return;
+  // the collide launch's work: this world's live contacts appended to their routine's list,
+  // counted first so each list takes one atomic per world
+  const int n_ids = ws->next_id;
+  auto type_of = [&](int id) { return id < n_ids && r.meta[id].alive ? r.meta[id].pair_type : -1; };
+  for (int base = wave * kBatchLanes; base < n_ids; base += kWaves * kBatchLanes) {
+    const int type = type_of(base + lane);
+    for (int t = 0; t < kPairTypes; ++t) {
+      const int n = __popcll(wave_ballot(type == t));
+      if (lane == 0 && n > 0) atomicAdd(&s_type_count[t], n);
+    }
+  }
+  __syncthreads();
+  if (tid < kPairTypes) s_type_next[tid] = s_type_count[tid] > 0 ? atomicAdd(&b.collide_counts[tid], s_type_count[tid]) : 0;
+  __syncthreads();
+  for (int base = wave * kBatchLanes; base < n_ids; base += kWaves * kBatchLanes) {
+    const int id = base + lane;
+    const int type = type_of(id);
+    if (id < n_ids && type < 0) r.change[id] = kChangeNone;
+    for (int t = 0; t < kPairTypes; ++t) {
+      const uint64_t m = wave_ballot(type == t);
+      if (m == 0) continue;
+      int at = 0;
+      if (lane == 0) at = atomicAdd(&s_type_next[t], __popcll(m));
+      at = __shfl(at, 0, kBatchLanes);
+      if (type == t) b.collide_items[(size_t)t * b.collide_type_cap + at + lanes_below(m, lane)] = make_int2(world, id);
+    }
+  }
 }
 
-// Collide pass of one world over its live contacts (collide_pass.rs:541-742).
-__global__ __launch_bounds__(kBatchCollideBlock) void k_batch_collide(BatchDevice b) {
-  const int world = b.order[blockIdx.x];
-  const BatchWorldDesc wd = b.worlds[world];
-  const WorldRefs r = world_refs(b, wd, world);
-  if (r.ws->error != kBatchOk) return;
-  const DeviceWorld dg = world_view(b, wd, b.states + wd.awake_off);
-  const int next_id = r.ws->next_id;
+// Collide pass (collide_pass.rs:541-742) over the lists the pair launch wrote: one lane per
+// live contact, the contacts of one collide routine together so a wavefront's lanes take
+// the same code path, and the contacts of every world spread over the whole device
+// whatever the world sizes. The grid is as large as can be resident at once and strides
+// over the lists.
+__global__ __launch_bounds__(kBatchCollideBlock, 2) void k_batch_collide(BatchDevice b) {
+  int count[kPairTypes], total = 0;
+  for (int t = 0; t < kPairTypes; ++t) {
+    count[t] = b.collide_counts[t];
+    total += count[t];
+  }
   int collided = 0;
-  for (int id = threadIdx.x; id < next_id; id += blockDim.x) {
-    uint8_t ch = kChangeNone;
-    if (r.meta[id].alive) {
-      ++collided;
-      ch = collide_contact(dg, id);
-      if (ch == kChangeDisjoint) {
-        const int a = r.meta[id].shape_a, c = r.meta[id].shape_b;
-        set_pair_bit(r, a < c ? a : c, a < c ? c : a, false);
-      }
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+    int t = 0, first = 0;
+    while (k >= first + count[t]) first += count[t++];
+    const int2 item = b.collide_items[(size_t)t * b.collide_type_cap + (k - first)];
+    const int world = item.x, id = item.y;
+    const BatchWorldDesc wd = b.worlds[world];
+    const WorldRefs r = world_refs(b, wd, world);
+    const DeviceWorld dg = world_view(b, wd, b.states + wd.awake_off);
+    ++collided;
+    const uint8_t ch = collide_contact(dg, id);
+    if (ch == kChangeDisjoint) {
+      const int a = r.meta[id].shape_a, c = r.meta[id].shape_b;
+      set_pair_bit(r, a < c ? a : c, a < c ? c : a, false);
     }
     r.change[id] = ch;
   }
-  if (collided) atomicAdd(&b.work[0], (unsigned long long)collided);
+  for (int off = kBatchLanes / 2; off > 0; off >>= 1) collided += __shfl_xor(collided, off, kBatchLanes);
+  if (threadIdx.x % kBatchLanes == 0 && collided) atomicAdd(&b.work[0], (unsigned long long)collided);
 }
 
 // Contact transitions in id order (one lane), the solve order, then prepare, the
@@ -493,6 +555,7 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
   __shared__ Colours colours;
   __shared__ int s_count, s_any;
   const int lane = threadIdx.x;
+  if (blockIdx.x == 0 && lane < kPairTypes) b.collide_counts[lane] = 0;  // the next pair launch lists anew
   const int world = b.order[blockIdx.x];
   const BatchWorldDesc wd = b.worlds[world];
   const WorldRefs r = world_refs(b, wd, world);
@@ -660,6 +723,15 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_continuous(BatchDevice b,
   if (hits) atomicAdd(&b.counters[kCntToiHits], hits);
 }
 
+// Workgroups of the collide launch: as many as the device keeps resident at once.
+int collide_grid(int device) {
+  int per_cu = 0, cus = 0;
+  BATCH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_batch_collide),
+                                                           kBatchCollideBlock, 0));
+  BATCH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  return std::max(1, per_cu) * std::max(1, cus);
+}
+
 }  // namespace
 
 void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hipStream_t s) {
@@ -667,8 +739,8 @@ void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hip
   for (int k = 0; k < steps; ++k) {
     BatchDevice bk = b;
     bk.tape_step = b.tape_step >= 0 ? b.tape_step + k : -1;
-    hipLaunchKernelGGL(k_batch_pairs, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, bk);
-    hipLaunchKernelGGL(k_batch_collide, dim3(b.n_worlds), dim3(kBatchCollideBlock), 0, s, bk);
+    hipLaunchKernelGGL(k_batch_pairs, dim3(b.n_worlds), dim3(kBatchPairsBlock), 0, s, bk);
+    hipLaunchKernelGGL(k_batch_collide, dim3(b.collide_blocks), dim3(kBatchCollideBlock), 0, s, bk);
     hipLaunchKernelGGL(k_batch_solve, dim3(b.n_worlds), dim3(kBatchLanes), 0, s, bk, p);
     if (b.enable_continuous) {
       const int blocks = std::max(1, std::min(b.n_worlds, 1024));
@@ -846,6 +918,10 @@ void BatchStepper::upload() {
   b.worlds = upload_vec(stream_, alloc<BatchWorldDesc>(nw, false), desc_);
   b.wstate = alloc<BatchWorldState>(nw, true);
   b.order = upload_vec(stream_, alloc<int32_t>(nw, false), order);
+  b.collide_type_cap = (int32_t)std::max<int64_t>(contact_total, 1);
+  b.collide_items = alloc<int2>((size_t)kPairTypes * b.collide_type_cap, false);
+  b.collide_counts = alloc<int32_t>(kPairTypes, true);
+  b.collide_blocks = collide_grid(device_);
   b.counters = alloc<int32_t>(kCounterCount, true);
   b.work = alloc<unsigned long long>(4, true);
   BATCH_CHECK(hipStreamSynchronize(stream_));  // the host vectors are read by the copies

@@ -247,10 +247,15 @@ __device__ AxisQuery separating_axis(const BoxHull& ha, const BoxHull& hb, Xf xf
   return res;
 }
 
-// convex_manifold.rs:905-1062
-__device__ void reduce_points(LocalManifold& m, LocalPoint* pts, int count) {
+__device__ __forceinline__ LocalPoint local_point(const ClipVertex& v) { return LocalPoint{v.position, v.separation, v.pair}; }
+
+// convex_manifold.rs:905-1062. The candidate points arrive in a clip buffer whose
+// positions already lie midway between the two surfaces.
+__device__ void reduce_points(LocalManifold& m, ClipVertex* pts, int count) {
   if (count <= 4) {
-    for (int i = 0; i < count; ++i) m.pts[i] = pts[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < count) m.pts[i] = local_point(pts[i]);
     m.point_count = count;
     return;
   }
@@ -261,11 +266,11 @@ __device__ void reduce_points(LocalManifold& m, LocalPoint* pts, int count) {
   Vec3 search = arbitrary_perp(normal);
   for (int i = 0; i < count; ++i) {
     if (pts[i].separation > spec) continue;
-    float score = -pts[i].separation + dot(search, pts[i].point);
+    float score = -pts[i].separation + dot(search, pts[i].position);
     if (bias * score > best_score) { best = i; best_score = score; }
   }
   if (best == kNullIndex) { m.point_count = 0; return; }
-  m.pts[0] = pts[best];
+  m.pts[0] = local_point(pts[best]);
   m.point_count = 1;
   pts[best] = pts[count - 1];
   count -= 1;
@@ -273,14 +278,14 @@ __device__ void reduce_points(LocalManifold& m, LocalPoint* pts, int count) {
   best_score = 0.0f;
   best = kNullIndex;
   for (int i = 0; i < count; ++i) {
-    Vec3 d = sub(pts[i].point, a);
+    Vec3 d = sub(pts[i].position, a);
     Vec3 v = mul_sub(d, dot(d, normal), normal);
     float s = maxf(0.0f, -pts[i].separation);
     float score = length_sq(v) + 4.0f * s * s;
     if (bias * score > best_score) { best_score = score; best = i; }
   }
   if (best_score < tol_sqr) return;
-  m.pts[1] = pts[best];
+  m.pts[1] = local_point(pts[best]);
   m.point_count = 2;
   pts[best] = pts[count - 1];
   count -= 1;
@@ -290,12 +295,12 @@ __device__ void reduce_points(LocalManifold& m, LocalPoint* pts, int count) {
   float best_area = 0.0f;
   Vec3 ba = sub(b, a);
   for (int i = 0; i < count; ++i) {
-    float area = dot(normal, cross(ba, sub(pts[i].point, a)));
+    float area = dot(normal, cross(ba, sub(pts[i].position, a)));
     float score = absf(area);
     if (bias * score >= best_score) { best_score = score; best = i; best_area = area; }
   }
   if (best == kNullIndex) return;
-  m.pts[2] = pts[best];
+  m.pts[2] = local_point(pts[best]);
   m.point_count = 3;
   pts[best] = pts[count - 1];
   count -= 1;
@@ -304,7 +309,7 @@ __device__ void reduce_points(LocalManifold& m, LocalPoint* pts, int count) {
   best = kNullIndex;
   float sign = best_area < 0.0f ? -1.0f : 1.0f;
   for (int i = 0; i < count; ++i) {
-    Vec3 p = pts[i].point;
+    Vec3 p = pts[i].position;
     float u1 = sign * dot(normal, cross(sub(p, a), ba));
     float u2 = sign * dot(normal, cross(sub(p, b), sub(c, b)));
     float u3 = sign * dot(normal, cross(sub(p, c), sub(a, c)));
@@ -312,8 +317,8 @@ __device__ void reduce_points(LocalManifold& m, LocalPoint* pts, int count) {
     if (bias * score > best_score) { best_score = score; best = i; }
   }
   if (best != kNullIndex) {
-    m.pts[m.point_count] = pts[best];
-    m.point_count += 1;
+    m.pts[3] = local_point(pts[best]);
+    m.point_count = 4;
   }
 }
 
@@ -409,12 +414,11 @@ __device__ bool face_a_contact(LocalManifold& m, const BoxHull& ha, const BoxHul
     if (e == c_box_face_edge[ref_face]) break;
   }
   m.normal = ref_plane.n;
-  LocalPoint pts[kClipSlots];
-  const ClipVertex* src = use1 ? poly1 : poly2;
+  ClipVertex* pts = use1 ? poly1 : poly2;  // becomes the candidate points in place
   float min_sep = kFltMax;
   for (int i = 0; i < count; ++i) {
-    pts[i] = LocalPoint{mul_sub(src[i].position, 0.5f * src[i].separation, ref_plane.n), src[i].separation, src[i].pair};
-    min_sep = minf(min_sep, src[i].separation);
+    pts[i].position = mul_sub(pts[i].position, 0.5f * pts[i].separation, ref_plane.n);
+    min_sep = minf(min_sep, pts[i].separation);
   }
   if (min_sep >= kSpeculativeDistance) { cache = SatCache{}; return false; }
   reduce_points(m, pts, count);
@@ -425,20 +429,26 @@ __device__ bool face_a_contact(LocalManifold& m, const BoxHull& ha, const BoxHul
   return true;
 }
 
-// convex_manifold.rs:1276-1336
-__device__ bool face_b_contact(LocalManifold& m, const BoxHull& ha, const BoxHull& hb, Xf b_to_a, int vertex_a, int ref_face_b,
-                               SatCache& cache) {
-  Xf a_to_b = invert_xf(b_to_a);
-  if (!face_a_contact(m, hb, ha, a_to_b, ref_face_b, vertex_a, cache)) { cache = SatCache{}; return false; }
+// The reference face on A (convex_manifold.rs:1150-1271) or, with on_b, on B
+// (convex_manifold.rs:1276-1336: A and B swapped, the result mapped back into A's
+// frame). ref_face belongs to the reference hull, vertex to the other one.
+__device__ __forceinline__ bool face_contact(LocalManifold& m, const BoxHull& ha, const BoxHull& hb, Xf b_to_a, bool on_b,
+                                             int ref_face, int vertex, SatCache& cache) {
+  const BoxHull& ref = on_b ? hb : ha;
+  const BoxHull& inc = on_b ? ha : hb;
+  if (!face_a_contact(m, ref, inc, on_b ? invert_xf(b_to_a) : b_to_a, ref_face, vertex, cache)) return false;
+  if (!on_b) return true;
   Mat3 mat = matrix_from_quat(b_to_a.q);
   m.normal = neg(mul_mv(mat, m.normal));
-  for (int i = 0; i < m.point_count; ++i) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= m.point_count) break;
     m.pts[i].point = add(mul_mv(mat, m.pts[i].point), b_to_a.p);
     m.pts[i].pair = flip_pair(m.pts[i].pair);
   }
   cache.type = kAxisFaceB;
-  cache.index_a = (uint8_t)vertex_a;
-  cache.index_b = (uint8_t)ref_face_b;
+  cache.index_a = (uint8_t)vertex;
+  cache.index_b = (uint8_t)ref_face;
   return true;
 }
 
@@ -471,27 +481,33 @@ __device__ __forceinline__ int support_vertex(const BoxHull& h, Vec3 d) {
   return best;
 }
 
-// convex_manifold.rs:1445-1845 (collide_hulls_with_scratch)
+// convex_manifold.rs:1445-1845 (collide_hulls_with_scratch). The face clip runs at
+// one site: first (when the cache names a face axis) to test the cached axis, then,
+// if that axis no longer holds, for the full query's best face. One site keeps a
+// single copy of the clip buffers in the compiled kernel.
 __device__ void collide_boxes(LocalManifold& m, const BoxHull& ha, const BoxHull& hb, Xf b_to_a, SatCache& cache) {
   m.point_count = 0;
   const float spec = kSpeculativeDistance;
   cache.hit = 0;
+  bool cached_face = false, on_b = false;
+  int ref_face = 0, vertex = 0;
   if (cache.type == kAxisFaceA) {
     Plane pl = ha.planes[cache.index_a];
     int vi = support_vertex(hb, neg(inv_rotate(b_to_a.q, pl.n)));
     float sep = plane_sep(pl, xf_point(b_to_a, hb.points[vi]));
     if (sep >= spec) { cache.hit = 1; return; }
-    SatCache local{};
-    bool touching = face_a_contact(m, ha, hb, b_to_a, cache.index_a, vi, local);
-    if (touching && absf(cache.separation - local.separation) < kLinearSlop) { cache.hit = 1; return; }
+    cached_face = true;
+    ref_face = cache.index_a;
+    vertex = vi;
   } else if (cache.type == kAxisFaceB) {
     Plane pl = hb.planes[cache.index_b];
     int vi = support_vertex(ha, neg(rotate(b_to_a.q, pl.n)));
     float sep = plane_sep(pl, inv_xf_point(b_to_a, ha.points[vi]));
     if (sep >= spec) { cache.hit = 1; return; }
-    SatCache local{};
-    bool touching = face_b_contact(m, ha, hb, b_to_a, vi, cache.index_b, local);
-    if (touching && absf(cache.separation - local.separation) < kLinearSlop) { cache.hit = 1; return; }
+    cached_face = true;
+    on_b = true;
+    ref_face = cache.index_b;
+    vertex = vi;
   } else if (cache.type == kAxisEdge) {
     int ia = cache.index_a, ib = cache.index_b;
     Vec3 p_a = ha.points[E_ORIGIN(ia)], q_a = ha.points[E_ORIGIN(ia + 1)];
@@ -516,19 +532,36 @@ __device__ void collide_boxes(LocalManifold& m, const BoxHull& ha, const BoxHull
     }
   }
 
-  m.point_count = 0;
-  cache = SatCache{};
-  AxisQuery q = separating_axis(ha, hb, b_to_a);
-  if (q.separated != kAxisInvalid) {
-    const SepAxis& s = q.separated == kAxisFaceA ? q.face_a : (q.separated == kAxisFaceB ? q.face_b : q.edge);
-    cache.type = q.separated;
-    cache.separation = s.separation;
-    cache.index_a = (uint8_t)s.index_a;
-    cache.index_b = (uint8_t)s.index_b;
-    return;
+  AxisQuery q;
+  for (;;) {
+    if (!cached_face) {
+      m.point_count = 0;
+      cache = SatCache{};
+      q = separating_axis(ha, hb, b_to_a);
+      if (q.separated != kAxisInvalid) {
+        SepAxis s = q.edge;
+        if (q.separated == kAxisFaceA) s = q.face_a;
+        else if (q.separated == kAxisFaceB) s = q.face_b;
+        cache.type = q.separated;
+        cache.separation = s.separation;
+        cache.index_a = (uint8_t)s.index_a;
+        cache.index_b = (uint8_t)s.index_b;
+        return;
+      }
+      on_b = !(q.face_a.separation > q.face_b.separation);
+      ref_face = on_b ? q.face_b.index_b : q.face_a.index_a;
+      vertex = on_b ? q.face_b.index_a : q.face_a.index_b;
+    }
+    SatCache local{};
+    bool touching = face_contact(m, ha, hb, b_to_a, on_b, ref_face, vertex, local);
+    if (cached_face) {
+      if (touching && absf(cache.separation - local.separation) < kLinearSlop) { cache.hit = 1; return; }
+      cached_face = false;  // the cached axis no longer holds: run the full query
+      continue;
+    }
+    cache = local;
+    break;
   }
-  if (q.face_a.separation > q.face_b.separation) face_a_contact(m, ha, hb, b_to_a, q.face_a.index_a, q.face_a.index_b, cache);
-  else face_b_contact(m, ha, hb, b_to_a, q.face_b.index_a, q.face_b.index_b, cache);
   if (q.edge.index_a == kNullIndex) return;
   float face_sep = maxf(q.face_a.separation, q.face_b.separation);
   if ((m.point_count == 0 && q.edge.separation > face_sep) || q.edge.separation > cache.separation + kLinearSlop) {

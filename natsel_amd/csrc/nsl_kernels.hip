@@ -76,7 +76,7 @@ __global__ void k_pairset_erase(PairSet set, const uint64_t* keys, int n) {
 // Every contact id below n whose awake-set flag is on: the awake set's contacts, touching or
 // not (collide_pass.rs:175-190; each is independent, so the order of the reference's list
 // does not matter). A disjoint contact turns its own flag off: the host destroys it.
-__global__ void k_collide(DeviceWorld d, int n) {
+__global__ __launch_bounds__(kBlock) void k_collide(DeviceWorld d, int n) {
   const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= n || !d.contact_active[id]) return;
   const uint8_t change = collide_contact(d, id);

@@ -104,13 +104,19 @@ __device__ bool compute_manifold(const DeviceWorld& d, Contact& c, const Shape& 
     c.manifold_count = 1;
   } else {
     old_count = c.point_count;
-    for (int i = 0; i < old_count; ++i) old_points[i] = c.points[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < old_count) old_points[i] = c.points[i];
   }
   c.point_count = lm.point_count;
   Mat3 ma = matrix_from_quat(xf_a.q);
   c.normal = mul_mv(ma, lm.normal);
   Vec3 dpos = sub_pos(xf_a.p, xf_b.p);
-  for (int i = 0; i < lm.point_count; ++i) {
+  // Point loops run over all four slots with a guard so every index is a constant and
+  // the contact, the local manifold and the old points stay in registers.
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= lm.point_count) break;
     ManifoldPoint t;
     t.anchor_a = mul_mv(ma, lm.pts[i].point);
     t.anchor_b = add(t.anchor_a, dpos);
@@ -121,7 +127,9 @@ __device__ bool compute_manifold(const DeviceWorld& d, Contact& c, const Shape& 
     t.base_separation = 0.0f;
     bool persisted = false;
     t.normal_impulse = 0.0f;
-    for (int j = 0; j < old_count; ++j) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= old_count) break;
       if (t.feature_id == old_points[j].feature_id) {
         t.normal_impulse = old_points[j].normal_impulse;
         persisted = true;
@@ -167,7 +175,9 @@ __device__ uint8_t collide_contact(const DeviceWorld& d, int32_t id) {
         Mat3 m_b = matrix_from_quat(mul_quat(xf_b.q, conjugate(c.cached_rot_b)));
         Vec3 dc = sub_pos(bb.center, ba.center);
         if (c.manifold_count > 0) {
-          for (int i = 0; i < c.point_count; ++i) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (i >= c.point_count) break;
             Vec3 r_a = mul_mv(m_a, c.points[i].anchor_a);
             Vec3 r_b = mul_mv(m_b, c.points[i].anchor_b);
             Vec3 dp = add(dc, sub(r_b, r_a));
@@ -203,7 +213,9 @@ __device__ uint8_t collide_contact(const DeviceWorld& d, int32_t id) {
     if (sa.enable_hit_events || sb.enable_hit_events) c.flags |= cflag::kHitEvent;
     else c.flags &= ~cflag::kHitEvent;
     Vec3 ca = rotate(xf_a.q, ba.local_center), cb = rotate(xf_b.q, bb.local_center);
-    for (int j = 0; j < c.point_count; ++j) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= c.point_count) break;
       c.points[j].anchor_a = sub(c.points[j].anchor_a, ca);
       c.points[j].anchor_b = sub(c.points[j].anchor_b, cb);
       c.points[j].base_separation = c.points[j].separation;
