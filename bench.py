@@ -212,6 +212,10 @@ def batched_measurement(worlds: int, steps: int) -> dict:
     }
 
 
+# World options every measured world shares (set from the command line).
+WORLD_OPTS: dict = {}
+
+
 def large_world_measurement(steps: int) -> dict:
     """The large single world: 100,000 mixed bodies in a seeded pile plus 2,000 chains of 16
     capsule links (30,000 revolute joints), steps 1..`steps` from the scene's start. One run
@@ -226,7 +230,7 @@ def large_world_measurement(steps: int) -> dict:
     n = sc.dynamic_count
     runs = []
     for timing in (False, True):
-        world = load_scene(sc)
+        world = load_scene(sc, **WORLD_OPTS)
         world.step(sc.dt, sc.substeps)  # uploads the world
         world.reset_kernel_stats()
         world.set_kernel_timing(timing)
@@ -286,7 +290,7 @@ def defaults_measurement(steps: int, cpu_seconds: float) -> dict:
     from oracle import pyoracle
 
     sc = scenes.pile_defaults(10_000)
-    world = load_scene(sc)
+    world = load_scene(sc, **WORLD_OPTS)
     world.step(sc.dt, sc.substeps)
     world.reset_kernel_stats()
     torch.cuda.synchronize()
@@ -446,6 +450,9 @@ def main() -> int:
     ap.add_argument("--blocks", type=int, default=0, help="workgroups of the step kernel (0: sized from the largest phase)")
     ap.add_argument("--small-colours", type=int, default=128,
                     help="colours of at most this many items run on one workgroup (0: a grid barrier after every colour)")
+    ap.add_argument("--plain-launch", action="store_true",
+                    help="launch the step kernel without the cooperative API (same grid, same results): rocprofv3 "
+                         "ends before writing its output for a process that made cooperative launches")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape of the step kernel instead of autotuning it after the settle")
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched measurement (0: skip it)")
@@ -464,6 +471,8 @@ def main() -> int:
     ap.add_argument("--window", default="", help="a:b, 1-based dispatches of --kernel (default: the timed window "
                                                  "of a run with these --steps/--warmup)")
     args = ap.parse_args()
+    if args.plain_launch:
+        WORLD_OPTS["plain_step_launch"] = True
     if args.summarize_profile:
         settle = settle_steps(args.steps, args.warmup)
         window = args.window or f"{settle + 1}:{settle + args.steps}"
@@ -491,7 +500,7 @@ def main() -> int:
 
     scene = scenes.pile(args.bodies)
     settle = settle_steps(args.steps, args.warmup)
-    world = load_scene(scene, device=0, solver_blocks=args.blocks, small_colour_items=args.small_colours)
+    world = load_scene(scene, device=0, solver_blocks=args.blocks, small_colour_items=args.small_colours, **WORLD_OPTS)
     # untimed: the pile's first `settle` steps (the drop and the settling), which include the
     # W warmup steps; the timed window is the run's last K steps
     torch.cuda.synchronize()
@@ -555,6 +564,7 @@ def main() -> int:
         "grid_barriers_per_launch": ks["barriers"] / launches,
         "small_colour_items": world_opts["small_colour_items"],
         "stage_records": world_opts["stage_records"],
+        "step_launch": "plain" if args.plain_launch else "cooperative",
         "workgroups": ks["solver_blocks"],
         "autotune": tuning,
     }

@@ -81,6 +81,10 @@ typedef struct nsl_options {
                                         run (0: every colour behind a grid barrier); results do not change */
   int32_t stage_records;             /* 0: 1 = the step kernel copies each lane's constraint record of the
                                         next colour to LDS while the current colour runs; results do not change */
+  int32_t plain_step_launch;         /* 0: the step kernel starts through hipLaunchCooperativeKernel (the runtime
+                                        guarantees that every workgroup is resident); 1 = an ordinary launch of
+                                        the same grid, sized to the occupancy limit, for tools that cannot
+                                        follow cooperative launches; results do not change */
 } nsl_options;
 
 typedef struct nsl_body_def {

@@ -29,6 +29,7 @@ class Options(ctypes.Structure):
         ("enable_restitution_propagation", ctypes.c_int32),
         ("small_colour_items", ctypes.c_int32),
         ("stage_records", ctypes.c_int32),
+        ("plain_step_launch", ctypes.c_int32),
     ]
 
 

@@ -60,6 +60,7 @@ void apply(nsl::WorldOptions& o, const nsl_options& in) {
   o.enable_restitution_propagation = in.enable_restitution_propagation != 0;
   o.small_colour_items = in.small_colour_items;
   o.stage_records = in.stage_records != 0 ? 1 : 0;
+  o.plain_step_launch = in.plain_step_launch != 0 ? 1 : 0;
 }
 
 nsl::Material material(const nsl_material* m) {
@@ -139,6 +140,7 @@ void nsl_default_options(nsl_options* out) {
   out->enable_restitution_propagation = o.enable_restitution_propagation;
   out->small_colour_items = o.small_colour_items;
   out->stage_records = o.stage_records;
+  out->plain_step_launch = o.plain_step_launch;
 }
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out) {

@@ -157,7 +157,7 @@ void launch_pairset_erase(PairSet set, const uint64_t* keys, int n, hipStream_t 
 // Prepare of joints, contact and overflow constraints, one lane per item (before the step kernel).
 void launch_prepare(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, hipStream_t s, KernelTimer* timer);
 // One launch (all workgroups resident): the substep schedule and restitution.
-void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
+void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, bool cooperative, hipStream_t s,
                        KernelTimer* timer);
 // Store of contact and overflow impulses and finalize of the bodies, one lane per item (after
 // the step kernel; finalize lists the fast bodies for the continuous pass).

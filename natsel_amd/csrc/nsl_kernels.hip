@@ -618,16 +618,21 @@ void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const St
   NSL_TIMED(t, kKFinalize, s, hipLaunchKernelGGL(k_store_finalize, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, plan, p));
 }
 
-void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, hipStream_t s,
-                       KernelTimer* t) {
+void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, bool cooperative,
+                       hipStream_t s, KernelTimer* t) {
   // A cooperative launch: the runtime starts the grid only when every workgroup can be
   // resident at once, which the grid barriers need (`blocks` is clamped by the caller to the
   // occupancy limit); the barriers' bounded wait still turns a stalled grid into an error.
+  // The plain launch of the same grid relies on that clamp alone.
   void* args[] = {const_cast<DeviceWorld*>(&d), const_cast<SolvePlan*>(&plan), const_cast<StepParams*>(&p)};
   hipError_t launch = hipSuccess;
-  NSL_TIMED(t, kKSolve, s,
-            launch = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_solve_step), dim3(blocks), dim3(kSolveBlock),
-                                                args, 0, s));
+  if (cooperative) {
+    NSL_TIMED(t, kKSolve, s,
+              launch = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_solve_step), dim3(blocks),
+                                                  dim3(kSolveBlock), args, 0, s));
+  } else {
+    NSL_TIMED(t, kKSolve, s, hipLaunchKernelGGL(k_solve_step, dim3(blocks), dim3(kSolveBlock), 0, s, d, plan, p));
+  }
   if (launch != hipSuccess) throw std::runtime_error(std::string("step kernel launch: ") + hipGetErrorString(launch));
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) throw std::runtime_error(std::string("step kernel launch: ") + hipGetErrorString(err));

@@ -61,6 +61,8 @@ struct WorldOptions {
   // 1: the step kernel copies each lane's constraint record of the next colour to LDS while
   // the current colour runs (results do not change)
   int32_t stage_records = 0;
+  // 1: the step kernel is launched without the cooperative API (same grid and results)
+  int32_t plain_step_launch = 0;
 };
 
 

@@ -1040,7 +1040,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   DeviceWorld dw = d_;
   if (!options.enable_sleep) dw.sleep_times = nullptr;
   launch_prepare(dw, plan, p, stream_, tm);
-  launch_solve_step(dw, plan, p, blocks, stream_, tm);
+  launch_solve_step(dw, plan, p, blocks, options.plain_step_launch == 0, stream_, tm);
   launch_store_finalize(dw, plan, p, stream_, tm);
   // continuous collision of the bodies finalize flagged fast, then of the bullets
   if (options.enable_continuous) {

@@ -96,6 +96,7 @@ class WorldOptions:  # world_runtime.rs:1243-1304
     enable_restitution_propagation: bool = False
     small_colour_items: int = 128  # colours this small run on one workgroup (0: off); results do not change
     stage_records: bool = False  # the step kernel copies next-colour constraint records to LDS; results do not change
+    plain_step_launch: bool = False  # an ordinary (not cooperative) launch of the same step-kernel grid
 
 
 def _material(d: ShapeDef) -> capi.Material:
@@ -136,6 +137,7 @@ def c_options(o: WorldOptions) -> capi.Options:
     c.enable_restitution_propagation = int(o.enable_restitution_propagation)
     c.small_colour_items = int(o.small_colour_items)
     c.stage_records = int(o.stage_records)
+    c.plain_step_launch = int(o.plain_step_launch)
     return c
 
 
@@ -366,10 +368,11 @@ def shape_def_of(s) -> ShapeDef:
                     s.is_sensor, s.enable_speculative_contact, s.enable_hit_events)
 
 
-def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0, small_colour_items: int = 128) -> World:
+def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0, small_colour_items: int = 128,
+               plain_step_launch: bool = False) -> World:
     """Build a World from a seeded scene description (natsel_amd.scenes)."""
     opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous, solver_blocks=solver_blocks,
-                        small_colour_items=small_colour_items,
+                        small_colour_items=small_colour_items, plain_step_launch=plain_step_launch,
                         gravity=tuple(scene.gravity), restitution_iterations=scene.restitution_iterations,
                         restitution_threshold=scene.restitution_threshold,
                         enable_restitution_propagation=scene.restitution_propagation)

@@ -26,3 +26,19 @@ def test_autotune_keeps_state_and_results(gpu_available):
         g.step(sc.dt, sc.substeps)
         o.step(sc.dt, sc.substeps)
         assert g.world_state_hash() == o.world_state_hash(), f"step {k} after tuning"
+
+
+def test_plain_step_launch_matches_oracle(gpu_available):
+    """The ordinary launch of the step kernel (plain_step_launch) runs the same grid as the
+    cooperative one and ends every step on the oracle's hash."""
+    from natsel_amd.world import load_scene
+    from oracle import pyoracle
+
+    sc = scenes.pile(1500, seed=12, size_x=10, size_y=20, size_z=10)
+    g, o = load_scene(sc, plain_step_launch=True), pyoracle.load_scene(sc)
+    assert g.options.plain_step_launch
+    for k in range(120):
+        g.step(sc.dt, sc.substeps)
+        o.step(sc.dt, sc.substeps)
+        if k % 20 == 19:
+            assert g.world_state_hash() == o.world_state_hash(), f"step {k}"
