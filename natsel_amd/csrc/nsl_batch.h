@@ -69,6 +69,8 @@ struct ContactMeta {  // per contact id: shapes, graph colour and index, livenes
   int32_t alive;
   float restitution;
   int32_t pair_type;  // the collide routine: see kPairTypes
+  int32_t body_a, body_b;  // the shapes' bodies (world body ids)
+  int32_t dynamic;         // bit 0: body_a is dynamic, bit 1: body_b is dynamic
 };
 
 // Collide routines by primary shape (hull > capsule > sphere, contact_creation.rs:88-102):

@@ -148,6 +148,9 @@ __device__ void create_contact_at(const WorldRefs& r, int32_t a, int32_t b, int3
   m.restitution = maxf(sa.restitution, sb.restitution);
   m.pair_type = sa.kind == kShapeHull ? (sb.kind == kShapeHull ? 0 : (sb.kind == kShapeSphere ? 1 : 2))
                                       : (sa.kind == kShapeCapsule ? (sb.kind == kShapeSphere ? 3 : 4) : 5);
+  m.body_a = pa.body;
+  m.body_b = pb.body;
+  m.dynamic = (ba.type == kBodyDynamic ? 1 : 0) | (bb.type == kBodyDynamic ? 2 : 0);
   r.meta[id] = m;
   r.non_touching[local] = id;
   Contact c{};
@@ -193,10 +196,10 @@ __device__ void remove_from_graph(const WorldRefs& r, int32_t body_a, int32_t bo
 // constraint_graph.rs:113-197
 __device__ void add_to_graph(const WorldRefs& r, int32_t id) {
   ContactMeta& c = r.meta[id];
-  const int32_t a = r.proxy[c.shape_a].body, b = r.proxy[c.shape_b].body;
-  const int32_t ta = r.body[a].type, tb = r.body[b].type;
+  const int32_t a = c.body_a, b = c.body_b;
+  const bool dyn_a = (c.dynamic & 1) != 0, dyn_b = (c.dynamic & 2) != 0;
   int32_t color = kOverflowColor;
-  if (ta == kBodyDynamic && tb == kBodyDynamic) {
+  if (dyn_a && dyn_b) {
     for (int i = 0; i < kDynamicColorCount; ++i) {
       if (colour_has(r, i, a) || colour_has(r, i, b)) continue;
       colour_set(r, i, a, true);
@@ -204,8 +207,8 @@ __device__ void add_to_graph(const WorldRefs& r, int32_t id) {
       color = i;
       break;
     }
-  } else if (ta == kBodyDynamic || tb == kBodyDynamic) {
-    const int32_t dyn = ta == kBodyDynamic ? a : b;
+  } else if (dyn_a || dyn_b) {
+    const int32_t dyn = dyn_a ? a : b;
     for (int i = kOverflowColor - 1; i >= 1; --i) {
       if (!colour_has(r, i, dyn)) {
         colour_set(r, i, dyn, true);
@@ -237,7 +240,7 @@ __device__ void destroy_contact(const WorldRefs& r, int32_t id) {
     const int32_t moved = swap_remove(r.non_touching, &ws->n_non_touching, c.local);
     if (moved != kNullIndex) r.meta[moved].local = c.local;
   } else {
-    remove_from_graph(r, r.proxy[c.shape_a].body, r.proxy[c.shape_b].body, c.color, c.local);
+    remove_from_graph(r, c.body_a, c.body_b, c.color, c.local);
   }
   c.alive = 0;
   c.color = kNullIndex;
@@ -261,7 +264,7 @@ __device__ void apply_change(const WorldRefs& r, int32_t id, uint8_t ch) {
     c.color = kNullIndex;
     c.local = r.ws->n_non_touching;
     r.non_touching[r.ws->n_non_touching++] = id;
-    remove_from_graph(r, r.proxy[c.shape_a].body, r.proxy[c.shape_b].body, color, local);
+    remove_from_graph(r, c.body_a, c.body_b, color, local);
   }
 }
 
@@ -311,6 +314,39 @@ struct Colours {
   int begin[kOverflowColor];
   int end[kOverflowColor];
 };
+
+// One solve stage over the colours in colour order, a barrier after each colour: each lane
+// runs item(slot, record) on its slots (slot = colour begin + lane + 64 k), and the record
+// of its slot in the next round (the colour's next 64 slots or the next colour's first) is
+// loaded before the current item runs, so that the load overlaps the item's work. A stage
+// writes each record only from its own item, so a record loaded one round early holds the
+// values the item would read.
+template <class Item>
+__device__ __forceinline__ void colour_stage(const DeviceWorld& d, const Colours& cs, int lane, Item item) {
+  if (cs.n == 0) return;
+  int a = 0, base = cs.begin[0];
+  bool live = base + lane < cs.end[0];
+  Constraint cur;
+  if (live) cur = d.constraints[base + lane];
+  for (;;) {
+    int na = a, nbase = base + kBatchLanes;
+    if (nbase >= cs.end[a]) {
+      na = a + 1;
+      nbase = na < cs.n ? cs.begin[na] : 0;
+    }
+    const bool more = na < cs.n;
+    const bool next_live = more && nbase + lane < cs.end[na];
+    Constraint next;
+    if (next_live) next = d.constraints[nbase + lane];
+    if (live) item(base + lane, cur);
+    if (na != a) wave_sync();
+    if (!more) break;
+    a = na;
+    base = nbase;
+    live = next_live;
+    cur = next;
+  }
+}
 
 // Pair update of one world (contact_creation.rs:901-1042) by a workgroup of four
 // wavefronts: every moved proxy against every shape (each wavefront takes every fourth
@@ -550,9 +586,12 @@ __global__ __launch_bounds__(kBatchLanes) void k_tape_final(BatchDevice b, int s
   }
 }
 
-__global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, StepParams p) {
+__global__ __launch_bounds__(kBatchLanes) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_solve(BatchDevice b, StepParams p) {
+  // s_states holds the world's colour bits during the transitions, then its body states
   __shared__ BodyState s_states[kBatchMaxAwake];
+  __shared__ BatchWorldState s_ws;
   __shared__ Colours colours;
+  __shared__ int s_colour_list[kOverflowColor];  // graph colour of each non-empty colour range
   __shared__ int s_count, s_any;
   const int lane = threadIdx.x;
   if (blockIdx.x == 0 && lane < kPairTypes) b.collide_counts[lane] = 0;  // the next pair launch lists anew
@@ -561,10 +600,23 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
   const WorldRefs r = world_refs(b, wd, world);
   const DeviceWorld dg = world_view(b, wd, b.states + wd.awake_off);
   const DeviceWorld dl = world_view(b, wd, s_states);
-  BatchWorldState* ws = r.ws;
-  if (ws->error != kBatchOk) return;
-  const int next_id = ws->next_id;
-  // ---- transitions in contact-id order (one lane walks the changed ids)
+  if (r.ws->error != kBatchOk) return;
+  // ---- transitions in contact-id order (one lane walks the changed ids) on the world's
+  // pool counters and colour bits copied to LDS, written back afterwards
+  constexpr int kWsWords = (int)(sizeof(BatchWorldState) / sizeof(int32_t));
+  static_assert(sizeof(BatchWorldState) % sizeof(int32_t) == 0, "world state moves as words");
+  for (int i = lane; i < kWsWords; i += kBatchLanes)
+    reinterpret_cast<int32_t*>(&s_ws)[i] = reinterpret_cast<const int32_t*>(r.ws)[i];
+  const int bit_words = kOverflowColor * r.bits_words;
+  const bool bits_in_lds = bit_words <= (int)(sizeof(s_states) / sizeof(uint32_t));
+  uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_states);
+  if (bits_in_lds)
+    for (int i = lane; i < bit_words; i += kBatchLanes) s_bits[i] = r.bits[i];
+  wave_sync();
+  WorldRefs rl = r;
+  rl.ws = &s_ws;
+  if (bits_in_lds) rl.bits = s_bits;
+  const int next_id = s_ws.next_id;
   for (int base = 0; base < next_id; base += kBatchLanes) {
     const int id = base + lane;
     uint64_t changed = wave_ballot(id < next_id && r.change[id] != kChangeNone);
@@ -572,22 +624,29 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
       while (changed) {
         const int k = base + (__ffsll((unsigned long long)changed) - 1);
         changed &= changed - 1;
-        apply_change(r, k, r.change[k]);
+        apply_change(rl, k, r.change[k]);
       }
     wave_sync();
   }
-  if (ws->error != kBatchOk) return;
-  if (lane == 0) ws->steps += 1;
+  if (lane == 0 && s_ws.error == kBatchOk) s_ws.steps += 1;
+  wave_sync();
+  for (int i = lane; i < kWsWords; i += kBatchLanes)
+    reinterpret_cast<int32_t*>(r.ws)[i] = reinterpret_cast<const int32_t*>(&s_ws)[i];
+  if (bits_in_lds)
+    for (int i = lane; i < bit_words; i += kBatchLanes) r.bits[i] = s_bits[i];
+  wave_sync();
+  if (s_ws.error != kBatchOk) return;
   if (wd.n_awake == 0) return;
 
   // ---- solve order: colours 0..22 in graph order, then the overflow list
   if (lane == 0) {
     int n = 0, slots = 0;
     for (int ci = 0; ci < kOverflowColor; ++ci) {
-      const int cnt = ws->color_count[ci];
+      const int cnt = s_ws.color_count[ci];
       if (cnt > 0) {
         colours.begin[n] = slots;
         colours.end[n] = slots + cnt;
+        s_colour_list[n] = ci;
         ++n;
       }
       slots += cnt;
@@ -596,37 +655,32 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
     s_count = slots;
     s_any = 0;
   }
+  for (int i = lane; i < wd.n_awake; i += kBatchLanes) s_states[i] = dg.states[i];
   wave_sync();
   const int n_constraints = s_count;
-  const int n_ovf = ws->n_overflow;
+  const int n_ovf = s_ws.n_overflow;
+
+  // ---- prepare (solver_stages.rs:543-589): each slot's contact id taken from its colour
+  // list and the constraint prepared by the same lane
   {
-    int slot = 0;
+    int points = 0;
     bool any_rest = false;
-    for (int ci = 0; ci < kOverflowColor; ++ci) {
-      const int cnt = ws->color_count[ci];
-      for (int k = lane; k < cnt; k += kBatchLanes) {
-        const int32_t id = r.colors[ci * r.color_cap + k];
-        dl.solve_ids[slot + k] = id;
-        any_rest = any_rest || r.meta[id].restitution > 0.0f;
-      }
-      slot += cnt;
+    int a = 0;
+    for (int slot = lane; slot < n_constraints; slot += kBatchLanes) {
+      while (slot >= colours.end[a]) ++a;
+      const int32_t id = r.colors[s_colour_list[a] * r.color_cap + (slot - colours.begin[a])];
+      dl.solve_ids[slot] = id;
+      float restitution = 0.0f;
+      points += prepare_contact(dl, slot, id, p, &restitution);
+      any_rest = any_rest || restitution > 0.0f;
     }
     for (int k = lane; k < n_ovf; k += kBatchLanes) {
       const int32_t id = r.overflow_list[k];
       dl.overflow_ids[k] = id;
       any_rest = any_rest || r.meta[id].restitution > 0.0f;
+      prepare_overflow_item(dl, k, p);
     }
     if (wave_ballot(any_rest) != 0 && lane == 0) s_any = 1;
-  }
-  for (int i = lane; i < wd.n_awake; i += kBatchLanes) s_states[i] = dg.states[i];
-  wave_sync();
-  const bool any_restitution = s_any != 0;
-
-  // ---- prepare, substeps, restitution, store, finalize (solver_stages.rs:543-589)
-  {
-    int points = 0;
-    for (int slot = lane; slot < n_constraints; slot += kBatchLanes) points += prepare_item(dl, slot, p);
-    for (int k = lane; k < n_ovf; k += kBatchLanes) prepare_overflow_item(dl, k, p);
     for (int off = 32; off > 0; off >>= 1) points += __shfl_xor(points, off, 64);
     if (lane == 0) {
       atomicAdd(&b.work[1], (unsigned long long)n_constraints);
@@ -635,6 +689,7 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
     }
   }
   wave_sync();
+  const bool any_restitution = s_any != 0;
   if (b.tape_step >= 0) {
     tape_record(b, world, wd, dl, r, n_constraints, n_ovf, any_restitution, colours, s_states);
     wave_sync();
@@ -646,31 +701,19 @@ __global__ __launch_bounds__(kBatchLanes) void k_batch_solve(BatchDevice b, Step
       if (lane == 0) overflow_warm_all(dl, n_ovf);
       wave_sync();
     }
-    for (int a = 0; a < colours.n; ++a) {
-      for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
-        warm_start_item(dl, slot, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
-      wave_sync();
-    }
+    colour_stage(dl, colours, lane, [&](int, const Constraint& c) { warm_start_record(dl, c); });
     if (n_ovf > 0) {
       if (lane == 0) overflow_solve_all(dl, n_ovf, p, 0);
       wave_sync();
     }
-    for (int a = 0; a < colours.n; ++a) {
-      for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
-        push_item(dl, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
-      wave_sync();
-    }
+    colour_stage(dl, colours, lane, [&](int slot, const Constraint& c) { push_record(dl, slot, p, c); });
     for (int i = lane; i < wd.n_awake; i += kBatchLanes) integrate_positions_item(dl, i, p);
     wave_sync();
     if (n_ovf > 0) {
       if (lane == 0) overflow_solve_all(dl, n_ovf, p, 1);
       wave_sync();
     }
-    for (int a = 0; a < colours.n; ++a) {
-      for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
-        relax_item(dl, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
-      wave_sync();
-    }
+    colour_stage(dl, colours, lane, [&](int slot, const Constraint& c) { relax_record(dl, slot, p, c); });
   }
   if (any_restitution) {
     for (int it = 0; it < b.restitution_iterations; ++it) {

@@ -234,8 +234,8 @@ __device__ uint8_t collide_contact(const DeviceWorld& d, int32_t id) {
 // constraint preparation (contact_solver.rs:395-662)
 
 // Returns the number of manifold points prepared.
-__device__ __forceinline__ int prepare_item(const DeviceWorld& d, int slot, const StepParams& p) {
-  const int32_t id = d.solve_ids[slot];
+__device__ __forceinline__ int prepare_contact(const DeviceWorld& d, int slot, int32_t id, const StepParams& p,
+                                               float* restitution = nullptr) {
   const Contact& contact = d.contacts[id];
   const float inv_tau = 1.0f / kSpeculativeDistance;
   Constraint cc;
@@ -328,7 +328,12 @@ __device__ __forceinline__ int prepare_item(const DeviceWorld& d, int slot, cons
     }
   }
   d.constraints[slot] = cc;
+  if (restitution) *restitution = cc.restitution;
   return cc.point_count;
+}
+
+__device__ __forceinline__ int prepare_item(const DeviceWorld& d, int slot, const StepParams& p) {
+  return prepare_contact(d, slot, d.solve_ids[slot], p);
 }
 
 // overflow_contact_solver.rs:92-257
@@ -751,6 +756,31 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
   }
   store_state(d, ia, ba);
   store_state(d, ib, bb);
+}
+
+// The three solve stages on a record the caller already holds (a copy of slot's record,
+// loaded early); impulses are written to the slot's record.
+__device__ __forceinline__ void warm_start_record(const DeviceWorld& d, const Constraint& c) {
+  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
+  const WarmImpulse wi = warm_impulse(c);
+  ba.w = sub(ba.w, sym_mv(c.inv_ia, wi.ang_a));
+  ba.v = sub(ba.v, scale(c.inv_mass_a, wi.linear));
+  bb.w = add(bb.w, sym_mv(c.inv_ib, wi.ang_b));
+  bb.v = add(bb.v, scale(c.inv_mass_b, wi.linear));
+  store_state(d, c.index_a, ba);
+  store_state(d, c.index_b, bb);
+}
+__device__ __forceinline__ void push_record(const DeviceWorld& d, int slot, const StepParams& p, const Constraint& c) {
+  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
+  push_core(c, d.constraints[slot], p, ba, bb);
+  store_state(d, c.index_a, ba);
+  store_state(d, c.index_b, bb);
+}
+__device__ __forceinline__ void relax_record(const DeviceWorld& d, int slot, const StepParams& p, const Constraint& c) {
+  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
+  relax_core(c, d.constraints[slot], p, ba, bb);
+  store_state(d, c.index_a, ba);
+  store_state(d, c.index_b, bb);
 }
 
 // contact_solver.rs:1135-1229
