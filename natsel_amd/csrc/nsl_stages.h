@@ -269,7 +269,9 @@ __device__ __forceinline__ int prepare_contact(const DeviceWorld& d, int slot, i
   cc.friction_impulse = Vec2{warm * dot(contact.friction_impulse, cc.tangent1), warm * dot(contact.friction_impulse, cc.tangent2)};
   Vec3 center_a = zero3(), center_b = zero3();
   float total_weight = 0.0f;
-  for (int i = 0; i < cc.point_count; ++i) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= cc.point_count) break;
     const ManifoldPoint& mp = contact.points[i];
     ConstraintPoint cp;
     cp.r_a = mp.anchor_a;
@@ -298,7 +300,9 @@ __device__ __forceinline__ int prepare_contact(const DeviceWorld& d, int slot, i
     cc.center_a = zero3();
     cc.center_b = zero3();
   }
-  for (int i = 0; i < cc.point_count; ++i) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= cc.point_count) break;
     Vec3 dd = sub(cc.points[i].r_a, cc.center_a);
     cc.points[i].lever_arm = sqrtf_(dot(dd, dd));
   }
@@ -321,7 +325,9 @@ __device__ __forceinline__ int prepare_contact(const DeviceWorld& d, int slot, i
   cc.twist_mass = k_twist > 0.0f ? 1.0f / k_twist : 0.0f;
   if (contact.restitution > 0.0f || (contact.flags & cflag::kHitEvent) != 0) {
     BodyState sa = load_state(d, ia), sb = load_state(d, ib);
-    for (int i = 0; i < cc.point_count; ++i) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= cc.point_count) break;
       Vec3 vr_a = add(sa.v, cross(sa.w, cc.points[i].r_a));
       Vec3 vr_b = add(sb.v, cross(sb.w, cc.points[i].r_b));
       cc.points[i].relative_velocity = dot(normal, sub(vr_b, vr_a));
@@ -375,7 +381,9 @@ __device__ __forceinline__ int prepare_overflow_item(const DeviceWorld& d, int k
   c.tangent_velocity2 = dot(contact.tangent_velocity, t2);
   Vec3 center_a = zero3(), center_b = zero3();
   float total_w = 0.0f;
-  for (int i = 0; i < c.point_count; ++i) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= c.point_count) break;
     ManifoldPoint mp = contact.points[i];
     ConstraintPoint cp;
     cp.r_a = mp.anchor_a;
@@ -411,7 +419,9 @@ __device__ __forceinline__ int prepare_overflow_item(const DeviceWorld& d, int k
   }
   c.center_a = center_a;
   c.center_b = center_b;
-  for (int i = 0; i < c.point_count; ++i) c.points[i].lever_arm = distance(c.points[i].r_a, center_a);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < c.point_count) c.points[i].lever_arm = distance(c.points[i].r_a, center_a);
   Vec3 rt_a1 = cross(center_a, t1), rt_a2 = cross(center_a, t2);
   Vec3 rt_b1 = cross(center_b, t1), rt_b2 = cross(center_b, t2);
   Mat2 km;
