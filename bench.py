@@ -169,10 +169,12 @@ def cpu_baseline(scene, settle: int, k: int, seconds: float, gpu_hashes: dict) -
     }
 
 
-def batched_measurement(worlds: int, steps: int) -> dict:
+def batched_measurement(worlds: int, steps: int, autotune: bool = True) -> dict:
     """Batched worlds (the log-uniform 8..256-body piles and stacks of
     natsel_amd.scenes.batched): every world stepped together, `steps` steps in one call
-    after one untimed step; device time from HIP events around the call."""
+    after one untimed step; device time from HIP events around the call. With `autotune`,
+    the launch shape is first chosen by nsl_batch_autotune (60 steps per candidate from a
+    snapshot of the batch after the untimed step, which is then restored)."""
     from natsel_amd import scenes
     from natsel_amd.batch import load_batch
 
@@ -180,6 +182,7 @@ def batched_measurement(worlds: int, steps: int) -> dict:
     dt, sub = scene_list[0].dt, scene_list[0].substeps
     b = load_batch(scene_list)
     b.step(dt, sub, 1)  # uploads the worlds
+    tuning = b.autotune(dt, sub, 60) if autotune else None
     s0 = b.stats()
     b.step(dt, sub, steps)
     s1 = b.stats()
@@ -209,6 +212,7 @@ def batched_measurement(worlds: int, steps: int) -> dict:
         "constraints_per_step": counts["constraints"] / steps,
         "points_per_step": counts["points"] / steps,
         "error_worlds": s1["error_worlds"],
+        "autotune": tuning,
     }
 
 
@@ -612,7 +616,7 @@ def main() -> int:
         out["cpu_baseline"] = cb
         _progress(f"cpu baseline: {cb['value'] / 1e6:.2f} M body-substeps/s on {cb['cores']} threads")
     if args.batched_worlds > 0:
-        out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps)
+        out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps, args.autotune)
         _progress(f"batched: {out['batched']['ms_per_step']:.3f} ms per step")
     if args.adjoint_worlds > 0:
         out["adjoint"] = adjoint_measurement(args.adjoint_worlds, args.adjoint_steps)

@@ -35,6 +35,7 @@
  *                                  on the host, solver_stages.rs:551-586)
  *   nsl_batch_*                    many Worlds at once: create_body / create_*_shape / step /
  *                                  world_state_hash per world, as above (no batch type in the reference)
+ *   nsl_batch_*tuning, _autotune   the batched step's launch shape (no counterpart, as nsl_world_autotune)
  *
  * Conventions: every call returns 0 on success or a negative status; on failure
  * nsl_last_error() returns a message. Pointers are plain host memory owned by the
@@ -409,6 +410,27 @@ int nsl_batch_get_contact_shapes(nsl_batch* batch, int32_t world, int32_t* shape
  * pair update. */
 int nsl_batch_get_shape_bounds(nsl_batch* batch, int32_t world, float* fat, uint8_t* moved);
 int nsl_batch_get_stats(nsl_batch* batch, nsl_batch_stats* out);
+
+/* Launch shape of nsl_batch_step; results do not depend on it. */
+typedef struct nsl_batch_tuning {
+  int32_t collide_blocks;   /* workgroups of the collide launch (it strides over the live contacts);
+                               0 = as many as the device keeps resident at once */
+  int32_t use_graph;        /* 1: a call replays a captured HIP graph of 8 steps as often as it fits,
+                               the remaining steps launched directly */
+  int32_t candidates;       /* nsl_batch_autotune: shapes timed */
+  int32_t reserved;
+  double step_ms;           /* nsl_batch_autotune: the chosen shape's device time per step */
+  double previous_ms;       /* nsl_batch_autotune: the same for the shape in use before the call */
+} nsl_batch_tuning;
+int nsl_batch_set_tuning(nsl_batch* batch, const nsl_batch_tuning* tuning);
+int nsl_batch_get_tuning(nsl_batch* batch, nsl_batch_tuning* out);
+/* Picks the launch shape by timing: from a device snapshot of the whole batch, runs `steps`
+   steps of (dt, sub_steps) under each candidate (collide workgroups 1, 2, 4, 8 or 16 times the
+   resident count, graph replay off / on), checks that every candidate ends on the same world hashes
+   (NSL_ERR_INTERNAL otherwise), restores the snapshot and keeps the fastest shape. The
+   snapshot holds a second copy of the batch's device arrays during the call. No reference
+   counterpart (the reference has no batch type). */
+int nsl_batch_autotune(nsl_batch* batch, float dt, int32_t sub_steps, int32_t steps, nsl_batch_tuning* out);
 
 /* Differentiable rollout of a batch. The reference has no differentiable path; these
  * entry points are for gradient-based callers (trajectory optimisation, system

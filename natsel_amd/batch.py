@@ -172,6 +172,31 @@ class Batch:
         capi.check(self._lib.nsl_batch_rollout_times(self._h, ctypes.byref(r), ctypes.byref(a)))
         return r.value, a.value
 
+    @staticmethod
+    def _tuning_dict(t) -> dict:
+        return {"collide_blocks": t.collide_blocks, "use_graph": bool(t.use_graph), "candidates": t.candidates,
+                "step_ms": t.step_ms, "previous_ms": t.previous_ms}
+
+    def tuning(self) -> dict:
+        """The batched step's launch shape (results do not depend on it)."""
+        t = capi.BatchTuning()
+        capi.check(self._lib.nsl_batch_get_tuning(self._h, ctypes.byref(t)))
+        return self._tuning_dict(t)
+
+    def set_tuning(self, collide_blocks: int = 0, use_graph: bool = False) -> None:
+        """collide_blocks: workgroups of the collide launch (0: as many as are resident);
+        use_graph: replay a captured HIP graph of 8 steps."""
+        t = capi.BatchTuning(collide_blocks=int(collide_blocks), use_graph=int(bool(use_graph)))
+        capi.check(self._lib.nsl_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, time_step: float, sub_step_count: int, steps: int = 8) -> dict:
+        """Time each candidate launch shape over `steps` steps from a snapshot of the batch,
+        check that all reach the same world hashes, restore the snapshot and keep the
+        fastest shape (nsl_batch_autotune)."""
+        t = capi.BatchTuning()
+        capi.check(self._lib.nsl_batch_autotune(self._h, time_step, sub_step_count, steps, ctypes.byref(t)))
+        return self._tuning_dict(t)
+
     def stats(self) -> dict:
         s = capi.BatchStats()
         capi.check(self._lib.nsl_batch_get_stats(self._h, ctypes.byref(s)))

@@ -177,6 +177,11 @@ class Tuning(ctypes.Structure):
                 ("step_kernel_us", ctypes.c_double), ("previous_us", ctypes.c_double), ("candidates", ctypes.c_int32)]
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("collide_blocks", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("candidates", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("step_ms", ctypes.c_double), ("previous_ms", ctypes.c_double)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [
         ("last_step_ms", ctypes.c_double),
@@ -273,6 +278,9 @@ SIGNATURES = {
     "nsl_batch_get_contact_shapes": (ctypes.c_int, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "nsl_batch_get_shape_bounds": (ctypes.c_int, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(ctypes.c_uint8)]),
     "nsl_batch_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(BatchStats)]),
+    "nsl_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(BatchTuning)]),
+    "nsl_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(BatchTuning)]),
+    "nsl_batch_autotune": (ctypes.c_int, [_P, _F, _I, _I, ctypes.POINTER(BatchTuning)]),
 }
 
 

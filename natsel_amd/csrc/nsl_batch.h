@@ -147,6 +147,19 @@ struct BatchDevice {
 
 void launch_batch_step(const BatchDevice& b, const StepParams& p, int steps, hipStream_t s);
 
+// Launch shape of a batch step (results do not depend on it): the collide launch's grid
+// (1..16 times the resident workgroups; the launch strides over the live contacts),
+// and whether the steps of a call replay a captured HIP graph of kBatchGraphSteps steps.
+struct BatchTuning {
+  int32_t collide_blocks = 0;  // 0: as many as are resident at once
+  int32_t use_graph = 0;
+  // filled by autotune(): candidates timed, the best's and the starting shape's device
+  // time per step (ms)
+  int32_t candidates = 0;
+  double step_ms = 0.0, previous_ms = 0.0;
+};
+constexpr int kBatchGraphSteps = 8;
+
 class BatchStepper {
  public:
   explicit BatchStepper(int device);
@@ -163,6 +176,14 @@ class BatchStepper {
   int32_t body_count(int32_t w) const { return (int32_t)scenes_.at(w)->bodies_.size(); }
 
   void step(float dt, int32_t sub_steps, int32_t steps);
+
+  // Launch shape: set, read, or chosen by timing `steps` steps per candidate from a
+  // snapshot of every device array of the batch, which is restored afterwards (so the
+  // batch's state and results are as before the call; every candidate must reach the same
+  // world hashes).
+  void set_tuning(const BatchTuning& t);
+  BatchTuning tuning() const { return tuning_; }
+  BatchTuning autotune(float dt, int32_t sub_steps, int32_t steps);
 
   // World w's reference state hash (body_queries.rs:432), for every world.
   void state_hashes(uint64_t* out);
@@ -207,6 +228,10 @@ class BatchStepper {
   void upload();
   void pull();
   template <class T> T* alloc(size_t n, bool zero);
+  void launch_steps(const StepParams& p, int32_t steps);
+  void drop_graph();
+  std::vector<void*> snapshot();                   // device copies of every allocation
+  void restore(const std::vector<void*>& copies);  // and back (the copies are kept)
   StepParams step_params(float dt, int32_t sub_steps) const;
   void free_tape();
 
@@ -217,6 +242,12 @@ class BatchStepper {
   std::vector<BatchWorldDesc> desc_;
   BatchDevice d_{};
   std::vector<void*> allocations_;
+  std::vector<size_t> allocation_bytes_;
+  int32_t resident_collide_blocks_ = 0;
+  BatchTuning tuning_{};
+  hipGraphExec_t graph_exec_ = nullptr;
+  StepParams graph_params_{};
+  int32_t graph_steps_ = 0, graph_collide_blocks_ = 0;
   bool uploaded_ = false;
   bool device_newer_ = false;
   float last_ms_ = 0.0f;

@@ -3,6 +3,7 @@
 #include "nsl_batch.h"
 
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -809,6 +810,7 @@ BatchStepper::BatchStepper(int device) : device_(device) {
 BatchStepper::~BatchStepper() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  drop_graph();
   free_tape();
   for (void* p : allocations_) (void)hipFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
@@ -835,6 +837,7 @@ T* BatchStepper::alloc(size_t n, bool zero) {
   const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
   BATCH_CHECK(hipMalloc(&p, bytes));
   allocations_.push_back(p);
+  allocation_bytes_.push_back(bytes);
   if (zero) BATCH_CHECK(hipMemsetAsync(p, 0, bytes, stream_));
   return p;
 }
@@ -964,7 +967,8 @@ void BatchStepper::upload() {
   b.collide_type_cap = (int32_t)std::max<int64_t>(contact_total, 1);
   b.collide_items = alloc<int2>((size_t)kPairTypes * b.collide_type_cap, false);
   b.collide_counts = alloc<int32_t>(kPairTypes, true);
-  b.collide_blocks = collide_grid(device_);
+  resident_collide_blocks_ = collide_grid(device_);
+  b.collide_blocks = tuning_.collide_blocks > 0 ? tuning_.collide_blocks : resident_collide_blocks_;
   b.counters = alloc<int32_t>(kCounterCount, true);
   b.work = alloc<unsigned long long>(4, true);
   BATCH_CHECK(hipStreamSynchronize(stream_));  // the host vectors are read by the copies
@@ -1000,13 +1004,136 @@ void BatchStepper::step(float dt, int32_t sub_steps, int32_t steps) {
   BATCH_CHECK(hipSetDevice(device_));
   if (!uploaded_) upload();
   const StepParams p = step_params(dt, sub_steps);
+  if (tuning_.use_graph) {
+    // capture before the timed region, so the host's capture does not count as device time
+    const int chunk = std::min<int32_t>(steps, kBatchGraphSteps);
+    if (steps >= chunk && (graph_exec_ == nullptr || graph_steps_ != chunk || graph_collide_blocks_ != d_.collide_blocks ||
+                           std::memcmp(&graph_params_, &p, sizeof(StepParams)) != 0)) {
+      drop_graph();
+      hipGraph_t graph = nullptr;
+      BATCH_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+      launch_batch_step(d_, p, chunk, stream_);
+      BATCH_CHECK(hipStreamEndCapture(stream_, &graph));
+      const hipError_t e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        graph_exec_ = nullptr;
+        throw std::runtime_error(std::string("batch step graph: ") + hipGetErrorString(e));
+      }
+      graph_params_ = p;
+      graph_steps_ = chunk;
+      graph_collide_blocks_ = d_.collide_blocks;
+    }
+  }
   BATCH_CHECK(hipEventRecord(ev0_, stream_));
-  launch_batch_step(d_, p, steps, stream_);
+  launch_steps(p, steps);
   BATCH_CHECK(hipEventRecord(ev1_, stream_));
   BATCH_CHECK(hipEventSynchronize(ev1_));
   BATCH_CHECK(hipEventElapsedTime(&last_ms_, ev0_, ev1_));
   body_steps_ += total_awake_ * (int64_t)steps;
   device_newer_ = true;
+}
+
+// The steps of one call: whole chunks by the captured graph (when tuning_.use_graph and
+// the graph matches this call's parameters), the rest by direct launches.
+void BatchStepper::launch_steps(const StepParams& p, int32_t steps) {
+  int32_t done = 0;
+  if (tuning_.use_graph && graph_exec_ != nullptr && graph_collide_blocks_ == d_.collide_blocks &&
+      std::memcmp(&graph_params_, &p, sizeof(StepParams)) == 0)
+    for (; steps - done >= graph_steps_; done += graph_steps_) BATCH_CHECK(hipGraphLaunch(graph_exec_, stream_));
+  if (done < steps) launch_batch_step(d_, p, steps - done, stream_);
+}
+
+void BatchStepper::drop_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  graph_exec_ = nullptr;
+  graph_steps_ = 0;
+}
+
+void BatchStepper::set_tuning(const BatchTuning& t) {
+  if (t.collide_blocks < 0) throw std::invalid_argument("collide_blocks must be >= 0");
+  tuning_.collide_blocks = t.collide_blocks;
+  tuning_.use_graph = t.use_graph != 0 ? 1 : 0;
+  if (uploaded_) d_.collide_blocks = tuning_.collide_blocks > 0 ? tuning_.collide_blocks : resident_collide_blocks_;
+}
+
+std::vector<void*> BatchStepper::snapshot() {
+  std::vector<void*> copies;
+  try {
+    for (size_t i = 0; i < allocations_.size(); ++i) {
+      void* c = nullptr;
+      BATCH_CHECK(hipMalloc(&c, allocation_bytes_[i]));
+      copies.push_back(c);
+      BATCH_CHECK(hipMemcpyAsync(c, allocations_[i], allocation_bytes_[i], hipMemcpyDeviceToDevice, stream_));
+    }
+    BATCH_CHECK(hipStreamSynchronize(stream_));
+  } catch (...) {
+    for (void* c : copies) (void)hipFree(c);
+    throw;
+  }
+  return copies;
+}
+
+void BatchStepper::restore(const std::vector<void*>& copies) {
+  for (size_t i = 0; i < allocations_.size(); ++i)
+    BATCH_CHECK(hipMemcpyAsync(allocations_[i], copies[i], allocation_bytes_[i], hipMemcpyDeviceToDevice, stream_));
+  BATCH_CHECK(hipStreamSynchronize(stream_));
+  device_newer_ = true;
+}
+
+BatchTuning BatchStepper::autotune(float dt, int32_t sub_steps, int32_t steps) {
+  if (!(dt > 0.0f) || steps <= 0) throw std::invalid_argument("autotune needs a positive time step and steps");
+  BATCH_CHECK(hipSetDevice(device_));
+  if (!uploaded_) upload();
+  const BatchTuning start = tuning_;
+  const float last_ms = last_ms_;
+  const int64_t body_steps = body_steps_;
+  const std::vector<void*> snap = snapshot();
+  std::vector<uint64_t> first_hashes, hashes((size_t)world_count());
+  BatchTuning best = start;
+  double best_ms = 0.0, start_ms = 0.0;
+  int candidates = 0;
+  try {
+    const int resident = resident_collide_blocks_;
+    for (int use_graph = 0; use_graph <= 1; ++use_graph)
+      for (int mult : {1, 2, 4, 8, 16}) {
+        BatchTuning t;
+        t.collide_blocks = resident * mult;
+        t.use_graph = use_graph;
+        set_tuning(t);
+        restore(snap);
+        step(dt, sub_steps, std::min<int32_t>(steps, kBatchGraphSteps));  // warm (and capture)
+        restore(snap);
+        step(dt, sub_steps, steps);
+        const double ms = last_ms_ / steps;
+        state_hashes(hashes.data());
+        if (first_hashes.empty()) first_hashes = hashes;
+        else if (hashes != first_hashes) throw std::runtime_error("batch autotune: a launch shape changed the results");
+        ++candidates;
+        if (t.collide_blocks == (start.collide_blocks > 0 ? start.collide_blocks : resident) && use_graph == start.use_graph)
+          start_ms = ms;
+        if (candidates == 1 || ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+  } catch (...) {
+    set_tuning(start);
+    restore(snap);
+    for (void* c : snap) (void)hipFree(c);
+    last_ms_ = last_ms;
+    body_steps_ = body_steps;
+    throw;
+  }
+  set_tuning(best);
+  restore(snap);
+  for (void* c : snap) (void)hipFree(c);
+  last_ms_ = last_ms;
+  body_steps_ = body_steps;
+  tuning_.candidates = candidates;
+  tuning_.step_ms = best_ms;
+  tuning_.previous_ms = start_ms;
+  return tuning_;
 }
 
 void BatchStepper::pull() {

@@ -658,6 +658,37 @@ int nsl_batch_rollout_times(nsl_batch* batch, double* record_ms, double* adjoint
   return NSL_OK;
 }
 
+static void tuning_out(const nsl::BatchTuning& t, nsl_batch_tuning* out) {
+  out->collide_blocks = t.collide_blocks;
+  out->use_graph = t.use_graph;
+  out->candidates = t.candidates;
+  out->reserved = 0;
+  out->step_ms = t.step_ms;
+  out->previous_ms = t.previous_ms;
+}
+
+int nsl_batch_set_tuning(nsl_batch* batch, const nsl_batch_tuning* tuning) {
+  if (!batch || !tuning) return fail(NSL_ERR_INVALID, "null argument");
+  if (tuning->collide_blocks < 0 || tuning->collide_blocks > (1 << 20)) return fail(NSL_ERR_INVALID, "collide_blocks out of range");
+  return guarded([&] {
+    nsl::BatchTuning t;
+    t.collide_blocks = tuning->collide_blocks;
+    t.use_graph = tuning->use_graph;
+    batch->impl->set_tuning(t);
+  });
+}
+
+int nsl_batch_get_tuning(nsl_batch* batch, nsl_batch_tuning* out) {
+  if (!batch || !out) return fail(NSL_ERR_INVALID, "null argument");
+  return guarded([&] { tuning_out(batch->impl->tuning(), out); });
+}
+
+int nsl_batch_autotune(nsl_batch* batch, float dt, int32_t sub_steps, int32_t steps, nsl_batch_tuning* out) {
+  if (!batch || !out) return fail(NSL_ERR_INVALID, "null argument");
+  if (steps < 1 || steps > 1000 || sub_steps < 1 || !(dt > 0.0f)) return fail(NSL_ERR_INVALID, "bad time step or step count");
+  return guarded([&] { tuning_out(batch->impl->autotune(dt, sub_steps, steps), out); });
+}
+
 int nsl_batch_get_stats(nsl_batch* batch, nsl_batch_stats* out) {
   if (!batch || !out) return fail(NSL_ERR_INVALID, "null argument");
   return guarded([&] {

@@ -664,7 +664,6 @@ __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const 
 __device__ __forceinline__ void relax_core(const Constraint& c, Constraint& out, const StepParams& p, BodyState& ba,
                                            BodyState& bb) {
   const int pc = c.point_count;
-  Vec3 twist_a, twist_b;
   ConstraintPoint pt[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) pt[i] = c.points[i];

@@ -78,3 +78,49 @@ def test_batch_body_arrays_match_single_world(gpu_available):
     for k, w in enumerate(singles):
         for x, y in zip(b.body_arrays(k), w.body_arrays()):
             assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+
+
+def test_batch_graph_replay_equals_direct_launches(gpu_available):
+    """Steps replayed from the captured HIP graph (8 steps a replay, the rest launched
+    directly) equal direct launches call after call, with the program's own eager work
+    (hash readback, stats, a one-step call) between the replays, and match the oracle."""
+    from natsel_amd.batch import load_batch
+
+    worlds = scenes.batched(count=12, seed=2468, lo=8, hi=96)
+    a = load_batch(worlds)
+    b = load_batch(worlds)
+    a.set_tuning(use_graph=True)
+    assert a.tuning()["use_graph"] and not b.tuning()["use_graph"]
+    o = _oracles(worlds)
+    done = 0
+    for steps in (8, 16, 1, 8, 21, 8):
+        a.step(1.0 / 60.0, 4, steps)
+        b.step(1.0 / 60.0, 4, steps)
+        done += steps
+        assert np.array_equal(a.state_hashes(), b.state_hashes()), f"after {done} steps"
+        assert a.stats()["error_worlds"] == 0
+    for w in o:
+        for _ in range(done):
+            w.step(1.0 / 60.0, 4)
+    _check(a, o, done)
+
+
+def test_batch_autotune_keeps_state_and_results(gpu_available):
+    """The batch autotuner times its candidates from a snapshot and restores it: the hashes
+    after the call equal those before, and the batch then steps as an untuned copy does."""
+    from natsel_amd.batch import load_batch
+
+    worlds = scenes.batched(count=16, seed=1357, lo=8, hi=128)
+    a = load_batch(worlds)
+    b = load_batch(worlds)
+    a.step(1.0 / 60.0, 4, 20)
+    b.step(1.0 / 60.0, 4, 20)
+    before = a.state_hashes()
+    t = a.autotune(1.0 / 60.0, 4, 8)
+    assert t["candidates"] == 10 and t["step_ms"] > 0.0 and t["step_ms"] <= t["previous_ms"]
+    assert np.array_equal(a.state_hashes(), before)
+    assert a.tuning()["collide_blocks"] == t["collide_blocks"]
+    for _ in range(3):
+        a.step(1.0 / 60.0, 4, 10)
+        b.step(1.0 / 60.0, 4, 10)
+        assert np.array_equal(a.state_hashes(), b.state_hashes())
