@@ -94,23 +94,38 @@ __global__ __launch_bounds__(kBlock) void k_collide(DeviceWorld d, int n) {
 // Polls of a barrier wait before the kernel gives up (a few seconds; far above any phase).
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 
+// The working workgroups of a step-kernel launch: their number (n), this workgroup's number
+// among them (wg) and whether they all run on one XCD.
+struct StepGrid {
+  GridBarrier* bar;
+  uint32_t n;
+  int wg;
+  bool one_xcd;
+};
+
+// XCD of the executing workgroup (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ uint32_t xcd_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15u; }
+
 // Grid-wide barrier over a launch whose workgroups are all resident. Arrivals count up
 // for the whole launch (k_prepare zeroes the count before it): barrier k of the launch is
-// complete when the count reaches (k + 1) * gridDim.x, which each workgroup derives from
-// the count it saw on arrival, so the last arrival's own increment releases the waiters.
-// Each workgroup's agent-scope release before it arrives and acquire after the barrier
-// make every store issued before the barrier visible to every load after it, across the
-// XCDs' L2 caches. A wait that exceeds the spin limit raises `timed_out` and every
+// complete when the count reaches (k + 1) * n, which each workgroup derives from the count
+// it saw on arrival, so the last arrival's own increment releases the waiters. Over all
+// XCDs, each workgroup's agent-scope release before it arrives and acquire after the
+// barrier make every store issued before the barrier visible to every load after it,
+// across the XCDs' L2 caches. On one XCD the workgroups share one L2, which every store
+// has reached once its wave's counters drain: each wave then only invalidates its CU's L1
+// after the barrier. A wait that exceeds the spin limit raises `timed_out` and every
 // workgroup leaves the kernel at its next barrier; returns false then.
-__device__ __forceinline__ bool grid_sync(GridBarrier* b) {
+__device__ __forceinline__ bool grid_sync(const StepGrid& g) {
   __shared__ int s_ok;
+  GridBarrier* b = g.bar;
   __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached L2 before lane 0 releases them
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (!g.one_xcd) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const uint32_t arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t target = (arrived / gridDim.x + 1u) * gridDim.x;
+    const uint32_t target = (arrived / g.n + 1u) * g.n;
     if (arrived + 1u != target) {
       uint32_t spins = 0;
       while (__hip_atomic_load(&b->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -124,10 +139,11 @@ __device__ __forceinline__ bool grid_sync(GridBarrier* b) {
         __builtin_amdgcn_s_sleep(1);
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!g.one_xcd) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   }
   __syncthreads();
+  if (g.one_xcd) asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");  // this CU's L1
   return s_ok != 0;
 }
 
@@ -145,8 +161,9 @@ __device__ void block_add(int32_t* dst, int v) {
 struct PhaseClock {
   uint64_t* log;
   int n;
+  bool lead;  // working workgroup 0
   __device__ __forceinline__ void mark(int cls) {
-    if (log != nullptr && blockIdx.x == 0 && threadIdx.x == 0 && n < kMaxPhaseLog) {
+    if (log != nullptr && lead && threadIdx.x == 0 && n < kMaxPhaseLog) {
       log[2 * n] = (uint64_t)cls;
       log[2 * n + 1] = wall_clock64();
     }
@@ -217,13 +234,16 @@ __device__ __forceinline__ void stage_record(const Constraint* src, float4* rec)
 // rows). A group of small colours runs on workgroup 0 alone, in colour order with a
 // workgroup barrier between colours (the workgroup shares one L1, so its stores are visible
 // to its loads after the barrier): the same items in the same order, fewer grid barriers.
-// With `rec` (two LDS buffers of kRecPieces * kSolveBlock pieces), a lane's first contact
-// item of a colour reads its record from LDS: the lane copied it there during the previous
-// colour's phase (a lane owns the same slot all launch and is the only writer of its
-// record, so the copy is current). `item(slot, ia, ib, staged, hook)`: `staged` is the LDS
-// copy or null, `hook` issues the next colour's copy after the item's body-state loads.
-template <class JointItem, class Item>
-__device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid, int nthreads, GridBarrier* bar, PhaseClock& clock,
+// With kStage (`rec`: two LDS buffers of kRecPieces * kSolveBlock pieces; the world has
+// contact slots), a lane's first contact item of a colour reads its record from LDS: the
+// lane copied it there during the previous colour's phase (a lane owns the same slot all
+// launch and is the only writer of its record, so the copy is current). `item(slot, ia, ib,
+// staged, hook)`: `staged` is the LDS copy or null, `hook` issues the next colour's copy
+// after the item's body-state loads. Every lane issues the copy (a lane with no item in the
+// next colour copies slot 0 into its unused buffer): with no branch around it the item's
+// wait for its body states does not also wait for the copy.
+template <bool kStage, class JointItem, class Item>
+__device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTable& t, int tid, int nthreads, const StepGrid& grid, PhaseClock& clock,
                             int cls, bool with_joints, JointItem joint, Item item, float4* rec = nullptr) {
   const int n_active = t.n;
   if (n_active == 0) return true;
@@ -233,7 +253,7 @@ __device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTa
   for (int a = 0; a < n_active;) {
     const int g = t.group_end[a];
     if (g > a + 1) {
-      if (blockIdx.x == 0) {
+      if (grid.wg == 0) {
         for (int c = a; c < g; ++c) {
           const int nj = joints_in(t, c, with_joints);
           const int jb = t.jbegin[c], cb = t.begin[c] - nj;
@@ -257,16 +277,20 @@ __device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTa
       const int n_items = nj + (t.end[a] - t.begin[a]);
       // the lane's first contact slot of the next colour when that colour runs on its own
       int next_slot = -1;
-      if (rec != nullptr && a + 1 < n_active && t.group_end[a + 1] == a + 2) {
+      if (kStage && a + 1 < n_active && t.group_end[a + 1] == a + 2) {
         const int nj2 = joints_in(t, a + 1, with_joints);
         const int s2 = t.begin[a + 1] + tid - nj2;
         if (tid >= nj2 && s2 < t.end[a + 1]) next_slot = s2;
       }
-      float4* next_rec = rec != nullptr ? rec + (cur ^ 1) * (kRecPieces * kSolveBlock) : nullptr;
+      float4* next_rec = kStage ? rec + (cur ^ 1) * (kRecPieces * kSolveBlock) : nullptr;
       auto hook = [&]() {
-        if (next_slot >= 0) stage_record(d.constraints + next_slot, next_rec);
+        if constexpr (kStage) stage_record(d.constraints + (next_slot >= 0 ? next_slot : 0), next_rec);
       };
-      const float4* mine = staged ? rec + cur * (kRecPieces * kSolveBlock) : nullptr;
+      const float4* mine = kStage && staged ? rec + cur * (kRecPieces * kSolveBlock) : nullptr;
+      if (mine != nullptr) {  // body indices of the staged record (its first piece: id, index_a, index_b, count)
+        const float4 head = mine[threadIdx.x];
+        pf = make_int2(__float_as_int(head.y), __float_as_int(head.z));
+      }
       int k = tid;
       if (k < nj) {
         joint(d.joint_ids[jb + k]);
@@ -288,8 +312,9 @@ __device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTa
       cur ^= 1;
       a += 1;
     }
-    if (a < n_active) pf = first_indices(d, t, a, tid, with_joints);
-    if (!grid_sync(bar)) return false;
+    // the next colour's first body indices: from HBM now, or (staged) from LDS after the barrier
+    if (a < n_active && !staged) pf = first_indices(d, t, a, tid, with_joints);
+    if (!grid_sync(grid)) return false;
     clock.mark(cls);
   }
   return true;
@@ -298,8 +323,8 @@ __device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTa
 // The overflow colour's levels, run by workgroup 0 (one workgroup barrier per level; the
 // workgroup shares one L1, so its stores are visible to its loads after the barrier).
 template <class JointItem, class Item>
-__device__ __forceinline__ void run_overflow_levels(const SolvePlan& plan, JointItem joint, Item item) {
-  if (blockIdx.x != 0) return;
+__device__ __forceinline__ void run_overflow_levels(const SolvePlan& plan, int wg, JointItem joint, Item item) {
+  if (wg != 0) return;
   for (int l = 0; l < plan.n_ovf_levels; ++l) {
     const int b = plan.ovf_level_begin[l], e = plan.ovf_level_begin[l + 1];
     for (int i = b + (int)threadIdx.x; i < e; i += (int)blockDim.x) {
@@ -317,17 +342,28 @@ __device__ __forceinline__ void run_overflow_levels(const SolvePlan& plan, Joint
 // and then colours 0..22 in order; then the restitution iterations. A grid barrier
 // separates dependent phases.
 __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, SolvePlan plan, StepParams p) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nthreads = gridDim.x * blockDim.x;
+  // the working workgroups: every workgroup, or those on XCD 0 numbered in arrival order
+  // (the others leave; should XCD 0 hold fewer than n_work, the first barrier times out)
+  __shared__ int s_wg;
+  if (threadIdx.x == 0)
+    s_wg = !plan.one_xcd ? (int)blockIdx.x
+                         : (xcd_id() == 0 ? (int)__hip_atomic_fetch_add(&plan.barrier->ticket, 1u, __ATOMIC_RELAXED,
+                                                                         __HIP_MEMORY_SCOPE_AGENT)
+                                          : -1);
+  __syncthreads();
+  const int wg = s_wg;
+  if (wg < 0 || wg >= plan.n_work) return;
+  const StepGrid grid{plan.barrier, (uint32_t)plan.n_work, wg, plan.one_xcd != 0};
+  const int tid = wg * blockDim.x + threadIdx.x;
+  const int nthreads = plan.n_work * blockDim.x;
   const int n_ovf = plan.n_overflow;
-  GridBarrier* bar = plan.barrier;
-  PhaseClock clock{plan.phase_log, 0};
+  PhaseClock clock{plan.phase_log, 0, wg == 0};
   clock.mark(-1);
 
   __shared__ ColourTable colours;
   // the staged constraint records of push and relax (two buffers, one per colour in flight)
   __shared__ float4 s_rec[2 * kRecPieces * kSolveBlock];
-  float4* rec = plan.stage_records ? s_rec : nullptr;
+  float4* rec = plan.stage_records ? s_rec : nullptr;  // set only when the world has contact slots
   if (threadIdx.x == 0) {
     int n = 0;
     for (int ci = 0; ci < kOverflowColor; ++ci)
@@ -349,7 +385,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
     // with warm start by body, velocities are integrated in its first phase
     if (plan.ws_table == nullptr) {
       for (int i = tid; i < d.n_awake; i += nthreads) integrate_velocities_item(d, i, p);
-      if (!grid_sync(bar)) return;
+      if (!grid_sync(grid)) return;
       clock.mark(kPIntegrateVelocities);
     }
 
@@ -372,7 +408,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
           }
         };
         run_overflow_levels(
-            plan,
+            plan, wg,
             [&](int32_t jid) {
               integrate_once(d.joints[jid].index_a);
               integrate_once(d.joints[jid].index_b);
@@ -406,7 +442,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         }
         for (; slot < plan.n_constraints; slot += nthreads) warm_delta_contact(d.constraints[slot], plan.ws_delta + 2 * slot);
       }
-      if (!grid_sync(bar)) return;
+      if (!grid_sync(grid)) return;
       clock.mark(kPWarmStart);
       for (int i = tid; i < d.n_awake; i += nthreads) {
         int32_t e[kOverflowColor];
@@ -436,61 +472,65 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         }
         d.states[i] = s;
       }
-      if (!grid_sync(bar)) return;
+      if (!grid_sync(grid)) return;
       clock.mark(kPWarmStart);
     } else {
       if (any_ovf) {
-        run_overflow_levels(plan, [&](int32_t jid) { warm_start_joint_item(d, jid); }, [&](int k) { overflow_warm_item(d, k); });
-        if (!grid_sync(bar)) return;
+        run_overflow_levels(plan, wg, [&](int32_t jid) { warm_start_joint_item(d, jid); }, [&](int k) { overflow_warm_item(d, k); });
+        if (!grid_sync(grid)) return;
         clock.mark(kPOverflow);
       }
-      if (!run_colours(d, colours, tid, nthreads, bar, clock, kPWarmStart, true,
+      if (!run_colours<false>(d, colours, tid, nthreads, grid, clock, kPWarmStart, true,
                        [&](int32_t jid) { warm_start_joint_item(d, jid); },
                        [&](int slot, int32_t ia, int32_t ib, const float4*, auto) { warm_start_item(d, slot, ia, ib); }))
         return;
     }
 
     if (any_ovf) {
-      run_overflow_levels(plan, [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
+      run_overflow_levels(plan, wg, [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
                           [&](int k) { overflow_solve_item(d, k, p, 0); });
-      if (!grid_sync(bar)) return;
+      if (!grid_sync(grid)) return;
       clock.mark(kPOverflow);
     }
-    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPPush, true,
-                     [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
-                     [&](int slot, int32_t ia, int32_t ib, const float4* staged, auto hook) {
-                       push_item(d, slot, p, ia, ib, staged, kSolveBlock, hook);
-                     },
-                     rec))
-      return;
+    {
+      auto joint = [&](int32_t jid) { solve_joint_item(d, jid, p, true); };
+      auto contact = [&](int slot, int32_t ia, int32_t ib, const float4* staged, auto hook) {
+        push_item(d, slot, p, ia, ib, staged, kSolveBlock, hook);
+      };
+      if (!(rec != nullptr ? run_colours<true>(d, colours, tid, nthreads, grid, clock, kPPush, true, joint, contact, rec)
+                           : run_colours<false>(d, colours, tid, nthreads, grid, clock, kPPush, true, joint, contact)))
+        return;
+    }
 
     for (int i = tid; i < d.n_awake; i += nthreads) integrate_positions_item(d, i, p);
-    if (!grid_sync(bar)) return;
+    if (!grid_sync(grid)) return;
     clock.mark(kPIntegratePositions);
 
     if (any_ovf) {
-      run_overflow_levels(plan, [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
+      run_overflow_levels(plan, wg, [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
                           [&](int k) { overflow_solve_item(d, k, p, 1); });
-      if (!grid_sync(bar)) return;
+      if (!grid_sync(grid)) return;
       clock.mark(kPOverflow);
     }
-    if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRelax, true,
-                     [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
-                     [&](int slot, int32_t ia, int32_t ib, const float4* staged, auto hook) {
-                       relax_item(d, slot, p, ia, ib, staged, kSolveBlock, hook);
-                     },
-                     rec))
-      return;
+    {
+      auto joint = [&](int32_t jid) { solve_joint_item(d, jid, p, false); };
+      auto contact = [&](int slot, int32_t ia, int32_t ib, const float4* staged, auto hook) {
+        relax_item(d, slot, p, ia, ib, staged, kSolveBlock, hook);
+      };
+      if (!(rec != nullptr ? run_colours<true>(d, colours, tid, nthreads, grid, clock, kPRelax, true, joint, contact, rec)
+                           : run_colours<false>(d, colours, tid, nthreads, grid, clock, kPRelax, true, joint, contact)))
+        return;
+    }
   }
 
   if (plan.any_restitution) {
     for (int it = 0; it < plan.restitution_iterations; ++it) {
       if (n_ovf > 0) {
-        run_overflow_levels(plan, no_joint, [&](int k) { overflow_solve_item(d, k, p, 2); });
-        if (!grid_sync(bar)) return;
+        run_overflow_levels(plan, wg, no_joint, [&](int k) { overflow_solve_item(d, k, p, 2); });
+        if (!grid_sync(grid)) return;
         clock.mark(kPOverflow);
       }
-      if (!run_colours(d, colours, tid, nthreads, bar, clock, kPRestitution, false, no_joint,
+      if (!run_colours<false>(d, colours, tid, nthreads, grid, clock, kPRestitution, false, no_joint,
                        [&](int slot, int32_t ia, int32_t ib, const float4*, auto) { restitution_item(d, slot, p, ia, ib); }))
         return;
     }
@@ -510,6 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
     // the step kernel's barrier counts its arrivals from zero (stream order: the previous
     // step kernel has finished, the next starts after this kernel)
     __hip_atomic_store(&plan.barrier->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&plan.barrier->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (plan.ws_zero != nullptr) *plan.ws_zero = WarmDelta{zero3(), zero3()};
   }
   for (int k = tid; k < plan.n_joints; k += nthreads) {
@@ -618,8 +659,11 @@ void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const St
   NSL_TIMED(t, kKFinalize, s, hipLaunchKernelGGL(k_store_finalize, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, plan, p));
 }
 
-void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, int blocks, bool cooperative,
-                       hipStream_t s, KernelTimer* t) {
+void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, bool cooperative, hipStream_t s,
+                       KernelTimer* t) {
+  // plan.n_work working workgroups; on one XCD, kXcdCount times as many are launched so that
+  // n_work of them land on XCD 0
+  const int blocks = plan.one_xcd ? plan.n_work * kXcdCount : plan.n_work;
   // A cooperative launch: the runtime starts the grid only when every workgroup can be
   // resident at once, which the grid barriers need (`blocks` is clamped by the caller to the
   // occupancy limit); the barriers' bounded wait still turns a stalled grid into an error.

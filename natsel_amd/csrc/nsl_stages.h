@@ -581,6 +581,16 @@ __device__ __forceinline__ void warm_delta_contact(const Constraint& c, WarmDelt
 // instruction, so consecutive lanes hold consecutive pieces).
 constexpr int kRecPieces = (int)(sizeof(Constraint) / 16);
 static_assert(sizeof(Constraint) % 16 == 0, "constraint records move in 16-byte pieces");
+static_assert(offsetof(Constraint, index_a) == 4 && offsetof(Constraint, index_b) == 8,
+              "the step kernel reads body indices from a staged record's first piece");
+
+// Waits for this wave's outstanding loads (the item's body states and staged-record reads)
+// and keeps the compiler from moving memory operations across the wait.
+__device__ __forceinline__ void wait_before_copy() {
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_sched_barrier(0);
+}
 
 __device__ __forceinline__ Constraint staged_record(const float4* rec, int lanes, int lane) {
   float4 piece[kRecPieces];
@@ -642,18 +652,22 @@ struct NoHook {
 };
 
 // `staged`: the record's copy in LDS (rec, lanes, lane) or null to read it from HBM. `hook`
-// runs right after the body-state loads are issued (the step kernel issues the next colour's
-// record copy there, so that copy overlaps this item's work).
+// runs once the body states have arrived and the staged copy has been read out of LDS (the
+// step kernel issues the next colour's record copy there, so that copy overlaps this item's
+// work). The body states are waited for before the copy is issued: with a copy to LDS in
+// flight, any wait for a load waits for every load, the copy included.
 template <class Hook = NoHook>
 __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib,
                                           const float4* staged = nullptr, int lanes = 0, Hook hook = Hook{}) {
   BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  hook();
   Constraint& out = d.constraints[slot];
   if (staged != nullptr) {
     const Constraint c = staged_record(staged, lanes, (int)threadIdx.x);
+    wait_before_copy();
+    hook();
     push_core(c, out, p, ba, bb);
   } else {
+    hook();
     push_core(out, out, p, ba, bb);
   }
   store_state(d, ia, ba);
@@ -755,12 +769,14 @@ template <class Hook = NoHook>
 __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib,
                                            const float4* staged = nullptr, int lanes = 0, Hook hook = Hook{}) {
   BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  hook();
   Constraint& out = d.constraints[slot];
   if (staged != nullptr) {
     const Constraint c = staged_record(staged, lanes, (int)threadIdx.x);
+    wait_before_copy();
+    hook();
     relax_core(c, out, p, ba, bb);
   } else {
+    hook();
     relax_core(out, out, p, ba, bb);
   }
   store_state(d, ia, ba);
