@@ -396,6 +396,8 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       // warm starting does not change), velocities are integrated and the overflow colour's
       // levels run; then each body applies its items' changes in colour order with the
       // operations the colour-by-colour walk applies to it: the same bits in two phases.
+      // (Computing the changes at the end of each relax item instead measured slower: the
+      // relax phases grew more than this phase shrank.)
       // Velocities are integrated here too (it writes only v and w, which this phase does not
       // otherwise read): workgroup 0 integrates a body of the overflow colour just before its
       // first overflow item in this substep, every lane integrates the other bodies.
