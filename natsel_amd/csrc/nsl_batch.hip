@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kBatchLanes) __attribute__((amdgpu_waves_per_eu(2))
       const int32_t id = r.colors[s_colour_list[a] * r.color_cap + (slot - colours.begin[a])];
       dl.solve_ids[slot] = id;
       float restitution = 0.0f;
-      points += prepare_contact(dl, slot, id, p, &restitution);
+      points += prepare_contact(dl, AosCons{dl.constraints}, slot, id, p, &restitution);
       any_rest = any_rest || restitution > 0.0f;
     }
     for (int k = lane; k < n_ovf; k += kBatchLanes) {
@@ -724,13 +724,13 @@ __global__ __launch_bounds__(kBatchLanes) __attribute__((amdgpu_waves_per_eu(2))
       }
       for (int a = 0; a < colours.n; ++a) {
         for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
-          restitution_item(dl, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
+          restitution_item(dl, AosCons{dl.constraints}, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
         wave_sync();
       }
     }
   }
   for (int k = lane; k < n_ovf; k += kBatchLanes) overflow_store_item(dl, k);
-  for (int slot = lane; slot < n_constraints; slot += kBatchLanes) store_item(dl, slot);
+  for (int slot = lane; slot < n_constraints; slot += kBatchLanes) store_item(dl, AosCons{dl.constraints}, slot);
   for (int i = lane; i < wd.n_awake; i += kBatchLanes) finalize_item(dl, i, p);
   wave_sync();
   for (int i = lane; i < wd.n_awake; i += kBatchLanes) dg.states[i] = s_states[i];

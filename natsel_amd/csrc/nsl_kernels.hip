@@ -212,19 +212,19 @@ __device__ __forceinline__ int2 first_indices(const DeviceWorld& d, const Colour
   const int nj = joints_in(t, a, with_joints);
   const int slot = t.begin[a] + tid - nj;
   if (tid < nj || slot >= t.end[a]) return make_int2(kNullIndex, kNullIndex);
-  return make_int2(d.constraints[slot].index_a, d.constraints[slot].index_b);
+  return blocked_cons(d).indices(slot);
 }
 
-// Copies constraint record `src` into `rec` (kRecPieces pieces of 16 bytes, piece j of this
+// Copies the record of slot `slot` into `rec` (kRecPieces pieces of 16 bytes, piece j of this
 // lane at rec[j * kSolveBlock + threadIdx.x]) with global_load_lds: the copy goes from HBM to
 // LDS without registers and completes by the next grid barrier (its wait for all memory
-// operations), so it overlaps the rest of the phase.
-__device__ __forceinline__ void stage_record(const Constraint* src, float4* rec) {
+// operations), so it overlaps the rest of the phase. Lanes with consecutive slots read one
+// contiguous KiB per piece (the blocked record layout).
+__device__ __forceinline__ void stage_record(const BlockedCons& cons, int slot, float4* rec) {
   const int wave_base = (int)threadIdx.x & ~63;
-  const char* g = reinterpret_cast<const char*>(src);
 #pragma unroll
   for (int j = 0; j < kRecPieces; ++j)
-    __builtin_amdgcn_global_load_lds(g + 16 * j, rec + j * kSolveBlock + wave_base, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(cons.piece(slot, j), rec + j * kSolveBlock + wave_base, 16, 0, 0);
 }
 
 // One stage over the active colours in order, a grid barrier after each colour or group
@@ -263,7 +263,10 @@ __device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTa
               joint(d.joint_ids[jb + k]);
             } else {
               const int slot = cb + k;
-              item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b, nullptr, NoHook{});
+              {
+                const int2 ix = blocked_cons(d).indices(slot);
+                item(slot, ix.x, ix.y, nullptr, NoHook{});
+              }
             }
           }
           if (c + 1 < g) __syncthreads();
@@ -284,7 +287,7 @@ __device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTa
       }
       float4* next_rec = kStage ? rec + (cur ^ 1) * (kRecPieces * kSolveBlock) : nullptr;
       auto hook = [&]() {
-        if constexpr (kStage) stage_record(d.constraints + (next_slot >= 0 ? next_slot : 0), next_rec);
+        if constexpr (kStage) stage_record(blocked_cons(d), next_slot >= 0 ? next_slot : 0, next_rec);
       };
       const float4* mine = kStage && staged ? rec + cur * (kRecPieces * kSolveBlock) : nullptr;
       if (mine != nullptr) {  // body indices of the staged record (its first piece: id, index_a, index_b, count)
@@ -305,7 +308,8 @@ __device__ __forceinline__ bool run_colours(const DeviceWorld& d, const ColourTa
           joint(d.joint_ids[jb + k]);
         } else {
           const int slot = cb + k;
-          item(slot, d.constraints[slot].index_a, d.constraints[slot].index_b, nullptr, NoHook{});
+          const int2 ix = blocked_cons(d).indices(slot);
+          item(slot, ix.x, ix.y, nullptr, NoHook{});
         }
       }
       staged = next_slot >= 0;
@@ -438,11 +442,11 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         for (; slot + (kU - 1) * nthreads < plan.n_constraints; slot += kU * nthreads) {
           Constraint c[kU];
 #pragma unroll
-          for (int u = 0; u < kU; ++u) c[u] = d.constraints[slot + u * nthreads];
+          for (int u = 0; u < kU; ++u) c[u] = blocked_cons(d).load(slot + u * nthreads);
 #pragma unroll
           for (int u = 0; u < kU; ++u) warm_delta_contact(c[u], plan.ws_delta + 2 * (slot + u * nthreads));
         }
-        for (; slot < plan.n_constraints; slot += nthreads) warm_delta_contact(d.constraints[slot], plan.ws_delta + 2 * slot);
+        for (; slot < plan.n_constraints; slot += nthreads) warm_delta_contact(blocked_cons(d).load(slot), plan.ws_delta + 2 * slot);
       }
       if (!grid_sync(grid)) return;
       clock.mark(kPWarmStart);
@@ -533,7 +537,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         clock.mark(kPOverflow);
       }
       if (!run_colours<false>(d, colours, tid, nthreads, grid, clock, kPRestitution, false, no_joint,
-                       [&](int slot, int32_t ia, int32_t ib, const float4*, auto) { restitution_item(d, slot, p, ia, ib); }))
+                       [&](int slot, int32_t ia, int32_t ib, const float4*, auto) { restitution_item(d, blocked_cons(d), slot, p, ia, ib); }))
         return;
     }
   }
@@ -575,15 +579,15 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
     }
   }
   for (int slot = tid; slot < plan.n_constraints; slot += nthreads) {
-    points += prepare_item(d, slot, p);
-    if (plan.ws_table != nullptr) {
-      int c = 0;
-      while (c < kOverflowColor - 1 && slot >= plan.cr.end[c]) ++c;  // slots are colour-major
-      const Constraint& cc = d.constraints[slot];
-      if (cc.index_a >= 0) plan.ws_table[c * d.n_awake + cc.index_a] = slot << 1;
-      if (cc.index_b >= 0) plan.ws_table[c * d.n_awake + cc.index_b] = slot << 1 | 1;
-      warm_delta_contact(cc, plan.ws_delta + 2 * slot);  // the first substep's
-    }
+    points += prepare_item(d, slot, p, [&](const Constraint& cc) {
+      if (plan.ws_table != nullptr) {
+        int c = 0;
+        while (c < kOverflowColor - 1 && slot >= plan.cr.end[c]) ++c;  // slots are colour-major
+        if (cc.index_a >= 0) plan.ws_table[c * d.n_awake + cc.index_a] = slot << 1;
+        if (cc.index_b >= 0) plan.ws_table[c * d.n_awake + cc.index_b] = slot << 1 | 1;
+        warm_delta_contact(cc, plan.ws_delta + 2 * slot);  // the first substep's
+      }
+    });
   }
   for (int k = tid; k < plan.n_overflow; k += nthreads) {
     ovf_points += prepare_overflow_item(d, k, p);
@@ -602,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_store_finalize(DeviceWorld d, SolveP
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
   for (int k = tid; k < plan.n_overflow; k += nthreads) overflow_store_item(d, k);
-  for (int slot = tid; slot < plan.n_constraints; slot += nthreads) store_item(d, slot);
+  for (int slot = tid; slot < plan.n_constraints; slot += nthreads) store_item(d, blocked_cons(d), slot);
   for (int i = tid; i < d.n_awake; i += nthreads) finalize_item(d, i, p);
 }
 

@@ -231,11 +231,100 @@ __device__ uint8_t collide_contact(const DeviceWorld& d, int32_t id) {
 }
 
 // ------------------------------------------------------------------------------
+// constraint record stores
+
+// A record moves in kRecPieces pieces of 16 bytes.
+constexpr int kRecPieces = (int)(sizeof(Constraint) / 16);
+static_assert(sizeof(Constraint) % 16 == 0, "constraint records move in 16-byte pieces");
+static_assert(offsetof(Constraint, index_a) == 4 && offsetof(Constraint, index_b) == 8,
+              "body indices sit in a record's first piece");
+
+// Records in slot order (the batched stepper).
+struct AosCons {
+  Constraint* p;
+  __device__ __forceinline__ Constraint load(int s) const { return p[s]; }
+  __device__ __forceinline__ void store(int s, const Constraint& c) const { p[s] = c; }
+  __device__ __forceinline__ int2 indices(int s) const { return make_int2(p[s].index_a, p[s].index_b); }
+  template <class T>
+  __device__ __forceinline__ void put(int s, size_t off, T v) const {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p + s) + off) = v;
+  }
+};
+
+// Records of the single world in blocks of kRecBlock slots (nsl_types.h), piece-major inside a
+// block: piece j of slot s at p[((s / kRecBlock) * kRecPieces + j) * kRecBlock + s % kRecBlock].
+// A wavefront touching consecutive slots reads or writes one contiguous KiB per piece (a record
+// in slot order would spread each such access over 64 cache lines). Capacity: whole blocks.
+struct BlockedCons {
+  float4* p;
+  __device__ __forceinline__ float4* piece(int s, int j) const {
+    return p + ((size_t)(s / kRecBlock) * kRecPieces + j) * kRecBlock + (s % kRecBlock);
+  }
+  __device__ __forceinline__ Constraint load(int s) const {
+    Constraint c;
+    float4* q = reinterpret_cast<float4*>(&c);  // 16-byte aligned, kRecPieces pieces
+#pragma unroll
+    for (int j = 0; j < kRecPieces; ++j) q[j] = *piece(s, j);
+    return c;
+  }
+  __device__ __forceinline__ void store(int s, const Constraint& c) const {
+    const float4* q = reinterpret_cast<const float4*>(&c);
+#pragma unroll
+    for (int j = 0; j < kRecPieces; ++j) *piece(s, j) = q[j];
+  }
+  __device__ __forceinline__ int2 indices(int s) const {
+    const float4 h = *piece(s, 0);
+    return make_int2(__float_as_int(h.y), __float_as_int(h.z));
+  }
+  // a field of T (4-byte words) at byte offset `off` of the record
+  template <class T>
+  __device__ __forceinline__ void put(int s, size_t off, T v) const {
+    static_assert(sizeof(T) % 4 == 0, "fields are 4-byte words");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (size_t k = 0; k < sizeof(T) / 4; ++k) {
+      const size_t o = off + 4 * k;
+      reinterpret_cast<uint32_t*>(piece(s, (int)(o / 16)))[(o % 16) / 4] = w[k];
+    }
+  }
+};
+__device__ __forceinline__ BlockedCons blocked_cons(const DeviceWorld& d) { return BlockedCons{reinterpret_cast<float4*>(d.constraints)}; }
+
+// The solver stages' writes to one slot's record.
+template <class Cons>
+struct ConOut {
+  Cons cons;
+  int slot;
+  static constexpr size_t point_off(int i) { return offsetof(Constraint, points) + (size_t)i * sizeof(ConstraintPoint); }
+  __device__ __forceinline__ void normal_impulse(int i, float v) const {
+    cons.put(slot, point_off(i) + offsetof(ConstraintPoint, normal_impulse), v);
+  }
+  __device__ __forceinline__ void total_normal_impulse(int i, float v) const {
+    cons.put(slot, point_off(i) + offsetof(ConstraintPoint, total_normal_impulse), v);
+  }
+  __device__ __forceinline__ void restitution_impulse(int i, float v) const {
+    cons.put(slot, point_off(i) + offsetof(ConstraintPoint, restitution_impulse), v);
+  }
+  __device__ __forceinline__ void twist_impulse(float v) const { cons.put(slot, offsetof(Constraint, twist_impulse), v); }
+  __device__ __forceinline__ void friction_impulse(Vec2 v) const { cons.put(slot, offsetof(Constraint, friction_impulse), v); }
+  __device__ __forceinline__ void rolling_impulse(Vec3 v) const { cons.put(slot, offsetof(Constraint, rolling_impulse), v); }
+};
+template <class Cons>
+__device__ __forceinline__ ConOut<Cons> con_out(Cons cons, int slot) { return ConOut<Cons>{cons, slot}; }
+
+// ------------------------------------------------------------------------------
 // constraint preparation (contact_solver.rs:395-662)
 
-// Returns the number of manifold points prepared.
-__device__ __forceinline__ int prepare_contact(const DeviceWorld& d, int slot, int32_t id, const StepParams& p,
-                                               float* restitution = nullptr) {
+struct NoPost {
+  __device__ void operator()(const Constraint&) const {}
+};
+
+// Returns the number of manifold points prepared; the record goes to `cons`, and `post` sees it
+// while it is still in registers.
+template <class Cons, class Post = NoPost>
+__device__ __forceinline__ int prepare_contact(const DeviceWorld& d, Cons cons, int slot, int32_t id, const StepParams& p,
+                                               float* restitution = nullptr, Post post = Post{}) {
   const Contact& contact = d.contacts[id];
   const float inv_tau = 1.0f / kSpeculativeDistance;
   Constraint cc;
@@ -333,13 +422,16 @@ __device__ __forceinline__ int prepare_contact(const DeviceWorld& d, int slot, i
       cc.points[i].relative_velocity = dot(normal, sub(vr_b, vr_a));
     }
   }
-  d.constraints[slot] = cc;
+  cons.store(slot, cc);
   if (restitution) *restitution = cc.restitution;
+  post(cc);
   return cc.point_count;
 }
 
-__device__ __forceinline__ int prepare_item(const DeviceWorld& d, int slot, const StepParams& p) {
-  return prepare_contact(d, slot, d.solve_ids[slot], p);
+// The single world's coloured slot `slot`.
+template <class Post>
+__device__ __forceinline__ int prepare_item(const DeviceWorld& d, int slot, const StepParams& p, Post post) {
+  return prepare_contact(d, blocked_cons(d), slot, d.solve_ids[slot], p, nullptr, post);
 }
 
 // overflow_contact_solver.rs:92-257
@@ -557,7 +649,7 @@ __device__ __forceinline__ WarmImpulse warm_impulse(const Constraint& c) {
 
 __device__ __forceinline__ void warm_start_item(const DeviceWorld& d, int slot, int32_t ia, int32_t ib) {
   BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  const Constraint& c = d.constraints[slot];
+  const Constraint c = blocked_cons(d).load(slot);
   const WarmImpulse wi = warm_impulse(c);
   ba.w = sub(ba.w, sym_mv(c.inv_ia, wi.ang_a));
   ba.v = sub(ba.v, scale(c.inv_mass_a, wi.linear));
@@ -579,10 +671,6 @@ __device__ __forceinline__ void warm_delta_contact(const Constraint& c, WarmDelt
 // A constraint record staged in LDS by the step kernel: 16-byte piece j of the record of
 // lane `lane` at rec[j * lanes + lane] (written by global_load_lds, one piece per lane per
 // instruction, so consecutive lanes hold consecutive pieces).
-constexpr int kRecPieces = (int)(sizeof(Constraint) / 16);
-static_assert(sizeof(Constraint) % 16 == 0, "constraint records move in 16-byte pieces");
-static_assert(offsetof(Constraint, index_a) == 4 && offsetof(Constraint, index_b) == 8,
-              "the step kernel reads body indices from a staged record's first piece");
 
 // Waits for this wave's outstanding loads (the item's body states and staged-record reads)
 // and keeps the compiler from moving memory operations across the wait.
@@ -601,9 +689,10 @@ __device__ __forceinline__ Constraint staged_record(const float4* rec, int lanes
   return c;
 }
 
-// contact_solver.rs:796-894. Reads the record from `c` and writes its impulses to `out`
-// (the same record, or `c` a staged copy of it).
-__device__ __forceinline__ void push_core(const Constraint& c, Constraint& out, const StepParams& p, BodyState& ba,
+// contact_solver.rs:796-894. Reads the record from `c` (a copy) and writes its impulses
+// through `out` (ConOut).
+template <class Out>
+__device__ __forceinline__ void push_core(const Constraint& c, const Out& out, const StepParams& p, BodyState& ba,
                                           BodyState& bb) {
   const int pc = c.point_count;
   ConstraintPoint pt[4];
@@ -635,7 +724,7 @@ __device__ __forceinline__ void push_core(const Constraint& c, Constraint& out, 
       float v_n = lnv + (dot(u_b, bb.w) - dot(u_a, ba.w));
       float neg_impulse = cp.normal_mass * (pm * v_n + velocity_bias) + pi * cp.normal_impulse;
       float impulse = maxf(-neg_impulse, -cp.normal_impulse);
-      out.points[i].normal_impulse = cp.normal_impulse + impulse;
+      out.normal_impulse(i, cp.normal_impulse + impulse);
       lnv += inv_mass * impulse;
       manifold_impulse += impulse;
       ba.w = sub(ba.w, scale(impulse, g_a));
@@ -660,22 +749,23 @@ template <class Hook = NoHook>
 __device__ __forceinline__ void push_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib,
                                           const float4* staged = nullptr, int lanes = 0, Hook hook = Hook{}) {
   BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  Constraint& out = d.constraints[slot];
+  const BlockedCons cons = blocked_cons(d);
   if (staged != nullptr) {
     const Constraint c = staged_record(staged, lanes, (int)threadIdx.x);
     wait_before_copy();
     hook();
-    push_core(c, out, p, ba, bb);
+    push_core(c, con_out(cons, slot), p, ba, bb);
   } else {
     hook();
-    push_core(out, out, p, ba, bb);
+    push_core(cons.load(slot), con_out(cons, slot), p, ba, bb);
   }
   store_state(d, ia, ba);
   store_state(d, ib, bb);
 }
 
-// contact_solver.rs:927-1097 (reads `c`, writes `out` as push_core)
-__device__ __forceinline__ void relax_core(const Constraint& c, Constraint& out, const StepParams& p, BodyState& ba,
+// contact_solver.rs:927-1097 (reads `c`, writes through `out` as push_core)
+template <class Out>
+__device__ __forceinline__ void relax_core(const Constraint& c, const Out& out, const StepParams& p, BodyState& ba,
                                            BodyState& bb) {
   const int pc = c.point_count;
   ConstraintPoint pt[4];
@@ -705,8 +795,8 @@ __device__ __forceinline__ void relax_core(const Constraint& c, Constraint& out,
       float neg_impulse = cp.normal_mass * (v_n + velocity_bias);
       float impulse = maxf(-neg_impulse, -cp.normal_impulse);
       float new_impulse = cp.normal_impulse + impulse;
-      out.points[i].normal_impulse = new_impulse;
-      out.points[i].total_normal_impulse = cp.total_normal_impulse + new_impulse;
+      out.normal_impulse(i, new_impulse);
+      out.total_normal_impulse(i, cp.total_normal_impulse + new_impulse);
       total_normal += new_impulse;
       total_twist_limit += cp.lever_arm * new_impulse;
       lnv += inv_mass * impulse;
@@ -723,7 +813,7 @@ __device__ __forceinline__ void relax_core(const Constraint& c, Constraint& out,
       bool mask = lsq > eps + max_impulse * max_impulse;
       float nrm = max_impulse / (sqrtf_(lsq) + eps);
       ri = scale(mask ? nrm : 1.0f, ri);
-      out.rolling_impulse = ri;
+      out.rolling_impulse(ri);
       delta = sub(ri, old);
       ba.w = sub(ba.w, sym_mv(c.inv_ia, delta));
       bb.w = add(bb.w, sym_mv(c.inv_ib, delta));
@@ -734,7 +824,7 @@ __device__ __forceinline__ void relax_core(const Constraint& c, Constraint& out,
       float old = c.twist_impulse;
       float impulse = -(c.twist_mass * twist_speed);
       impulse = maxf(-max_lambda - old, minf(impulse, max_lambda - old));
-      out.twist_impulse = old + impulse;
+      out.twist_impulse(old + impulse);
       ba.w = sub(ba.w, scale(impulse, sym_mv(c.inv_ia, normal)));
       bb.w = add(bb.w, scale(impulse, sym_mv(c.inv_ib, normal)));
     }
@@ -753,7 +843,7 @@ __device__ __forceinline__ void relax_core(const Constraint& c, Constraint& out,
       float sc = mask ? nrm : 1.0f;
       ni = Vec2{sc * ni.x, sc * ni.y};
       di = Vec2{ni.x - fi_old.x, ni.y - fi_old.y};
-      out.friction_impulse = ni;
+      out.friction_impulse(ni);
       Vec3 pp = scale(di.x, t1);
       pp = add(pp, scale(di.y, t2));
       ba.w = sub(ba.w, sym_mv(c.inv_ia, cross(r_a, pp)));
@@ -769,15 +859,15 @@ template <class Hook = NoHook>
 __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib,
                                            const float4* staged = nullptr, int lanes = 0, Hook hook = Hook{}) {
   BodyState ba = load_state(d, ia), bb = load_state(d, ib);
-  Constraint& out = d.constraints[slot];
+  const BlockedCons cons = blocked_cons(d);
   if (staged != nullptr) {
     const Constraint c = staged_record(staged, lanes, (int)threadIdx.x);
     wait_before_copy();
     hook();
-    relax_core(c, out, p, ba, bb);
+    relax_core(c, con_out(cons, slot), p, ba, bb);
   } else {
     hook();
-    relax_core(out, out, p, ba, bb);
+    relax_core(cons.load(slot), con_out(cons, slot), p, ba, bb);
   }
   store_state(d, ia, ba);
   store_state(d, ib, bb);
@@ -797,20 +887,23 @@ __device__ __forceinline__ void warm_start_record(const DeviceWorld& d, const Co
 }
 __device__ __forceinline__ void push_record(const DeviceWorld& d, int slot, const StepParams& p, const Constraint& c) {
   BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
-  push_core(c, d.constraints[slot], p, ba, bb);
+  push_core(c, con_out(AosCons{d.constraints}, slot), p, ba, bb);
   store_state(d, c.index_a, ba);
   store_state(d, c.index_b, bb);
 }
 __device__ __forceinline__ void relax_record(const DeviceWorld& d, int slot, const StepParams& p, const Constraint& c) {
   BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
-  relax_core(c, d.constraints[slot], p, ba, bb);
+  relax_core(c, con_out(AosCons{d.constraints}, slot), p, ba, bb);
   store_state(d, c.index_a, ba);
   store_state(d, c.index_b, bb);
 }
 
 // contact_solver.rs:1135-1229
-__device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot, const StepParams& p, int32_t ia, int32_t ib) {
-  Constraint& c = d.constraints[slot];
+template <class Cons>
+__device__ __forceinline__ void restitution_item(const DeviceWorld& d, Cons cons, int slot, const StepParams& p, int32_t ia,
+                                                 int32_t ib) {
+  const Constraint c = cons.load(slot);
+  const ConOut<Cons> out = con_out(cons, slot);
   const bool propagate = p.enable_restitution_propagation != 0;
   if (!propagate && c.restitution == 0.0f) return;
   BodyState ba = load_state(d, ia), bb = load_state(d, ib);
@@ -843,9 +936,9 @@ __device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot,
       float allowance = c.restitution * (compression + approach) - cp.restitution_impulse;
       float max_impulse = approach + maxf(allowance, 0.0f);
       impulse = armed ? minf(impulse, max_impulse) : impulse;
-      c.points[i].normal_impulse = cp.normal_impulse + impulse;
-      c.points[i].restitution_impulse = cp.restitution_impulse + (impulse - approach);
-      c.points[i].total_normal_impulse = cp.total_normal_impulse + impulse;
+      out.normal_impulse(i, cp.normal_impulse + impulse);
+      out.restitution_impulse(i, cp.restitution_impulse + (impulse - approach));
+      out.total_normal_impulse(i, cp.total_normal_impulse + impulse);
       Vec3 pp = scale(impulse, normal);
       ba.w = sub(ba.w, sym_mv(c.inv_ia, cross(cp.r_a, pp)));
       ba.v = sub(ba.v, scale(c.inv_mass_a, pp));
@@ -858,13 +951,16 @@ __device__ __forceinline__ void restitution_item(const DeviceWorld& d, int slot,
 }
 
 // contact_solver.rs:1258-1299
-__device__ __forceinline__ void store_item(const DeviceWorld& d, int slot) {
-  const Constraint& c = d.constraints[slot];
+template <class Cons>
+__device__ __forceinline__ void store_item(const DeviceWorld& d, Cons cons, int slot) {
+  const Constraint c = cons.load(slot);
   Contact& contact = d.contacts[c.contact_id];
   if (contact.manifold_count <= 0 || c.manifold_count <= 0) return;
   bool hit = (contact.flags & cflag::kHitEvent) != 0;
   int count = mini(contact.point_count, c.point_count);
-  for (int i = 0; i < count; ++i) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {  // constant indices: the record stays in registers
+    if (i >= count) break;
     contact.points[i].normal_impulse = c.points[i].normal_impulse;
     contact.points[i].total_normal_impulse = c.points[i].total_normal_impulse;
     if (hit) contact.points[i].normal_velocity = c.points[i].relative_velocity;

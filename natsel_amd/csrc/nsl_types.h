@@ -145,6 +145,9 @@ struct alignas(16) Constraint {
   ConstraintPoint points[4];
 };
 
+// Slots per block of the single world's blocked constraint-record layout (one wavefront).
+constexpr int kRecBlock = 64;
+
 // Sequential overflow-colour constraint (overflow_contact_solver.rs:17-33): full
 // matrices, different arithmetic from the coloured path.
 struct OverflowConstraint {

@@ -983,7 +983,8 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   cr.begin[kOverflowColor] = cr.end[kOverflowColor] = (int32_t)solve.size();
   if (!solve.empty()) {
     ensure(d_.solve_ids, cap_solve_, solve.size(), false);
-    ensure(d_.constraints, cap_constraints_, solve.size(), false);
+    // the step kernels keep records in blocks of kRecBlock slots
+    ensure(d_.constraints, cap_constraints_, (solve.size() + kRecBlock - 1) / kRecBlock * kRecBlock, false);
     NSL_CHECK(hipMemcpyAsync(d_.solve_ids, solve.data(), solve.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
   }
   if (!ovf.empty()) {
