@@ -457,8 +457,6 @@ def main() -> int:
     ap.add_argument("--plain-launch", action="store_true",
                     help="launch the step kernel without the cooperative API (same grid, same results): rocprofv3 "
                          "ends before writing its output for a process that made cooperative launches")
-    ap.add_argument("--one-xcd", action="store_true",
-                    help="start the step kernel on one XCD (the autotuner may still change it)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape of the step kernel instead of autotuning it after the settle")
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched measurement (0: skip it)")
@@ -479,8 +477,6 @@ def main() -> int:
     args = ap.parse_args()
     if args.plain_launch:
         WORLD_OPTS["plain_step_launch"] = True
-    if args.one_xcd:
-        WORLD_OPTS["step_one_xcd"] = True
     if args.summarize_profile:
         settle = settle_steps(args.steps, args.warmup)
         window = args.window or f"{settle + 1}:{settle + args.steps}"
@@ -544,8 +540,7 @@ def main() -> int:
     world.set_kernel_timing(False)
     ks_phases = world.kernel_stats()
     rerun_equal = world.world_state_hash() == hashes["end"]
-    world_opts = {"small_colour_items": world.options.small_colour_items, "stage_records": int(world.options.stage_records),
-                  "step_one_xcd": int(world.options.step_one_xcd)}
+    world_opts = {"small_colour_items": world.options.small_colour_items, "stage_records": int(world.options.stage_records)}
     _progress(f"pile: {args.steps} timed steps, {1e3 * elapsed / args.steps:.3f} ms per step")
     del snap
     world.close()  # release device memory now, not at interpreter exit
@@ -573,7 +568,6 @@ def main() -> int:
         "grid_barriers_per_launch": ks["barriers"] / launches,
         "small_colour_items": world_opts["small_colour_items"],
         "stage_records": world_opts["stage_records"],
-        "step_one_xcd": world_opts["step_one_xcd"],
         "step_launch": "plain" if args.plain_launch else "cooperative",
         "workgroups": ks["solver_blocks"],
         "autotune": tuning,

@@ -86,9 +86,6 @@ typedef struct nsl_options {
                                         guarantees that every workgroup is resident); 1 = an ordinary launch of
                                         the same grid, sized to the occupancy limit, for tools that cannot
                                         follow cooperative launches; results do not change */
-  int32_t step_one_xcd;              /* 0: 1 = the step kernel's workgroups all run on one XCD (at most its
-                                        CUs), so its grid barriers need no write-back and invalidate of the
-                                        other XCDs' L2 caches; results do not change */
 } nsl_options;
 
 typedef struct nsl_body_def {
@@ -349,7 +346,6 @@ typedef struct nsl_tuning {
   double step_kernel_us;    /* average step-kernel time of that shape over the trial steps */
   double previous_us;       /* the same for the options the world had before the call */
   int32_t candidates;       /* shapes tried */
-  int32_t step_one_xcd;     /* now set in the world's options */
 } nsl_tuning;
 /* Picks the step kernel's launch shape for this world as it is now: from a snapshot, runs
    `steps` steps of (dt, sub_steps) under each candidate (workgroups around the size the

@@ -30,7 +30,6 @@ class Options(ctypes.Structure):
         ("small_colour_items", ctypes.c_int32),
         ("stage_records", ctypes.c_int32),
         ("plain_step_launch", ctypes.c_int32),
-        ("step_one_xcd", ctypes.c_int32),
     ]
 
 
@@ -175,8 +174,7 @@ class LocalManifold(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("solver_blocks", ctypes.c_int32), ("small_colour_items", ctypes.c_int32), ("stage_records", ctypes.c_int32),
-                ("step_kernel_us", ctypes.c_double), ("previous_us", ctypes.c_double), ("candidates", ctypes.c_int32),
-                ("step_one_xcd", ctypes.c_int32)]
+                ("step_kernel_us", ctypes.c_double), ("previous_us", ctypes.c_double), ("candidates", ctypes.c_int32)]
 
 
 class BatchTuning(ctypes.Structure):

@@ -61,7 +61,6 @@ void apply(nsl::WorldOptions& o, const nsl_options& in) {
   o.small_colour_items = in.small_colour_items;
   o.stage_records = in.stage_records != 0 ? 1 : 0;
   o.plain_step_launch = in.plain_step_launch != 0 ? 1 : 0;
-  o.step_one_xcd = in.step_one_xcd != 0 ? 1 : 0;
 }
 
 nsl::Material material(const nsl_material* m) {
@@ -142,7 +141,6 @@ void nsl_default_options(nsl_options* out) {
   out->small_colour_items = o.small_colour_items;
   out->stage_records = o.stage_records;
   out->plain_step_launch = o.plain_step_launch;
-  out->step_one_xcd = o.step_one_xcd;
 }
 
 int nsl_world_create(int device, const nsl_options* options, nsl_world** out) {
@@ -482,7 +480,6 @@ int nsl_world_autotune(nsl_world* world, float dt, int32_t sub_steps, int32_t st
     out->step_kernel_us = t.best_us;
     out->previous_us = t.previous_us;
     out->candidates = t.candidates;
-    out->step_one_xcd = t.one_xcd;
   });
 }
 

@@ -84,11 +84,10 @@ struct KernelTimer {
 };
 
 // Grid barrier state of the step kernel (zeroed at creation; count holds the arrivals of
-// the current launch and ticket the workgroup numbers handed out on one XCD, both zeroed by
-// k_prepare before each launch).
+// the current launch, zeroed by k_prepare before each).
 struct GridBarrier {
   uint32_t count;
-  uint32_t ticket;
+  uint32_t gen;  // unused
   uint32_t timed_out;
   uint32_t pad;
 };
@@ -122,11 +121,7 @@ struct SolvePlan {
   int32_t restitution_iterations;
   int32_t small_colour_items;     // runs of colours this small go to workgroup 0 (0: off)
   int32_t stage_records;          // push and relax read their records from LDS copies made a colour ahead
-  // The step kernel's working workgroups (the barrier's arrival count per barrier). With
-  // one_xcd the grid is kXcdCount times wider and only workgroups on XCD 0 work: they share
-  // that XCD's L2, so a grid barrier needs no cross-XCD write-back and invalidate.
-  int32_t n_work;
-  int32_t one_xcd;
+  int32_t n_work;                 // the step kernel's workgroups (the barrier's arrivals per barrier)
   // Warm start by body (every awake body dynamic): ws_table[c * n_awake + i] is the item of
   // colour c (0..22) that touches awake body i, with the body's side s (0: a, 1: b): a
   // contact slot as (slot << 1 | s), a joint as -((joint id << 1 | s) + 2), or -1. k_prepare
@@ -172,8 +167,5 @@ void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const St
 // Workgroups of the step kernel that fit on the device at once.
 int solve_step_max_blocks(int device);
 constexpr int kSolveBlock = 128;  // measured: 128 lanes beat 64, 256 and 512 on the 10k pile
-// XCDs (each with its own L2) of the device; the hardware hands workgroup i of a launch to
-// XCD i mod kXcdCount.
-constexpr int kXcdCount = 8;
 
 }  // namespace nsl

@@ -94,36 +94,35 @@ __global__ __launch_bounds__(kBlock) void k_collide(DeviceWorld d, int n) {
 // Polls of a barrier wait before the kernel gives up (a few seconds; far above any phase).
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 
-// The working workgroups of a step-kernel launch: their number (n), this workgroup's number
-// among them (wg) and whether they all run on one XCD.
+// The workgroups of a step-kernel launch: their number (n) and this workgroup's (wg).
 struct StepGrid {
   GridBarrier* bar;
   uint32_t n;
   int wg;
-  bool one_xcd;
 };
-
-// XCD of the executing workgroup (hardware register XCC_ID, bits 3:0).
-__device__ __forceinline__ uint32_t xcd_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15u; }
 
 // Grid-wide barrier over a launch whose workgroups are all resident. Arrivals count up
 // for the whole launch (k_prepare zeroes the count before it): barrier k of the launch is
 // complete when the count reaches (k + 1) * n, which each workgroup derives from the count
-// it saw on arrival, so the last arrival's own increment releases the waiters. Over all
-// XCDs, each workgroup's agent-scope release before it arrives and acquire after the
-// barrier make every store issued before the barrier visible to every load after it,
-// across the XCDs' L2 caches. On one XCD the workgroups share one L2, which every store
-// has reached once its wave's counters drain: each wave then only invalidates its CU's L1
-// after the barrier. A wait that exceeds the spin limit raises `timed_out` and every
-// workgroup leaves the kernel at its next barrier; returns false then.
+// it saw on arrival, so the last arrival's own increment releases the waiters. Each
+// workgroup's agent-scope release before it arrives and acquire after the barrier make every
+// store issued before the barrier visible to every load after it, across the XCDs' L2
+// caches. (Keeping the grid on one XCD with only an L1 invalidation measured faster but
+// lost updates: an invalidation at workgroup scope leaves the CU's L1 lines in place; with
+// an agent-scope acquire it was slower than the grid over all XCDs.) A wait that exceeds
+// the spin limit raises `timed_out` and every workgroup leaves the kernel at its next
+// barrier; returns false then.
 __device__ __forceinline__ bool grid_sync(const StepGrid& g) {
   __shared__ int s_ok;
   GridBarrier* b = g.bar;
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached L2 before lane 0 releases them
+  // every store of this wave is issued before the wait (the wait alone does not keep the
+  // compiler from sinking stores below it), and has reached L2 after it
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
-    if (!g.one_xcd) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const uint32_t arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t target = (arrived / g.n + 1u) * g.n;
     if (arrived + 1u != target) {
@@ -139,11 +138,10 @@ __device__ __forceinline__ bool grid_sync(const StepGrid& g) {
         __builtin_amdgcn_s_sleep(1);
       }
     }
-    if (!g.one_xcd) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   }
   __syncthreads();
-  if (g.one_xcd) asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");  // this CU's L1
   return s_ok != 0;
 }
 
@@ -346,18 +344,8 @@ __device__ __forceinline__ void run_overflow_levels(const SolvePlan& plan, int w
 // and then colours 0..22 in order; then the restitution iterations. A grid barrier
 // separates dependent phases.
 __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, SolvePlan plan, StepParams p) {
-  // the working workgroups: every workgroup, or those on XCD 0 numbered in arrival order
-  // (the others leave; should XCD 0 hold fewer than n_work, the first barrier times out)
-  __shared__ int s_wg;
-  if (threadIdx.x == 0)
-    s_wg = !plan.one_xcd ? (int)blockIdx.x
-                         : (xcd_id() == 0 ? (int)__hip_atomic_fetch_add(&plan.barrier->ticket, 1u, __ATOMIC_RELAXED,
-                                                                         __HIP_MEMORY_SCOPE_AGENT)
-                                          : -1);
-  __syncthreads();
-  const int wg = s_wg;
-  if (wg < 0 || wg >= plan.n_work) return;
-  const StepGrid grid{plan.barrier, (uint32_t)plan.n_work, wg, plan.one_xcd != 0};
+  const int wg = (int)blockIdx.x;
+  const StepGrid grid{plan.barrier, (uint32_t)plan.n_work, wg};
   const int tid = wg * blockDim.x + threadIdx.x;
   const int nthreads = plan.n_work * blockDim.x;
   const int n_ovf = plan.n_overflow;
@@ -400,8 +388,9 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       // warm starting does not change), velocities are integrated and the overflow colour's
       // levels run; then each body applies its items' changes in colour order with the
       // operations the colour-by-colour walk applies to it: the same bits in two phases.
-      // (Computing the changes at the end of each relax item instead measured slower: the
-      // relax phases grew more than this phase shrank.)
+      // (Computing the changes at the end of each relax item, or on the lanes a relax phase
+      // leaves without an item, measured slower: the relax phases grew more than this phase
+      // shrank.)
       // Velocities are integrated here too (it writes only v and w, which this phase does not
       // otherwise read): workgroup 0 integrates a body of the overflow colour just before its
       // first overflow item in this substep, every lane integrates the other bodies.
@@ -556,7 +545,6 @@ __global__ __launch_bounds__(kBlock) void k_prepare(DeviceWorld d, SolvePlan pla
     // the step kernel's barrier counts its arrivals from zero (stream order: the previous
     // step kernel has finished, the next starts after this kernel)
     __hip_atomic_store(&plan.barrier->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&plan.barrier->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (plan.ws_zero != nullptr) *plan.ws_zero = WarmDelta{zero3(), zero3()};
   }
   for (int k = tid; k < plan.n_joints; k += nthreads) {
@@ -667,9 +655,7 @@ void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const St
 
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, bool cooperative, hipStream_t s,
                        KernelTimer* t) {
-  // plan.n_work working workgroups; on one XCD, kXcdCount times as many are launched so that
-  // n_work of them land on XCD 0
-  const int blocks = plan.one_xcd ? plan.n_work * kXcdCount : plan.n_work;
+  const int blocks = plan.n_work;
   // A cooperative launch: the runtime starts the grid only when every workgroup can be
   // resident at once, which the grid barriers need (`blocks` is clamped by the caller to the
   // occupancy limit); the barriers' bounded wait still turns a stalled grid into an error.

@@ -63,9 +63,6 @@ struct WorldOptions {
   int32_t stage_records = 0;
   // 1: the step kernel is launched without the cooperative API (same grid and results)
   int32_t plain_step_launch = 0;
-  // 1: the step kernel's workgroups all run on one XCD (at most its CUs) and share its L2
-  // (same results)
-  int32_t step_one_xcd = 0;
 };
 
 

@@ -92,7 +92,7 @@ class GpuWorld : public HostScene {
   struct Snapshot;
   Snapshot* save_state();  // owned by the caller; free with destroy_snapshot
   struct Tuning {
-    int blocks = 0, small_colour_items = 0, stage_records = 0, one_xcd = 0, candidates = 0;
+    int blocks = 0, small_colour_items = 0, stage_records = 0, candidates = 0;
     double best_us = 0.0, previous_us = 0.0;
   };
   // Times candidate launch shapes of the step kernel from a snapshot and keeps the fastest

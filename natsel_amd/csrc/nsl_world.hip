@@ -678,36 +678,25 @@ GpuWorld::Tuning GpuWorld::autotune(float dt, int32_t sub_steps, int32_t steps) 
     t.blocks = d0;
     t.small_colour_items = saved.small_colour_items;
     t.stage_records = saved.stage_records;
-    t.one_xcd = saved.step_one_xcd;
-    // on one XCD at most its share of the device's workgroups
-    std::vector<int> sizes_xcd;
-    for (int b : sizes) {
-      const int c = std::max(1, std::min(b, max_blocks_ / kXcdCount));
-      if (std::find(sizes_xcd.begin(), sizes_xcd.end(), c) == sizes_xcd.end()) sizes_xcd.push_back(c);
-    }
-    for (int one_xcd : {0, 1})
-      for (int b : one_xcd ? sizes_xcd : sizes) {
-        for (int small : {0, 128, 256})
-          for (int stage : {0, 1}) {
-            WorldOptions o = saved;
-            o.solver_blocks = b;
-            o.small_colour_items = small;
-            o.stage_records = stage;
-            o.step_one_xcd = one_xcd;
-            uint64_t h = 0;
-            int used = 0;
-            const double us = trial(o, &h, &used);
-            t.candidates += 1;
-            if (h != h0) throw std::runtime_error("autotune: a launch shape changed the results (it must not)");
-            if (us < t.best_us) {
-              t.best_us = us;
-              t.blocks = b;
-              t.small_colour_items = small;
-              t.stage_records = stage;
-              t.one_xcd = one_xcd;
-            }
+    for (int b : sizes)
+      for (int small : {0, 128, 256})
+        for (int stage : {0, 1}) {
+          WorldOptions o = saved;
+          o.solver_blocks = b;
+          o.small_colour_items = small;
+          o.stage_records = stage;
+          uint64_t h = 0;
+          int used = 0;
+          const double us = trial(o, &h, &used);
+          t.candidates += 1;
+          if (h != h0) throw std::runtime_error("autotune: a launch shape changed the results (it must not)");
+          if (us < t.best_us) {
+            t.best_us = us;
+            t.blocks = b;
+            t.small_colour_items = small;
+            t.stage_records = stage;
           }
-      }
+        }
   } catch (...) {
     options = saved;
     restore_state(*snap);
@@ -718,7 +707,6 @@ GpuWorld::Tuning GpuWorld::autotune(float dt, int32_t sub_steps, int32_t steps) 
   options.solver_blocks = t.blocks;
   options.small_colour_items = t.small_colour_items;
   options.stage_records = t.stage_records;
-  options.step_one_xcd = t.one_xcd;
   restore_state(*snap);
   set_kernel_timing(was_timing, was_phase);
   reset_kernel_stats();
@@ -1049,10 +1037,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   // more workgroups add grid-barrier arrivals faster than they add useful lanes: on the
   // settled 132k-body world 96 workgroups of 128 lanes beat 48..80 and 112..256
   blocks = std::max(1, std::min(blocks, std::min(max_blocks_, 96)));
-  // on one XCD the grid is kXcdCount times wider and only XCD 0's workgroups work
-  if (options.step_one_xcd) blocks = std::max(1, std::min(blocks, max_blocks_ / kXcdCount));
   plan.n_work = blocks;
-  plan.one_xcd = options.step_one_xcd;
   // taken after every buffer above is in place: the kernels get the pointers of this step
   DeviceWorld dw = d_;
   if (!options.enable_sleep) dw.sleep_times = nullptr;

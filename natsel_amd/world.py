@@ -97,7 +97,6 @@ class WorldOptions:  # world_runtime.rs:1243-1304
     small_colour_items: int = 128  # colours this small run on one workgroup (0: off); results do not change
     stage_records: bool = False  # the step kernel copies next-colour constraint records to LDS; results do not change
     plain_step_launch: bool = False  # an ordinary (not cooperative) launch of the same step-kernel grid
-    step_one_xcd: bool = False  # the step kernel's workgroups all on one XCD (shared L2); results do not change
 
 
 def _material(d: ShapeDef) -> capi.Material:
@@ -139,7 +138,6 @@ def c_options(o: WorldOptions) -> capi.Options:
     c.small_colour_items = int(o.small_colour_items)
     c.stage_records = int(o.stage_records)
     c.plain_step_launch = int(o.plain_step_launch)
-    c.step_one_xcd = int(o.step_one_xcd)
     return c
 
 
@@ -334,7 +332,6 @@ class World:
         self._opts.solver_blocks = t.solver_blocks
         self._opts.small_colour_items = t.small_colour_items
         self._opts.stage_records = bool(t.stage_records)
-        self._opts.step_one_xcd = bool(t.step_one_xcd)
         return {n: getattr(t, n) for n, _ in capi.Tuning._fields_}
 
     def reset_kernel_stats(self) -> None:
@@ -372,10 +369,10 @@ def shape_def_of(s) -> ShapeDef:
 
 
 def load_scene(scene: Scene, device: int = 0, solver_blocks: int = 0, small_colour_items: int = 128,
-               plain_step_launch: bool = False, step_one_xcd: bool = False, stage_records: bool = False) -> World:
+               plain_step_launch: bool = False, stage_records: bool = False) -> World:
     """Build a World from a seeded scene description (natsel_amd.scenes)."""
     opts = WorldOptions(enable_sleep=scene.enable_sleep, enable_continuous=scene.enable_continuous, solver_blocks=solver_blocks,
-                        small_colour_items=small_colour_items, plain_step_launch=plain_step_launch, step_one_xcd=step_one_xcd,
+                        small_colour_items=small_colour_items, plain_step_launch=plain_step_launch,
                         stage_records=stage_records,
                         gravity=tuple(scene.gravity), restitution_iterations=scene.restitution_iterations,
                         restitution_threshold=scene.restitution_threshold,

@@ -43,28 +43,3 @@ def test_plain_step_launch_matches_oracle(gpu_available):
         if k % 20 == 19:
             assert g.world_state_hash() == o.world_state_hash(), f"step {k}"
 
-
-@pytest.mark.parametrize("scene_name", ["pile", "chain_pile", "overflow_hub"])
-@pytest.mark.parametrize("plain", [False, True])
-def test_one_xcd_step_kernel_matches_oracle(gpu_available, scene_name, plain):
-    """The step kernel on one XCD (step_one_xcd: workgroups on XCD 0 only, grid barriers on
-    that XCD's L2 with an L1 invalidate) ends every step on the oracle's hash: a mixed pile,
-    a pile with revolute chains (joint rows in the colours) and the overflow hub (the
-    overflow colour's levels on the first working workgroup)."""
-    from natsel_amd.world import load_scene
-    from oracle import pyoracle
-
-    if scene_name == "pile":
-        sc, steps = scenes.pile(1500, seed=13, size_x=10, size_y=20, size_z=10), 120
-    elif scene_name == "chain_pile":
-        sc, steps = scenes.chain_pile(200, chains=20), 120
-    else:
-        sc, steps = scenes.overflow_hub(enable_sleep=False), 60
-    print(f"one_xcd {scene_name} plain={plain}", flush=True)
-    g, o = load_scene(sc, step_one_xcd=True, plain_step_launch=plain), pyoracle.load_scene(sc)
-    assert g.options.step_one_xcd
-    for k in range(steps):
-        g.step(sc.dt, sc.substeps)
-        o.step(sc.dt, sc.substeps)
-        assert g.world_state_hash() == o.world_state_hash(), f"step {k}"
-    assert g.kernel_stats()["solver_blocks"] <= 32
