@@ -386,6 +386,7 @@ def adjoint_measurement(worlds: int, steps: int, fd_worlds: int = 16) -> dict:
         check[mode] = {"max_rel_error": max(errs), "median_rel_error": float(np.median(errs)),
                        "components_within_1e-3": float(np.concatenate(within).mean())}
     rb.close()
+    check["cpu_reference"] = adjoint_cpu_reference(sc, targets, sample, steps, r, offs)
     fs, fas = fwd_ms / 1e3, (rec_ms + adj_ms) / 1e3
     return {
         "metric": "world-steps/s (differentiable rollout: forward vs forward+adjoint)",
@@ -399,6 +400,70 @@ def adjoint_measurement(worlds: int, steps: int, fd_worlds: int = 16) -> dict:
         "status_nonzero_worlds": int((r["status"] != 0).sum()),
         "gradient_check": check,
     }
+
+
+def adjoint_cpu_reference(sc: list, targets, sample, steps: int, r: dict, offs) -> dict:
+    """CPU baseline and check of the differentiable rollout: the oracle's reference
+    (oracle/ref_adjoint.cpp: each sampled world recorded on the oracle, the model repeated in
+    float and differentiated in double by reverse-mode automatic differentiation, one host
+    thread) timed on the sampled worlds, its gradients against the GPU adjoint's, and central
+    differences (epsilon 1e-3) of the oracle's own float step (contacts re-detected) for a few
+    velocity components of two worlds against the GPU adjoint."""
+    import dataclasses
+
+    import numpy as np
+
+    from oracle import pyoracle
+
+    errs, exact, t_ref = [], True, 0.0
+    for k in sample:
+        t0 = time.perf_counter()
+        ref = pyoracle.Rollout(sc[k], steps)
+        g = ref.gradient(targets[k])
+        t_ref += time.perf_counter() - t0
+        ref.close()
+        exact = exact and g["exact"] and g["status"] == 0
+        a, b = offs[k], offs[k + 1]
+        gpu = np.concatenate([r["grad_v"][a:b], r["grad_w"][a:b], r["grad_friction"][a:b, None]], axis=1)
+        cpu = np.concatenate([g["grad_v"], g["grad_w"], g["grad_friction"][:, None]], axis=1)
+        errs.append(float(np.abs(gpu - cpu).max() / np.abs(cpu).max()))
+    step_fd = []
+    for k in sample[:2]:
+        s = sc[k]
+        dyn = [i for i, bd in enumerate(s.bodies) if bd.body_type == 2]
+        half = np.array([sh.params[0] for sh in s.shapes if sh.body in dyn])
+        mass = 1000.0 * (2.0 * half) ** 3
+
+        def loss_of(scene):
+            w = pyoracle.load_scene(scene)
+            for _ in range(steps):
+                w.step(scene.dt, scene.substeps)
+            pos = w.body_arrays()[0][dyn]
+            w.close()
+            d = (pos * mass[:, None]).sum(0) / mass.sum() - np.asarray(targets[k], np.float64)
+            return float(d @ d)
+
+        def perturbed(bi, c, e):
+            bodies = list(s.bodies)
+            v = list(bodies[dyn[bi]].linear_velocity)
+            v[c] = float(np.float32(v[c] + e))
+            bodies[dyn[bi]] = dataclasses.replace(bodies[dyn[bi]], linear_velocity=tuple(v))
+            return dataclasses.replace(s, bodies=bodies)
+
+        gv = r["grad_v"][offs[k]:offs[k + 1]]
+        scale = float(np.abs(gv).max())
+        for bi in (0, 11, 27):
+            for c in (0, 2):
+                fd = (loss_of(perturbed(bi, c, 1e-3)) - loss_of(perturbed(bi, c, -1e-3))) / 2e-3
+                step_fd.append(abs(float(gv[bi][c]) - fd) / scale)
+    return {"value": len(sample) * steps / t_ref, "unit": "world-steps/s (recording + reference gradient)", "cores": 1,
+            "kind": "port", "sample": f"{len(sample)} sampled worlds x {steps} steps, one host thread",
+            "model_repeats_recorded_steps_bit_for_bit": exact,
+            "gpu_vs_cpu_reference_max_rel_error": max(errs), "gpu_vs_cpu_reference_median_rel_error": float(np.median(errs)),
+            "gpu_vs_oracle_step_fd": {"epsilon": 1e-3, "components": len(step_fd), "max_rel_error": max(step_fd),
+                                      "median_rel_error": float(np.median(step_fd)),
+                                      "note": "error / the world's largest velocity gradient; the step re-detects contacts "
+                                              "(the model holds them fixed), in float"}}
 
 
 def summarize_profile(path: str, kernel: str, window: str) -> dict:

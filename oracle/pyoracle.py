@@ -73,6 +73,15 @@ def lib():
         L.nslo_first_island_contact.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
         L.nslo_unlink_and_split.argtypes = [P, I, I]
         L.nslo_check_islands.argtypes = [P]
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.nslo_rollout_record.argtypes = [P, F, I, I]
+        L.nslo_rollout_record.restype = P
+        L.nslo_rollout_destroy.argtypes = [P]
+        L.nslo_rollout_bodies.argtypes = [P]
+        L.nslo_rollout_status.argtypes = [P]
+        L.nslo_rollout_gradient.argtypes = [P, fp, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.nslo_rollout_loss.argtypes = [P, fp, I, ctypes.c_double, I]
+        L.nslo_rollout_loss.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -314,6 +323,53 @@ def load_scene(scene) -> OracleWorld:
     for j in scene.joints:
         w.create_revolute_joint(dataclasses.replace(j, body_a=ids[j.body_a], body_b=ids[j.body_b]))
     return w
+
+
+class Rollout:
+    """The oracle's reference for the differentiable rollout (ref_adjoint.cpp): the scene
+    stepped `steps` steps on the oracle with its tape recorded, the frozen-active-set model
+    run in float (which must repeat every recorded step bit for bit) and in double under
+    reverse-mode differentiation. Parameters per awake body i: 7 i + {0..2} linear velocity,
+    {3..5} angular velocity, 6 friction."""
+
+    def __init__(self, scene, steps: int):
+        w = load_scene(scene)
+        self.L = w.L
+        self.h = self.L.nslo_rollout_record(w.h, float(scene.dt), int(scene.substeps), int(steps))
+        w.close()
+        self.bodies = int(self.L.nslo_rollout_bodies(self.h))
+        self.status = int(self.L.nslo_rollout_status(self.h))
+
+    def close(self):
+        if self.h:
+            self.L.nslo_rollout_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gradient(self, target) -> dict:
+        """Loss |com - target|^2 and its gradient with respect to every awake body's initial
+        linear and angular velocity and friction; `exact`: the float model repeated every
+        recorded step bit for bit."""
+        t = np.ascontiguousarray(target, np.float32)
+        n = self.bodies
+        gv, gw, gf = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        loss, exact = ctypes.c_double(), ctypes.c_int()
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+        status = self.L.nslo_rollout_gradient(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss),
+                                              dp(gv), dp(gw), dp(gf), ctypes.byref(exact))
+        return {"loss": loss.value, "grad_v": gv, "grad_w": gw, "grad_friction": gf, "exact": bool(exact.value),
+                "status": int(status)}
+
+    def loss(self, target, param: int = -1, delta: float = 0.0, frozen: bool = True) -> float:
+        """The model's loss in double precision with parameter `param` moved by `delta`."""
+        t = np.ascontiguousarray(target, np.float32)
+        return float(self.L.nslo_rollout_loss(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(param),
+                                              float(delta), int(frozen)))
 
 
 # --- narrow phase -------------------------------------------------------------------------

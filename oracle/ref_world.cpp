@@ -1004,6 +1004,7 @@ void World::step(float time_step, int sub_step_count) {
     }
     prepare_joints(ctx);
     prepare_contacts(ctx);
+    if (after_prepare) after_prepare(*this, ctx);
     double t3 = omp_get_wtime();
     solve_stages(ctx);
     double t4 = omp_get_wtime();

@@ -16,6 +16,7 @@
 // revolute joint are not restated: the pinned scenes run with them inert.
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <unordered_set>
 #include <vector>
@@ -418,6 +419,9 @@ class World {
   bool enable_sleep = true, enable_warm_starting = true, enable_continuous = true, enable_speculative = true;
 
   StepStats stats;
+  // called after each step's prepare, before the solver stages (the rollout recorder of
+  // ref_adjoint.cpp); not part of the reference
+  std::function<void(World&, const StepContext&)> after_prepare;
 
   // --- state (public so the checker can inspect intermediate results) ---
   std::vector<Body> bodies;
