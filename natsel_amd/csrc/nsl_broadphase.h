@@ -3,9 +3,11 @@
 // The reference queries a dynamic AABB tree with each moved proxy; the resulting pair set
 // does not depend on the tree: every pair of shapes whose fat AABBs overlap, with at least
 // one proxy moved, at least one dynamic body, different bodies, a passing collision filter
-// (shape_runtime.rs:285-292) and no existing contact. Here that set is found with a sort
-// along x and a load-balanced sweep over the x-overlapping pairs that hold a moved proxy,
-// and returned as sorted pair keys, which is the order the reference creates contacts in.
+// (shape_runtime.rs:285-292) and no existing contact. It is returned as sorted pair keys,
+// which is the order the reference creates contacts in. Here that set is found on a uniform
+// grid (nsl_broadphase.hip): each proxy in the cells its fat AABB spans, pairs tested within
+// a cell and reported in one cell only, proxies larger than two cells per axis tested against
+// every proxy.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -35,8 +37,10 @@ class PairFinder {
   PairFinder(const PairFinder&) = delete;
   PairFinder& operator=(const PairFinder&) = delete;
 
-  // New pair keys in ascending order. Clears the moved flags and the moved counter.
-  // Synchronises the stream (once when there is no new pair, twice otherwise).
+  // New pair keys in ascending order. Clears the moved flags and the moved counter. Few
+  // moved proxies are tested directly against every proxy, more through the grid; the keys
+  // do not depend on the path. Synchronises the stream two or three times, and once more
+  // when there are new pairs.
   void find(const DeviceWorld& d, const Proxy* proxies, int n_shapes, std::vector<uint64_t>& keys);
 
   // Candidate pairs tested by the last find (for profiling).
@@ -46,18 +50,16 @@ class PairFinder {
   template <class T> void ensure(T*& ptr, size_t& cap, size_t n);
 
   hipStream_t stream_;
-  uint32_t* keys_in_ = nullptr;
-  uint32_t* keys_out_ = nullptr;
-  int32_t* ids_in_ = nullptr;
-  int32_t* ids_out_ = nullptr;
-  int64_t* counts_ = nullptr;
-  int64_t* offsets_ = nullptr;
-  int32_t* mv_ = nullptr;         // per sorted position: moved (0/1)
-  int32_t* mprefix_ = nullptr;    // exclusive sum of mv_
-  int32_t* moved_pos_ = nullptr;  // the moved positions in order
-  size_t cap_mv_ = 0, cap_mprefix_ = 0, cap_moved_pos_ = 0;
-  size_t cap_keys_in_ = 0, cap_keys_out_ = 0, cap_ids_in_ = 0, cap_ids_out_ = 0, cap_counts_ = 0,
-         cap_offsets_ = 0;
+  int32_t* counts_ = nullptr;      // grid entries per proxy
+  int32_t* offsets_ = nullptr;     // exclusive sum of counts_
+  int32_t* large_ = nullptr;       // proxies tested against every proxy (or the moved ones, on the direct path)
+  void* grid_ = nullptr;           // cell size and the extent statistics it comes from
+  uint32_t* ekeys_in_ = nullptr;   // grid entries: cell hash
+  uint32_t* ekeys_out_ = nullptr;
+  uint32_t* evals_in_ = nullptr;   // grid entries: (proxy << 3) | corner offset
+  uint32_t* evals_out_ = nullptr;
+  size_t cap_counts_ = 0, cap_offsets_ = 0, cap_large_ = 0, cap_ekeys_in_ = 0, cap_ekeys_out_ = 0, cap_evals_in_ = 0,
+         cap_evals_out_ = 0;
   uint64_t* pairs_ = nullptr;        // found keys, unsorted
   uint64_t* pairs_sorted_ = nullptr;
   size_t cap_pairs_ = 0, cap_pairs_sorted_ = 0;
@@ -65,6 +67,7 @@ class PairFinder {
   uint8_t* temp_ = nullptr;
   size_t cap_temp_ = 0;
   int64_t* host_ = nullptr;          // pinned: found_ copy
+  int32_t* host_entries_ = nullptr;  // pinned: the moved count, then the grid's entry count
   uint64_t* host_keys_ = nullptr;    // pinned
   size_t cap_host_keys_ = 0;
 };
