@@ -29,6 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRIAL_STEPS = 5         # steps per launch-shape trial of the pile's step kernel
+BATCH_TRIAL_STEPS = 60  # steps per launch-shape trial of the batch
+BATCH_GRAPH_STEPS = 8   # steps of one captured batch graph (kBatchGraphSteps)
 
 # Algorithmic bytes per item: distinct fields the reference formula reads plus writes,
 # f32 = 4 B, each body-state gather counted per constraint touch. Coloured constraint
@@ -182,7 +185,9 @@ def batched_measurement(worlds: int, steps: int, autotune: bool = True) -> dict:
     dt, sub = scene_list[0].dt, scene_list[0].substeps
     b = load_batch(scene_list)
     b.step(dt, sub, 1)  # uploads the worlds
-    tuning = b.autotune(dt, sub, 60) if autotune else None
+    tuning = b.autotune(dt, sub, BATCH_TRIAL_STEPS) if autotune else None
+    # each candidate runs a warm (capture) call and a timed call from the snapshot
+    trial_steps = tuning["candidates"] * (min(BATCH_TRIAL_STEPS, BATCH_GRAPH_STEPS) + BATCH_TRIAL_STEPS) if tuning else 0
     s0 = b.stats()
     b.step(dt, sub, steps)
     s1 = b.stats()
@@ -213,6 +218,9 @@ def batched_measurement(worlds: int, steps: int, autotune: bool = True) -> dict:
         "points_per_step": counts["points"] / steps,
         "error_worlds": s1["error_worlds"],
         "autotune": tuning,
+        "dispatch_window": [2 + trial_steps, 1 + trial_steps + steps],
+        "dispatch_window_note": "each batch kernel runs once per step: one untimed step, the launch-shape trials' "
+                                f"{trial_steps} steps, then the timed steps (this process's first batch)",
     }
 
 
@@ -580,7 +588,9 @@ def main() -> int:
     settle_s = time.perf_counter() - s0
     # the step kernel's launch shape for the settled pile, by timing (results do not change;
     # the world is restored to the settled state)
-    tuning = world.autotune(scene.dt, scene.substeps, 5) if args.autotune else None
+    tuning = world.autotune(scene.dt, scene.substeps, TRIAL_STEPS) if args.autotune else None
+    # every trial (the previous shape, then each candidate) runs TRIAL_STEPS steps
+    trial_steps = (tuning["candidates"] + 1) * TRIAL_STEPS if tuning else 0
     hashes = {"settle": world.world_state_hash()}
     _progress(f"pile: {settle} untimed steps in {settle_s:.2f} s")
     snap = world.save_state()
@@ -665,9 +675,11 @@ def main() -> int:
         "pair_tests_per_step": ks["pair_tests"] / max(ks["steps"], 1),
         "fast_bodies": ks["fast_bodies"],
         "timed_window": {"first_step": settle + 1, "last_step": settle + args.steps, "untimed_steps_before": settle,
-                         "step_kernel_dispatches": [settle + 1, settle + args.steps],
-                         "note": "the step kernel runs once per step, so the timed window is its dispatches "
-                                 "first_step..last_step of this process (rocprofv3 summaries average over those)"},
+                         "step_kernel_dispatches": [settle + trial_steps + 1, settle + trial_steps + args.steps],
+                         "note": "the step kernel, k_collide, k_prepare and k_store_finalize run once per step; after "
+                                 "the untimed steps the launch-shape trials run " + str(trial_steps) + " steps, so the "
+                                 "timed window is each kernel's dispatches step_kernel_dispatches of this process "
+                                 "(rocprofv3 summaries average over those)"},
         "full_schedule": {"steps": settle + args.steps, "seconds": settle_s + elapsed,
                           "value": n * scene.substeps * (settle + args.steps) / (settle_s + elapsed),
                           "note": "the whole run from the drop (first step includes the upload), untimed steps included"},
