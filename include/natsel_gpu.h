@@ -355,6 +355,25 @@ typedef struct nsl_tuning {
    and keeps the fastest shape in the world's options. No reference counterpart: the
    reference sizes its stage blocks by fixed rules (solver_stages.rs:490-525). */
 int nsl_world_autotune(nsl_world* world, float dt, int32_t sub_steps, int32_t steps, nsl_tuning* out);
+/* Per-colour residuals of the coloured contact solver. No reference counterpart: a
+   convergence diagnostic of the graph-coloured relax stage (contact_solver.rs:927-1097).
+   While on, each step ends with one more relax sweep over the coloured contact constraints,
+   colour by colour in the solver's order, from the state the step's solver ends in (after
+   the last relax and restitution), on a copy of the body states: the world's state and the
+   step's results are unchanged. Each wavefront reduces its constraints' impulse changes with
+   lane shuffles; per colour: the sum over points of |normal impulse change| (wavefront
+   partials added in slot order in double), the largest such change, the largest length of
+   a friction impulse change, and the number of constraints. Joints and the overflow colour
+   are not part of the sweep. The results are the last step's; zeros before the first step
+   with residuals on. */
+#define NSL_GRAPH_COLOURS 23 /* colours 0..22; 23 is the overflow colour (constraint_graph.rs:20) */
+typedef struct nsl_colour_residual {
+  double sum_normal;
+  float max_normal, max_friction;
+  int32_t constraints, reserved;
+} nsl_colour_residual;
+int nsl_world_set_residuals(nsl_world* world, int32_t on);
+int nsl_world_get_residuals(nsl_world* world, nsl_colour_residual* out, int32_t n);
 int nsl_world_reset_kernel_stats(nsl_world* world);
 int nsl_world_get_kernel_stats(nsl_world* world, nsl_kernel_stats* out);
 

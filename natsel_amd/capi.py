@@ -177,6 +177,14 @@ class Tuning(ctypes.Structure):
                 ("step_kernel_us", ctypes.c_double), ("previous_us", ctypes.c_double), ("candidates", ctypes.c_int32)]
 
 
+class ColourResidual(ctypes.Structure):
+    _fields_ = [("sum_normal", ctypes.c_double), ("max_normal", ctypes.c_float), ("max_friction", ctypes.c_float),
+                ("constraints", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+GRAPH_COLOURS = 23  # NSL_GRAPH_COLOURS: colours 0..22 (23 is the overflow colour)
+
+
 class BatchTuning(ctypes.Structure):
     _fields_ = [("collide_blocks", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("candidates", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("step_ms", ctypes.c_double), ("previous_ms", ctypes.c_double)]
@@ -244,6 +252,8 @@ SIGNATURES = {
     "nsl_world_get_profile": (ctypes.c_int, [_P, ctypes.POINTER(Profile)]),
     "nsl_world_set_kernel_timing": (ctypes.c_int, [_P, _I]),
     "nsl_world_autotune": (ctypes.c_int, [_P, _F, _I, _I, ctypes.POINTER(Tuning)]),
+    "nsl_world_set_residuals": (ctypes.c_int, [_P, _I]),
+    "nsl_world_get_residuals": (ctypes.c_int, [_P, ctypes.POINTER(ColourResidual), _I]),
     "nsl_world_reset_kernel_stats": (ctypes.c_int, [_P]),
     "nsl_world_get_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStats)]),
     "nsl_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Options), ctypes.POINTER(_P)]),

@@ -609,7 +609,7 @@ __global__ __launch_bounds__(kBatchLanes) __attribute__((amdgpu_waves_per_eu(2))
   for (int i = lane; i < kWsWords; i += kBatchLanes)
     reinterpret_cast<int32_t*>(&s_ws)[i] = reinterpret_cast<const int32_t*>(r.ws)[i];
   const int bit_words = kOverflowColor * r.bits_words;
-  const bool bits_in_lds = bit_words <= (int)(sizeof(s_states) / sizeof(uint32_t));
+  const bool bits_in_lds = bit_words <= (int)(sizeof(s_states) / (sizeof(uint32_t)));
   uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_states);
   if (bits_in_lds)
     for (int i = lane; i < bit_words; i += kBatchLanes) s_bits[i] = r.bits[i];

@@ -483,6 +483,26 @@ int nsl_world_autotune(nsl_world* world, float dt, int32_t sub_steps, int32_t st
   });
 }
 
+int nsl_world_set_residuals(nsl_world* world, int32_t on) {
+  if (!world) return fail(NSL_ERR_INVALID, "null world");
+  return guarded([&] { world->impl->set_residuals(on != 0); });
+}
+
+int nsl_world_get_residuals(nsl_world* world, nsl_colour_residual* out, int32_t n) {
+  if (!world || !out) return fail(NSL_ERR_INVALID, "null argument");
+  if (n < 0 || n > NSL_GRAPH_COLOURS) return fail(NSL_ERR_INVALID, "more colours than the graph has");
+  return guarded([&] {
+    const nsl::GpuWorld::ColourResidual* r = world->impl->residuals();
+    for (int32_t i = 0; i < n; ++i) {
+      out[i].sum_normal = r[i].sum_normal;
+      out[i].max_normal = r[i].max_normal;
+      out[i].max_friction = r[i].max_friction;
+      out[i].constraints = r[i].constraints;
+      out[i].reserved = 0;
+    }
+  });
+}
+
 int nsl_world_reset_kernel_stats(nsl_world* world) {
   if (!world) return fail(NSL_ERR_INVALID, "null world");
   return guarded([&] { world->impl->reset_kernel_stats(); });

@@ -160,6 +160,15 @@ void launch_prepare(const DeviceWorld& d, const SolvePlan& plan, const StepParam
 // One launch (all workgroups resident): the substep schedule and restitution.
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, bool cooperative, hipStream_t s,
                        KernelTimer* timer);
+// One wavefront's part of a colour's residual (k_colour_residual): the sum and the largest
+// of its points' normal impulse changes and the largest friction impulse change.
+struct ResidualPartial {
+  float sum_normal, max_normal, max_friction, pad;
+};
+// The residual sweep of one colour's coloured slots [begin, begin + count) on the body states
+// of `d` (a copy): one partial per 64 slots, in slot order, to `out`.
+void launch_colour_residual(const DeviceWorld& d, int begin, int count, const StepParams& p, ResidualPartial* out,
+                            hipStream_t s);
 // Store of contact and overflow impulses and finalize of the bodies, one lane per item (after
 // the step kernel; finalize lists the fast bodies for the continuous pass).
 void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, hipStream_t s,

@@ -598,6 +598,26 @@ __global__ __launch_bounds__(kBlock) void k_store_finalize(DeviceWorld d, SolveP
   for (int i = tid; i < d.n_awake; i += nthreads) finalize_item(d, i, p);
 }
 
+// The residual sweep of one colour: one lane per coloured slot of [begin, begin + count),
+// `d.states` a copy of the body states. Each wavefront reduces its lanes' residuals with
+// lane shuffles (sum by the butterfly 32, 16, 8, 4, 2, 1; maxima) and lane 0 writes the
+// wavefront's partial (ceil(count / 64) of them); lanes past the colour contribute zeros.
+__global__ __launch_bounds__(kBlock) void k_colour_residual(DeviceWorld d, int begin, int count, StepParams p,
+                                                            ResidualPartial* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  Residual r;
+  if (i < count) r = residual_item(d, begin + i, p);
+  float sum = r.sum_normal, mx = r.max_normal, fr = r.friction;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    sum += __shfl_xor(sum, off, 64);
+    mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    fr = fmaxf(fr, __shfl_xor(fr, off, 64));
+  }
+  // lane 0's index is its wavefront's first: only wavefronts that hold a slot write
+  if ((threadIdx.x & 63) == 0 && i < count) out[i >> 6] = ResidualPartial{sum, mx, fr, 0.0f};
+}
+
 inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
 
 // Launch wrapped in a timer mark when timing is on.
@@ -651,6 +671,12 @@ void launch_prepare(const DeviceWorld& d, const SolvePlan& plan, const StepParam
 void launch_store_finalize(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, hipStream_t s, KernelTimer* t) {
   const int n = std::max(std::max(plan.n_constraints, d.n_awake), std::max(plan.n_overflow, 1));
   NSL_TIMED(t, kKFinalize, s, hipLaunchKernelGGL(k_store_finalize, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, plan, p));
+}
+
+void launch_colour_residual(const DeviceWorld& d, int begin, int count, const StepParams& p, ResidualPartial* out,
+                            hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_colour_residual, dim3(blocks_for(count)), dim3(kBlock), 0, s, d, begin, count, p, out);
 }
 
 void launch_solve_step(const DeviceWorld& d, const SolvePlan& plan, const StepParams& p, bool cooperative, hipStream_t s,

@@ -873,6 +873,45 @@ __device__ __forceinline__ void relax_item(const DeviceWorld& d, int slot, const
   store_state(d, ib, bb);
 }
 
+// The residual sweep's output: relax_core's impulse writes are not kept; each is compared
+// with the record's impulse instead. Per constraint: the sum and the largest |change| of
+// its points' normal impulses and the length of its friction impulse's change.
+struct ResidualOut {
+  const Constraint* c;
+  float* sum_normal;
+  float* max_normal;
+  float* friction;
+  __device__ __forceinline__ void normal_impulse(int i, float v) const {
+    const float r = fabsf(v - c->points[i].normal_impulse);
+    *sum_normal += r;
+    *max_normal = fmaxf(*max_normal, r);
+  }
+  __device__ __forceinline__ void total_normal_impulse(int, float) const {}
+  __device__ __forceinline__ void twist_impulse(float) const {}
+  __device__ __forceinline__ void friction_impulse(Vec2 v) const {
+    const float dx = v.x - c->friction_impulse.x, dy = v.y - c->friction_impulse.y;
+    *friction = sqrtf_(dx * dx + dy * dy);
+  }
+  __device__ __forceinline__ void rolling_impulse(Vec3) const {}
+};
+
+struct Residual {
+  float sum_normal = 0.0f, max_normal = 0.0f, friction = 0.0f;
+};
+
+// One constraint of the residual sweep: relax (contact_solver.rs:927-1097) from the record's
+// final impulses on the body states of `d` (a copy), which it updates; the record is not
+// written.
+__device__ __forceinline__ Residual residual_item(const DeviceWorld& d, int slot, const StepParams& p) {
+  const Constraint c = blocked_cons(d).load(slot);
+  BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
+  Residual r;
+  relax_core(c, ResidualOut{&c, &r.sum_normal, &r.max_normal, &r.friction}, p, ba, bb);
+  store_state(d, c.index_a, ba);
+  store_state(d, c.index_b, bb);
+  return r;
+}
+
 // The three solve stages on a record the caller already holds (a copy of slot's record,
 // loaded early); impulses are written to the slot's record.
 __device__ __forceinline__ void warm_start_record(const DeviceWorld& d, const Constraint& c) {

@@ -85,6 +85,20 @@ class GpuWorld : public HostScene {
   const KernelStats& kernel_stats() const { return kstats_; }
   void reset_kernel_stats();
 
+  // Per-colour residuals of the coloured contact solver: while on, each step ends with one
+  // more relax sweep over the coloured contact constraints, colour by colour, from the state
+  // the solver ends in, on a copy of the body states (the world's state is unchanged). Per
+  // colour: the sum over points of |normal impulse change| (wavefront partials added in slot
+  // order in double), the largest such change, the largest |friction impulse change| and
+  // the number of constraints. No reference counterpart: a convergence diagnostic.
+  struct ColourResidual {
+    double sum_normal = 0.0;
+    float max_normal = 0.0f, max_friction = 0.0f;
+    int32_t constraints = 0, reserved = 0;
+  };
+  void set_residuals(bool on) { residuals_on_ = on; }
+  const ColourResidual* residuals() const { return residuals_; }  // [kOverflowColor], of the last step
+
   // Everything that decides later steps (world_state.rs:340-438 save_state): body states
   // and sims, fat AABBs and moved flags, contacts with their manifolds and caches, the
   // pair-key set, joints, the id pool, the non-touching list, the colour lists and body
@@ -256,6 +270,9 @@ class GpuWorld : public HostScene {
   int32_t* changes_pinned_ = nullptr;  // download staging: [0] count, then the list
   size_t cap_changes_pinned_ = 0, cap_change_list_ = 0, last_changes_ = 0;
   void ensure_pinned_changes(size_t n);
+  // the residual sweep after the step kernel, and its per-colour results after the step's sync
+  void launch_residual_sweep(const DeviceWorld& dw, const ColorRanges& cr, const StepParams& p);
+  void collect_residuals(const ColorRanges& cr);
   std::vector<uint8_t> moved_host_;  // staging for uploads and pulls
   std::vector<Aabb> fat_host_;
   std::vector<Aabb> aabb_host_;
@@ -270,6 +287,15 @@ class GpuWorld : public HostScene {
   int32_t* d_ws_table_ = nullptr;  // warm start by body: the item of each colour per awake body
   WarmDelta* d_ws_delta_ = nullptr;  // and each item's velocity change per side
   int32_t* d_ovf_done_ = nullptr;  // overflow bodies' velocity integration marks
+  // residual sweep: the body-state copy, the wavefront partials (device and pinned) and the
+  // last step's per-colour results
+  bool residuals_on_ = false;
+  BodyState* d_res_states_ = nullptr;
+  ResidualPartial* d_res_partials_ = nullptr;
+  ResidualPartial* res_partials_host_ = nullptr;
+  size_t cap_res_states_ = 0, cap_res_partials_ = 0, cap_res_partials_host_ = 0;
+  int32_t res_wave_begin_[kOverflowColor + 1] = {};
+  ColourResidual residuals_[kOverflowColor] = {};
   size_t cap_ws_table_ = 0, cap_ws_delta_ = 0, cap_ovf_done_ = 0;
   // device pair finder and the pair-key set it checks (new contacts insert, disjoint ones erase)
   std::unique_ptr<PairFinder> finder_;

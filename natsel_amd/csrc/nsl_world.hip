@@ -57,12 +57,14 @@ GpuWorld::~GpuWorld() {
                   d_.counters, d_init_ids_, d_init_records_, d_phase_log_, d_proxies_,
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
                   d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
-                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active, d_ws_table_, d_ws_delta_, d_ovf_done_};
+                  d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active, d_ws_table_, d_ws_delta_, d_ovf_done_,
+                  d_res_states_, d_res_partials_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (changes_pinned_) (void)hipHostFree(changes_pinned_);
   if (readback_) (void)hipHostFree(readback_);
+  if (res_partials_host_) (void)hipHostFree(res_partials_host_);
   for (int i = 0; i < 2 * timer_.capacity; ++i) (void)hipEventDestroy(timer_.events[i]);
   delete[] timer_.events;
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -1043,6 +1045,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   if (!options.enable_sleep) dw.sleep_times = nullptr;
   launch_prepare(dw, plan, p, stream_, tm);
   launch_solve_step(dw, plan, p, options.plain_step_launch == 0, stream_, tm);
+  if (residuals_on_) launch_residual_sweep(dw, cr, p);
   launch_store_finalize(dw, plan, p, stream_, tm);
   // continuous collision of the bodies finalize flagged fast, then of the bullets
   if (options.enable_continuous) {
@@ -1062,6 +1065,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   NSL_CHECK(hipStreamSynchronize(stream_));
   std::memcpy(counters_host_, readback_->counters, sizeof(counters_host_));
   barrier_host_ = readback_->barrier;
+  if (residuals_on_) collect_residuals(cr);
   if (barrier_host_.timed_out) {
     // the step kernel stopped part-way and store/finalize ran on its partial results: every
     // later call on this world fails until a snapshot is restored
@@ -1171,6 +1175,51 @@ void GpuWorld::ensure_pinned_changes(size_t n) {
   }
   cap_changes_pinned_ = std::max<size_t>(2 * n, 4096);
   NSL_CHECK(hipHostMalloc((void**)&changes_pinned_, cap_changes_pinned_ * sizeof(int32_t), hipHostMallocDefault));
+}
+
+void GpuWorld::launch_residual_sweep(const DeviceWorld& dw, const ColorRanges& cr, const StepParams& p) {
+  int waves = 0;
+  for (int ci = 0; ci < kOverflowColor; ++ci) {
+    res_wave_begin_[ci] = waves;
+    waves += (std::max(cr.end[ci] - cr.begin[ci], 0) + 63) / 64;
+  }
+  res_wave_begin_[kOverflowColor] = waves;
+  if (waves == 0) return;
+  ensure(d_res_states_, cap_res_states_, (size_t)std::max(d_.n_awake, 1), false);
+  ensure(d_res_partials_, cap_res_partials_, (size_t)waves, false);
+  if ((size_t)waves > cap_res_partials_host_) {
+    if (res_partials_host_) {
+      NSL_CHECK(hipStreamSynchronize(stream_));
+      NSL_CHECK(hipHostFree(res_partials_host_));
+    }
+    cap_res_partials_host_ = std::max<size_t>(2 * (size_t)waves, 256);
+    NSL_CHECK(hipHostMalloc((void**)&res_partials_host_, cap_res_partials_host_ * sizeof(ResidualPartial),
+                            hipHostMallocDefault));
+  }
+  NSL_CHECK(hipMemcpyAsync(d_res_states_, d_.states, (size_t)d_.n_awake * sizeof(BodyState), hipMemcpyDeviceToDevice,
+                           stream_));
+  DeviceWorld copy = dw;
+  copy.states = d_res_states_;
+  for (int ci = 0; ci < kOverflowColor; ++ci)
+    launch_colour_residual(copy, cr.begin[ci], cr.end[ci] - cr.begin[ci], p, d_res_partials_ + res_wave_begin_[ci],
+                           stream_);
+  NSL_CHECK(hipGetLastError());
+  NSL_CHECK(hipMemcpyAsync(res_partials_host_, d_res_partials_, (size_t)waves * sizeof(ResidualPartial),
+                           hipMemcpyDeviceToHost, stream_));
+}
+
+void GpuWorld::collect_residuals(const ColorRanges& cr) {
+  for (int ci = 0; ci < kOverflowColor; ++ci) {
+    ColourResidual r;
+    r.constraints = std::max(cr.end[ci] - cr.begin[ci], 0);
+    for (int w = res_wave_begin_[ci]; w < res_wave_begin_[ci + 1]; ++w) {
+      const ResidualPartial& q = res_partials_host_[w];
+      r.sum_normal += (double)q.sum_normal;
+      r.max_normal = std::max(r.max_normal, q.max_normal);
+      r.max_friction = std::max(r.max_friction, q.max_friction);
+    }
+    residuals_[ci] = r;
+  }
 }
 
 void GpuWorld::set_kernel_timing(bool on, bool phases) {

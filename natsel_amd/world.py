@@ -334,6 +334,18 @@ class World:
         self._opts.stage_records = bool(t.stage_records)
         return {n: getattr(t, n) for n, _ in capi.Tuning._fields_}
 
+    def set_residuals(self, on: bool) -> None:
+        """Per-colour residual sweep after every step while on (a diagnostic: one more relax
+        sweep over the coloured contacts on a copy of the body states; results unchanged)."""
+        capi.check(self._lib.nsl_world_set_residuals(self._h, 1 if on else 0))
+
+    def colour_residuals(self) -> list:
+        """The last step's residuals per colour 0..22: dicts of sum_normal, max_normal,
+        max_friction (impulse changes of one more relax sweep) and constraints."""
+        arr = (capi.ColourResidual * capi.GRAPH_COLOURS)()
+        capi.check(self._lib.nsl_world_get_residuals(self._h, arr, capi.GRAPH_COLOURS))
+        return [{n: getattr(r, n) for n, _ in capi.ColourResidual._fields_ if n != "reserved"} for r in arr]
+
     def reset_kernel_stats(self) -> None:
         capi.check(self._lib.nsl_world_reset_kernel_stats(self._h))
 

@@ -45,6 +45,8 @@ def lib():
         L.nslo_create_sphere.argtypes = [P, I, fp, F, fp, lp]
         L.nslo_create_capsule.argtypes = [P, I, fp, fp, F, fp, lp]
         L.nslo_step.argtypes = [P, F, I]
+        L.nslo_set_residuals.argtypes = [P, I]
+        L.nslo_get_residuals.argtypes = [P, ctypes.POINTER(ctypes.c_double), fp, fp, ctypes.POINTER(ctypes.c_int32)]
         L.nslo_state_hash.argtypes = [P]
         L.nslo_state_hash.restype = ctypes.c_uint64
         L.nslo_body_capacity.argtypes = [P]
@@ -290,6 +292,17 @@ class OracleWorld:
         ip = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
         self.L.nslo_get_contact_shapes(self.h, ip(a), ip(b))
         return a, b
+
+    def set_residuals(self, on: bool) -> None:
+        self.L.nslo_set_residuals(self.h, 1 if on else 0)
+
+    def colour_residuals(self) -> list:
+        """The last step's residual sweep per colour 0..22 (World::colour_residuals)."""
+        s = (ctypes.c_double * 23)()
+        mn, mf = (ctypes.c_float * 23)(), (ctypes.c_float * 23)()
+        k = (ctypes.c_int32 * 23)()
+        self.L.nslo_get_residuals(self.h, s, mn, mf, k)
+        return [{"sum_normal": s[i], "max_normal": mn[i], "max_friction": mf[i], "constraints": k[i]} for i in range(23)]
 
     def contact_colors(self):
         n = self.L.nslo_contact_capacity(self.h)

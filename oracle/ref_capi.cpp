@@ -128,6 +128,20 @@ void* nslo_save_state(void* wp) { return new World(*static_cast<World*>(wp)); }
 void nslo_restore_state(void* wp, void* sp) { *static_cast<World*>(wp) = *static_cast<World*>(sp); }
 
 void nslo_step(void* wp, float dt, int substeps) { static_cast<World*>(wp)->step(dt, substeps); }
+
+// The per-colour residual sweep (World::colour_residuals) after every step while on; the
+// last step's results per colour 0..22: sum (double), largest normal and friction changes,
+// constraints.
+void nslo_set_residuals(void* wp, int on) { static_cast<World*>(wp)->residuals_on = on != 0; }
+void nslo_get_residuals(void* wp, double* sum, float* max_normal, float* max_friction, int32_t* constraints) {
+  const World* w = static_cast<World*>(wp);
+  for (int ci = 0; ci < 23; ++ci) {
+    sum[ci] = w->residuals[ci].sum_normal;
+    max_normal[ci] = w->residuals[ci].max_normal;
+    max_friction[ci] = w->residuals[ci].max_friction;
+    constraints[ci] = w->residuals[ci].constraints;
+  }
+}
 uint64_t nslo_state_hash(void* wp) { return static_cast<World*>(wp)->state_hash(); }
 int nslo_body_capacity(void* wp) { return (int)static_cast<World*>(wp)->bodies.size(); }
 int nslo_contact_count(void* wp) { return static_cast<World*>(wp)->contact_count(); }

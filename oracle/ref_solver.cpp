@@ -555,6 +555,58 @@ void World::relax_color(int ci, const StepContext& ctx) {
   }
 }
 
+// The residual sweep (no reference counterpart; the checker of the GPU's per-colour
+// residuals): relax_color once more over each colour's contact constraints, in colour order,
+// from the state the solver stages end in; the impulse changes it makes are measured and
+// then the constraints and body states are put back. Per constraint: the sum (in point
+// order) and the largest |normal impulse change| and the length of the friction impulse
+// change. The GPU reduces 64 constraints at a time with a lane butterfly (32, 16, 8, 4, 2,
+// 1) and adds the partials in double; the same grouping is reproduced here.
+void World::colour_residuals(const StepContext& ctx) {
+  const std::vector<BodyState> states = awake_states;
+  for (int ci = 0; ci < kGraphColorCount - 1; ++ci) {
+    const std::vector<ContactConstraint>& list = color_constraints[ci];
+    std::vector<ManifoldConstraint> before;
+    for (const ContactConstraint& c : list)
+      for (int mo = 0; mo < c.manifold_count; ++mo) before.push_back(manifold_pool[c.manifold_start + mo]);
+    relax_color(ci, ctx);
+    ColourResidual r;
+    r.constraints = (int32_t)list.size();
+    size_t b = 0;
+    for (size_t w0 = 0; w0 < list.size(); w0 += 64) {
+      float sum[64];
+      for (int l = 0; l < 64; ++l) {
+        sum[l] = 0.0f;
+        if (w0 + l >= list.size()) continue;
+        const ContactConstraint& c = list[w0 + l];
+        for (int mo = 0; mo < c.manifold_count; ++mo, ++b) {
+          const ManifoldConstraint& old = before[b];
+          const ManifoldConstraint& now = manifold_pool[c.manifold_start + mo];
+          for (int p = 0; p < now.point_count; ++p) {
+            const float d = std::fabs(now.points[p].normal_impulse - old.points[p].normal_impulse);
+            sum[l] += d;
+            r.max_normal = std::max(r.max_normal, d);
+          }
+          const float dx = now.friction_impulse.x - old.friction_impulse.x;
+          const float dy = now.friction_impulse.y - old.friction_impulse.y;
+          r.max_friction = std::max(r.max_friction, std::sqrt(dx * dx + dy * dy));
+        }
+      }
+      for (int off = 32; off >= 1; off >>= 1) {
+        float next[64];
+        for (int l = 0; l < 64; ++l) next[l] = sum[l] + sum[l ^ off];
+        std::copy(next, next + 64, sum);
+      }
+      r.sum_normal += (double)sum[0];
+    }
+    b = 0;
+    for (const ContactConstraint& c : list)
+      for (int mo = 0; mo < c.manifold_count; ++mo) manifold_pool[c.manifold_start + mo] = before[b++];
+    residuals[ci] = r;
+  }
+  awake_states = states;
+}
+
 // contact_solver.rs:1135-1229
 void World::restitution_color(int ci, const StepContext& ctx) {
   const float inv_h = ctx.inv_h;

@@ -1007,6 +1007,7 @@ void World::step(float time_step, int sub_step_count) {
     if (after_prepare) after_prepare(*this, ctx);
     double t3 = omp_get_wtime();
     solve_stages(ctx);
+    if (residuals_on) colour_residuals(ctx);
     double t4 = omp_get_wtime();
     finalize_bodies(ctx);
     mark_awake_islands();

@@ -422,6 +422,15 @@ class World {
   // called after each step's prepare, before the solver stages (the rollout recorder of
   // ref_adjoint.cpp); not part of the reference
   std::function<void(World&, const StepContext&)> after_prepare;
+  // Per-colour residual sweep after each step's solver stages while on (not part of the
+  // reference; the checker of the GPU's diagnostic): see colour_residuals.
+  struct ColourResidual {
+    double sum_normal = 0.0;
+    float max_normal = 0.0f, max_friction = 0.0f;
+    int32_t constraints = 0;
+  };
+  bool residuals_on = false;
+  ColourResidual residuals[23];
 
   // --- state (public so the checker can inspect intermediate results) ---
   std::vector<Body> bodies;
@@ -508,6 +517,7 @@ class World {
   void warm_start_color(int ci);
   void push_color(int ci, const StepContext& ctx);
   void relax_color(int ci, const StepContext& ctx);
+  void colour_residuals(const StepContext& ctx);
   void restitution_color(int ci, const StepContext& ctx);
   void store_color(int ci);
   void warm_start_overflow();

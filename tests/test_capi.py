@@ -48,7 +48,7 @@ C_STRUCTS = {
     "KernelStats": "nsl_kernel_stats", "ToiInput": "nsl_toi_input", "ToiOutput": "nsl_toi_output",
     "RevoluteJointDef": "nsl_revolute_joint_def", "BatchStats": "nsl_batch_stats", "ShapeGeometry": "nsl_shape_geometry",
     "CollideCache": "nsl_collide_cache", "CollideQuery": "nsl_collide_query", "LocalManifold": "nsl_local_manifold",
-    "Tuning": "nsl_tuning", "BatchTuning": "nsl_batch_tuning",
+    "Tuning": "nsl_tuning", "BatchTuning": "nsl_batch_tuning", "ColourResidual": "nsl_colour_residual",
 }
 
 
