@@ -137,6 +137,10 @@ struct SolvePlan {
   // overflow colour (its velocity is integrated by any lane), else the last substep + 1 whose
   // velocity workgroup 0 integrated (0: none yet). The host sets -1; k_prepare marks.
   int32_t* ovf_done;
+  // The overflow colour's dynamic bodies (awake indices, each once): with warm start by body,
+  // workgroup 0 does their per-body work before the overflow push and relax levels.
+  const int32_t* ovf_bodies;
+  int32_t n_ovf_bodies;
   GridBarrier* barrier;
   uint64_t* phase_log;            // null: no phase timing; else [2 * kMaxPhaseLog]: (class, wall clock)
 };

@@ -371,6 +371,17 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
   }
   __syncthreads();
   const bool any_ovf = plan.n_ovf_levels > 0;
+  // With warm start by body, the overflow colour's push and relax levels run on workgroup 0
+  // at the end of the phase before them (the body phase of warm start, the position
+  // integration), after workgroup 0 has done those phases' per-body work for the overflow
+  // colour's bodies itself (the other lanes skip them): the overflow items see the body
+  // states the sequential walk gives them, without a grid barrier of their own.
+  const bool ovf_merged = any_ovf && plan.ws_table != nullptr;
+  // In those two phases workgroup 0 does only the overflow colour's work when other
+  // workgroups can take the rest: its chain runs beside their per-body work.
+  const bool wg0_ovf_only = ovf_merged && plan.n_work > 1;
+  const int body_tid = wg0_ovf_only ? tid - (int)blockDim.x : tid;
+  const int body_threads = wg0_ovf_only ? nthreads - (int)blockDim.x : nthreads;
   auto no_joint = [](int32_t) {};
 
   for (int sub = 0; sub < p.sub_steps; ++sub) {
@@ -439,7 +450,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
       }
       if (!grid_sync(grid)) return;
       clock.mark(kPWarmStart);
-      for (int i = tid; i < d.n_awake; i += nthreads) {
+      auto ws_body = [&](int i) {
         int32_t e[kOverflowColor];
 #pragma unroll
         for (int c = 0; c < kOverflowColor; ++c) e[c] = plan.ws_table[c * d.n_awake + i];
@@ -466,6 +477,16 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
           }
         }
         d.states[i] = s;
+      };
+      if (body_tid >= 0)
+        for (int i = body_tid; i < d.n_awake; i += body_threads)
+          if (!ovf_merged || plan.ovf_done[i] < 0) ws_body(i);
+      if (ovf_merged && wg == 0) {
+        __syncthreads();
+        for (int k = (int)threadIdx.x; k < plan.n_ovf_bodies; k += (int)blockDim.x) ws_body(plan.ovf_bodies[k]);
+        __syncthreads();
+        run_overflow_levels(plan, wg, [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
+                            [&](int k) { overflow_solve_item(d, k, p, 0); });
       }
       if (!grid_sync(grid)) return;
       clock.mark(kPWarmStart);
@@ -481,7 +502,7 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         return;
     }
 
-    if (any_ovf) {
+    if (any_ovf && !ovf_merged) {
       run_overflow_levels(plan, wg, [&](int32_t jid) { solve_joint_item(d, jid, p, true); },
                           [&](int k) { overflow_solve_item(d, k, p, 0); });
       if (!grid_sync(grid)) return;
@@ -497,11 +518,21 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve_step(DeviceWorld d, Solve
         return;
     }
 
-    for (int i = tid; i < d.n_awake; i += nthreads) integrate_positions_item(d, i, p);
+    if (body_tid >= 0)
+      for (int i = body_tid; i < d.n_awake; i += body_threads)
+        if (!ovf_merged || plan.ovf_done[i] < 0) integrate_positions_item(d, i, p);
+    if (ovf_merged && wg == 0) {
+      __syncthreads();
+      for (int k = (int)threadIdx.x; k < plan.n_ovf_bodies; k += (int)blockDim.x)
+        integrate_positions_item(d, plan.ovf_bodies[k], p);
+      __syncthreads();
+      run_overflow_levels(plan, wg, [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
+                          [&](int k) { overflow_solve_item(d, k, p, 1); });
+    }
     if (!grid_sync(grid)) return;
     clock.mark(kPIntegratePositions);
 
-    if (any_ovf) {
+    if (any_ovf && !ovf_merged) {
       run_overflow_levels(plan, wg, [&](int32_t jid) { solve_joint_item(d, jid, p, false); },
                           [&](int k) { overflow_solve_item(d, k, p, 1); });
       if (!grid_sync(grid)) return;

@@ -287,6 +287,9 @@ class GpuWorld : public HostScene {
   int32_t* d_ws_table_ = nullptr;  // warm start by body: the item of each colour per awake body
   WarmDelta* d_ws_delta_ = nullptr;  // and each item's velocity change per side
   int32_t* d_ovf_done_ = nullptr;  // overflow bodies' velocity integration marks
+  int32_t* d_ovf_bodies_ = nullptr;  // the overflow colour's dynamic bodies, each once
+  size_t cap_ovf_bodies_ = 0;
+  std::vector<int32_t> ovf_bodies_host_;
   // residual sweep: the body-state copy, the wavefront partials (device and pinned) and the
   // last step's per-colour results
   bool residuals_on_ = false;

@@ -58,7 +58,7 @@ GpuWorld::~GpuWorld() {
                   d_rebuild_keys_, d_.pairs.slots, d_.joints, d_.joint_ids, d_.joint_pairs.slots, d_erase_keys_,
                   d_ovf_items_, d_ovf_levels_, d_.shape_aabbs, d_.fast_ids, d_candidates_, d_.sleep_times,
                   d_.sleep_sims, d_patch_, d_patch_sims_, d_.change_list, d_.contact_active, d_ws_table_, d_ws_delta_, d_ovf_done_,
-                  d_res_states_, d_res_partials_};
+                  d_res_states_, d_res_partials_, d_ovf_bodies_};
   finder_.reset();
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -557,6 +557,7 @@ int32_t GpuWorld::build_overflow_levels() {
   const std::vector<int32_t>& ovf_joints = colors_[kOverflowColor].joints;
   const std::vector<int32_t>& ovf = colors_[kOverflowColor].overflow;
   const size_t n = ovf_joints.size() + ovf.size();
+  ovf_bodies_host_.clear();
   if (n == 0) return 0;
   if (body_level_.size() < bodies_.size()) body_level_.resize(bodies_.size(), 0);
   std::vector<int32_t> level(n);
@@ -579,7 +580,13 @@ int32_t GpuWorld::build_overflow_levels() {
     if (da) { body_level_[a] = l + 1; touched.push_back(a); }
     if (db) { body_level_[b] = l + 1; touched.push_back(b); }
   }
-  for (int32_t t : touched) body_level_[t] = 0;
+  // reset the levels, listing each dynamic body once (its awake index)
+  ovf_bodies_host_.clear();
+  for (int32_t t : touched)
+    if (body_level_[t] != 0) {
+      body_level_[t] = 0;
+      if (bodies_[t].set == 2) ovf_bodies_host_.push_back(bodies_[t].local);
+    }
   ovf_level_host_.assign(n_levels + 1, 0);
   for (size_t i = 0; i < n; ++i) ovf_level_host_[level[i] + 1] += 1;
   for (int32_t l = 0; l < n_levels; ++l) ovf_level_host_[l + 1] += ovf_level_host_[l];
@@ -592,6 +599,11 @@ int32_t GpuWorld::build_overflow_levels() {
   NSL_CHECK(hipMemcpyAsync(d_ovf_items_, ovf_items_host_.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
   NSL_CHECK(hipMemcpyAsync(d_ovf_levels_, ovf_level_host_.data(), ovf_level_host_.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, stream_));
+  if (!ovf_bodies_host_.empty()) {
+    ensure(d_ovf_bodies_, cap_ovf_bodies_, ovf_bodies_host_.size(), false);
+    NSL_CHECK(hipMemcpyAsync(d_ovf_bodies_, ovf_bodies_host_.data(), ovf_bodies_host_.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice, stream_));
+  }
   return n_levels;
 }
 
@@ -1005,6 +1017,8 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
   plan.n_ovf_levels = build_overflow_levels();
   plan.ovf_items = d_ovf_items_;
   plan.ovf_level_begin = d_ovf_levels_;
+  plan.ovf_bodies = d_ovf_bodies_;
+  plan.n_ovf_bodies = (int32_t)ovf_bodies_host_.size();
   plan.n_constraints = (int32_t)solve.size();
   plan.n_overflow = (int32_t)ovf.size();
   plan.any_restitution = any_restitution ? 1 : 0;
@@ -1138,7 +1152,9 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
     // warm start by body: the delta phase (with the overflow levels and the velocity
     // integration) and the body phase
     const int iv_ws_phases = plan.ws_table != nullptr ? 2 : 1 + phases + ovf_phase;
-    int64_t barriers = (int64_t)p.sub_steps * (1 + iv_ws_phases + 2 * (phases + ovf_phase));
+    // (with warm start by body the overflow push and relax levels end the phases before them)
+    const int ovf_solve_phases = plan.ws_table != nullptr ? 0 : ovf_phase;
+    int64_t barriers = (int64_t)p.sub_steps * (1 + iv_ws_phases + 2 * (phases + ovf_solve_phases));
     if (any_restitution) barriers += (int64_t)options.restitution_iterations * (phases + (ovf.empty() ? 0 : 1));
     kstats_.barriers += barriers;
   }
