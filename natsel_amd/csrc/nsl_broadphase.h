@@ -39,9 +39,12 @@ class PairFinder {
 
   // New pair keys in ascending order. Clears the moved flags and the moved counter. Few
   // moved proxies are tested directly against every proxy, more through the grid; the keys
-  // do not depend on the path. Synchronises the stream two or three times, and once more
-  // when there are new pairs.
-  void find(const DeviceWorld& d, const Proxy* proxies, int n_shapes, std::vector<uint64_t>& keys);
+  // do not depend on the path. `moved_hint`: the number of moved proxies if the caller
+  // knows it (it picks the path without a round trip; a wrong hint costs time, not
+  // results), else -1. Synchronises the stream once to three times, and once more when
+  // there are new pairs.
+  void find(const DeviceWorld& d, const Proxy* proxies, int n_shapes, std::vector<uint64_t>& keys,
+            int64_t moved_hint = -1);
 
   // Candidate pairs tested by the last find (for profiling).
   int64_t last_tests() const { return host_[1]; }

@@ -65,10 +65,27 @@ __global__ void k_grid_extent(int n, const Aabb* fat, const Proxy* proxies, Grid
     c += __shfl_xor(c, off, 64);
     m = fmaxf(m, __shfl_xor(m, off, 64));
   }
-  if ((threadIdx.x & 63) == 0 && c > 0.0f) {
-    atomicAdd(&g->sum, e);
-    atomicAdd(&g->count, c);
-    atomicMax(reinterpret_cast<unsigned int*>(&g->max_ext), __float_as_uint(m));  // non-negative: bit order is value order
+  // the workgroup's wavefronts combined in LDS: one set of atomics per workgroup (one per
+  // wavefront serialised thousands of atomics on three words)
+  __shared__ float s_part[3][kBlock / 64];
+  const int wave = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_part[0][wave] = e;
+    s_part[1][wave] = c;
+    s_part[2][wave] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)blockDim.x / 64; ++w) {
+      e += s_part[0][w];
+      c += s_part[1][w];
+      m = fmaxf(m, s_part[2][w]);
+    }
+    if (c > 0.0f) {
+      atomicAdd(&g->sum, e);
+      atomicAdd(&g->count, c);
+      atomicMax(reinterpret_cast<unsigned int*>(&g->max_ext), __float_as_uint(m));  // non-negative: bit order is value order
+    }
   }
 }
 
@@ -261,7 +278,7 @@ void PairFinder::ensure(T*& ptr, size_t& cap, size_t n) {
   cap = new_cap;
 }
 
-void PairFinder::find(const DeviceWorld& d, const Proxy* proxies, int n, std::vector<uint64_t>& keys) {
+void PairFinder::find(const DeviceWorld& d, const Proxy* proxies, int n, std::vector<uint64_t>& keys, int64_t moved_hint) {
   keys.clear();
   host_[0] = host_[1] = 0;
   if (n < 2) {
@@ -278,10 +295,17 @@ void PairFinder::find(const DeviceWorld& d, const Proxy* proxies, int n, std::ve
   // the moved proxies, listed and counted
   BP_CHECK(hipMemsetAsync(counts_ + n, 0, sizeof(int32_t), stream_));
   hipLaunchKernelGGL(k_list_moved, dim3(nb), dim3(kBlock), 0, stream_, n, d.shape_moved, large_, counts_ + n);
-  BP_CHECK(hipMemcpyAsync(host_entries_, counts_ + n, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
-  BP_CHECK(hipStreamSynchronize(stream_));
-  const int64_t n_moved = *host_entries_;
-  if (n_moved * (int64_t)n <= kDirectTests) {
+  // The path only changes the work: with a count of moved proxies from the caller that
+  // clearly fits the direct path, the direct kernel reads the listed count itself and the
+  // count's round trip to the host is skipped.
+  const bool direct_by_hint = moved_hint >= 0 && moved_hint * (int64_t)n <= kDirectTests / 2;
+  int64_t n_moved = moved_hint;
+  if (!direct_by_hint) {
+    BP_CHECK(hipMemcpyAsync(host_entries_, counts_ + n, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+    BP_CHECK(hipStreamSynchronize(stream_));
+    n_moved = *host_entries_;
+  }
+  if (direct_by_hint || n_moved * (int64_t)n <= kDirectTests) {
     // few moved proxies: each against every proxy, no grid
     for (int attempt = 0; attempt < 2; ++attempt) {
       BP_CHECK(hipMemsetAsync(found_, 0, 2 * sizeof(int64_t), stream_));

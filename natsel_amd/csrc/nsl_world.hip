@@ -216,7 +216,9 @@ void GpuWorld::pull_bodies() {
 
 void GpuWorld::update_pairs() {
   if (!pairs_pending_ && counters_host_[6] == 0) return;
-  finder_->find(d_, d_proxies_, (int)n_device_shapes_, new_keys_);
+  // the last step's finalize counted the proxies it moved; other moves (new shapes) leave
+  // the count to the finder
+  finder_->find(d_, d_proxies_, (int)n_device_shapes_, new_keys_, pairs_pending_ ? -1 : (int64_t)counters_host_[6]);
   for (HShape& s : hshapes_) s.proxy_moved = false;
   pairs_pending_ = false;
   kstats_.pair_tests += finder_->last_tests();
