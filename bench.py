@@ -6,7 +6,7 @@ lattice cells of a 20 x 60 x 20 m region into a static 5-wall container; dt = 1/
 4 substeps, f32, continuous collision and sleeping off; the pile's run is 1000 steps. A
 "step" is one World::step (pair update, narrow phase, constraint preparation, 4 soft-step
 substeps, impulse storage, finalize). The timed K steps are the last K of the 1000-step run
-(the settled pile, ~48k contact points per step): the first 1000 - K steps (at least W)
+(the settled pile, ~33k constraints and ~57k contact points per step): the first 1000 - K steps (at least W)
 run untimed, then K steps are timed with a device synchronise on both sides.
 
 value = dynamic bodies x substeps x K / seconds (inputs resident in HBM: the scene is
@@ -252,6 +252,7 @@ def large_world_measurement(steps: int) -> dict:
             world.step(sc.dt, sc.substeps)
         torch.cuda.synchronize()
         runs.append((time.perf_counter() - t0, world.kernel_stats()))
+        _progress(f"large world: {steps} steps, phase clock {'on' if timing else 'off'}")
         world.close()
     elapsed, _ = runs[0]
     ks = runs[1][1]
@@ -325,9 +326,17 @@ def defaults_measurement(steps: int, cpu_seconds: float) -> dict:
         ref.step(sc.dt, sc.substeps)
         n += 1
     cpu_s = time.perf_counter() - c0
-    check = load_scene(sc)
+    # the GPU over the oracle's steps 1..n (like for like with the oracle's rate; the first
+    # step includes the upload), ending on the hash compared with the oracle's
+    check = load_scene(sc, **WORLD_OPTS)
+    check.reset_kernel_stats()
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
     for _ in range(n):
         check.step(sc.dt, sc.substeps)
+    torch.cuda.synchronize()
+    gpu_sample_s = time.perf_counter() - g0
+    gpu_sample_substeps = check.kernel_stats()["bodies"] * sc.substeps
     same = check.world_state_hash() == ref.world_state_hash()
     check.close()
     return {
@@ -346,6 +355,9 @@ def defaults_measurement(steps: int, cpu_seconds: float) -> dict:
         "cpu_baseline": {"value": awake * sc.substeps / cpu_s, "unit": "body-substeps/s", "cores": pyoracle.max_threads(),
                          "kind": "port", "sample": f"CPU oracle, steps 1..{n} of the same scene", "steps": n, "seconds": cpu_s},
         "hash_equal_to_oracle_after_sample": same,
+        "gpu_same_steps": {"value": gpu_sample_substeps / gpu_sample_s, "steps": n,
+                           "gpu_over_cpu": (gpu_sample_substeps / gpu_sample_s) / (awake * sc.substeps / cpu_s),
+                           "note": "the GPU over the oracle's steps 1..n, awake body-substeps counted the same way"},
     }
 
 
@@ -582,8 +594,10 @@ def main() -> int:
     # W warmup steps; the timed window is the run's last K steps
     torch.cuda.synchronize()
     s0 = time.perf_counter()
-    for _ in range(settle):
+    for k in range(settle):
         world.step(scene.dt, scene.substeps)
+        if (k + 1) % 250 == 0:
+            _progress(f"pile: untimed step {k + 1}")
     torch.cuda.synchronize()
     settle_s = time.perf_counter() - s0
     # the step kernel's launch shape for the settled pile, by timing (results do not change;
@@ -615,6 +629,19 @@ def main() -> int:
     world.set_kernel_timing(False)
     ks_phases = world.kernel_stats()
     rerun_equal = world.world_state_hash() == hashes["end"]
+    # the solver's per-colour residuals on the first timed step (a diagnostic sweep after the
+    # step kernel; untimed, results unchanged)
+    world.restore_state(snap)
+    world.set_residuals(True)
+    world.step(scene.dt, scene.substeps)
+    world.set_residuals(False)
+    colour_res = [r for r in world.colour_residuals() if r["constraints"] > 0]
+    residuals = {"step": settle + 1, "colours": len(colour_res),
+                 "max_normal_impulse_change": max((r["max_normal"] for r in colour_res), default=0.0),
+                 "sum_normal_impulse_change": sum(r["sum_normal"] for r in colour_res),
+                 "max_friction_impulse_change": max((r["max_friction"] for r in colour_res), default=0.0),
+                 "per_colour_max_normal": [r["max_normal"] for r in colour_res],
+                 "note": "one more relax sweep over the coloured contacts after the step (N s impulses)"}
     world_opts = {"small_colour_items": world.options.small_colour_items, "stage_records": int(world.options.stage_records)}
     _progress(f"pile: {args.steps} timed steps, {1e3 * elapsed / args.steps:.3f} ms per step")
     del snap
@@ -686,6 +713,7 @@ def main() -> int:
         "hash_after_settle": f"{hashes['settle']:016x}",
         "hash_after_window": f"{hashes['end']:016x}",
         "rerun_from_snapshot_same_hash": rerun_equal,
+        "solver_residuals": residuals,
     }
     if not args.no_cpu:
         cb = cpu_baseline(scene, settle, args.steps, args.cpu_seconds, hashes)
