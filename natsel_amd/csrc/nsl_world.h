@@ -267,6 +267,9 @@ class GpuWorld : public HostScene {
   int32_t graph_restitution_ = 0;  // graph contacts with restitution > 0
   // contacts whose state changed in the last collide: id << 2 | ContactChange, ascending
   std::vector<int32_t> changes_;
+  std::vector<uint64_t> change_bits_;  // sort_changes: contact ids present
+  std::vector<uint8_t> change_code_;   // sort_changes: the change of each present id
+  void sort_changes(const int32_t* list, size_t m, size_t ncap);
   int32_t* changes_pinned_ = nullptr;  // download staging: [0] count, then the list
   size_t cap_changes_pinned_ = 0, cap_change_list_ = 0, last_changes_ = 0;
   void ensure_pinned_changes(size_t n);

@@ -288,7 +288,8 @@ void GpuWorld::create_contact(int32_t a, int32_t b) {
   hc.body[1] = sb.body;
   link_body_contact(id);
   if (hc.set != 2) mark_contact_dirty(id);  // the device turns new contacts on
-  Contact c{};
+  init_ids_.push_back(id);
+  Contact& c = init_records_.emplace_back();  // zeroed in place: the record the device starts from
   c.shape_a = a;
   c.shape_b = b;
   c.enc_a = encode(ba);
@@ -301,8 +302,6 @@ void GpuWorld::create_contact(int32_t a, int32_t b) {
   c.cached_rot_a = quat_identity();
   c.cached_rot_b = quat_identity();
   hc.created = ++contact_serial_;
-  init_ids_.push_back(id);
-  init_records_.push_back(c);
 }
 
 template <class T>
@@ -951,8 +950,7 @@ void GpuWorld::step(float dt, int32_t sub_steps) {
       NSL_CHECK(hipMemcpyAsync(changes_pinned_ + 1, d_.change_list, m * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
       NSL_CHECK(hipStreamSynchronize(stream_));
     }
-    changes_.assign(changes_pinned_ + 1, changes_pinned_ + 1 + m);
-    std::sort(changes_.begin(), changes_.end());  // contact-id order (id in the high bits)
+    sort_changes(changes_pinned_ + 1, m, ncap);  // contact-id order (id in the high bits)
     last_changes_ = m;
   }
   const double t2 = now_ms();
@@ -1238,6 +1236,31 @@ void GpuWorld::collect_residuals(const ColorRanges& cr) {
     }
     residuals_[ci] = r;
   }
+}
+
+// The collide kernel's change list in contact-id order. Each contact id appears at most once
+// and lies below `ncap`: long lists go through a bitmap of ids scanned in order (linear in
+// ncap / 64 + m), short ones through a comparison sort.
+void GpuWorld::sort_changes(const int32_t* list, size_t m, size_t ncap) {
+  changes_.assign(list, list + m);
+  if (m < 4096) {
+    std::sort(changes_.begin(), changes_.end());
+    return;
+  }
+  const size_t words = (ncap + 63) / 64;
+  change_bits_.assign(words, 0ull);
+  if (change_code_.size() < ncap) change_code_.resize(ncap);
+  for (const int32_t v : changes_) {
+    const uint32_t id = (uint32_t)v >> 2;
+    change_bits_[id >> 6] |= 1ull << (id & 63);
+    change_code_[id] = (uint8_t)(v & 3);
+  }
+  size_t k = 0;
+  for (size_t w = 0; w < words; ++w)
+    for (uint64_t bits = change_bits_[w]; bits != 0; bits &= bits - 1) {
+      const uint32_t id = (uint32_t)(w * 64 + (size_t)__builtin_ctzll(bits));
+      changes_[k++] = (int32_t)(id << 2 | change_code_[id]);
+    }
 }
 
 void GpuWorld::set_kernel_timing(bool on, bool phases) {
