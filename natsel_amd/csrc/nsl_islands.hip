@@ -380,8 +380,8 @@ void GpuWorld::remove_joint_from_graph(int32_t id) {
   HJoint& j = hjoints_[(size_t)id];
   const int32_t color = j.color, local = j.local;
   if (color != kOverflowColor) {
-    colors_[color].clear(j.body_a);
-    colors_[color].clear(j.body_b);
+    colour_clear(color, j.body_a);
+    colour_clear(color, j.body_b);
   }
   std::vector<int32_t>& list = colors_[color].joints;
   if (swap_out(list, (size_t)local) != kNullIndex) hjoints_[(size_t)list[(size_t)local]].local = local;

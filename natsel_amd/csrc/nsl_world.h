@@ -159,14 +159,32 @@ class GpuWorld : public HostScene {
     void release(int32_t id) { free.push_back(id); }  // no live id equals next
   };
   struct ColorSet {
-    std::vector<uint64_t> bits;
     std::vector<int32_t> convex;
     std::vector<int32_t> overflow;
     std::vector<int32_t> joints;  // joint ids in graph order (appended, swap-removed)
-    bool get(int32_t b) const { uint32_t w = (uint32_t)b >> 6; return w < bits.size() && ((bits[w] >> (b & 63)) & 1); }
-    void set(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w >= bits.size()) bits.resize(w + 1, 0); bits[w] |= 1ull << (b & 63); }
-    void clear(int32_t b) { uint32_t w = (uint32_t)b >> 6; if (w < bits.size()) bits[w] &= ~(1ull << (b & 63)); }
   };
+  // The colours' body bits (constraint_graph.rs: each colour's body set), kept per body: bit
+  // c of body_colours_[b] is set when colour c holds body b, so a colour search is one mask.
+  std::vector<uint32_t> body_colours_;
+  uint32_t colours_of(int32_t b) const { return (size_t)b < body_colours_.size() ? body_colours_[(size_t)b] : 0u; }
+  void colour_set(int c, int32_t b) {
+    if ((size_t)b >= body_colours_.size()) body_colours_.resize(std::max((size_t)b + 1, bodies_.size()), 0u);
+    body_colours_[(size_t)b] |= 1u << c;
+  }
+  void colour_clear(int c, int32_t b) {
+    if ((size_t)b < body_colours_.size()) body_colours_[(size_t)b] &= ~(1u << c);
+  }
+  // The colour the reference's search picks for two dynamic bodies (the first colour below
+  // kDynamicColorCount holding neither) or for one dynamic body beside a static one (the
+  // last colour from kOverflowColor - 1 down to 1 not holding it); kOverflowColor if none.
+  int32_t free_colour_dynamic(int32_t a, int32_t b) const {
+    const uint32_t avail = ~(colours_of(a) | colours_of(b)) & ((1u << kDynamicColorCount) - 1u);
+    return avail != 0u ? __builtin_ctz(avail) : kOverflowColor;
+  }
+  int32_t free_colour_static(int32_t dyn) const {
+    const uint32_t avail = ~colours_of(dyn) & (((1u << kOverflowColor) - 1u) & ~1u);
+    return avail != 0u ? 31 - __builtin_clz(avail) : kOverflowColor;
+  }
 
   int32_t add_shape(int32_t body, Shape shape, const Material& m, float radius) override;
 

@@ -349,22 +349,15 @@ void GpuWorld::add_to_graph(int32_t id) {
   int32_t ta = bodies_[a].type, tb = bodies_[b].type;
   int32_t color = kOverflowColor;
   if (ta == kBodyDynamic && tb == kBodyDynamic) {
-    for (int i = 0; i < kDynamicColorCount; ++i) {
-      if (colors_[i].get(a) || colors_[i].get(b)) continue;
-      colors_[i].set(a);
-      colors_[i].set(b);
-      color = i;
-      break;
+    color = free_colour_dynamic(a, b);
+    if (color != kOverflowColor) {
+      colour_set(color, a);
+      colour_set(color, b);
     }
   } else if (ta == kBodyDynamic || tb == kBodyDynamic) {
-    int32_t dyn = ta == kBodyDynamic ? a : b;
-    for (int i = kOverflowColor - 1; i >= 1; --i) {
-      if (!colors_[i].get(dyn)) {
-        colors_[i].set(dyn);
-        color = i;
-        break;
-      }
-    }
+    const int32_t dyn = ta == kBodyDynamic ? a : b;
+    color = free_colour_static(dyn);
+    if (color != kOverflowColor) colour_set(color, dyn);
   }
   c.color = color;
   graph_restitution_ += c.restitution > 0.0f;
@@ -382,8 +375,8 @@ void GpuWorld::remove_from_graph(int32_t body_a, int32_t body_b, int32_t color, 
   const int32_t id = color == kOverflowColor ? colors_[color].overflow[(size_t)local] : colors_[color].convex[(size_t)local];
   graph_restitution_ -= hcontacts_[(size_t)id].restitution > 0.0f;
   if (color != kOverflowColor) {
-    colors_[color].clear(body_a);
-    colors_[color].clear(body_b);
+    colour_clear(color, body_a);
+    colour_clear(color, body_b);
     if (swap_remove(colors_[color].convex, (size_t)local) != kNullIndex) hcontacts_[colors_[color].convex[local]].local = local;
   } else {
     if (swap_remove(colors_[color].overflow, (size_t)local) != kNullIndex) hcontacts_[colors_[color].overflow[local]].local = local;
@@ -423,20 +416,17 @@ void GpuWorld::state_pass() {
 int32_t GpuWorld::assign_joint_color(int32_t a, int32_t b) {
   const int32_t ta = bodies_[a].type, tb = bodies_[b].type;
   if (ta == kBodyDynamic && tb == kBodyDynamic) {
-    for (int i = 0; i < kDynamicColorCount; ++i) {
-      if (colors_[i].get(a) || colors_[i].get(b)) continue;
-      colors_[i].set(a);
-      colors_[i].set(b);
-      return i;
+    const int32_t i = free_colour_dynamic(a, b);
+    if (i != kOverflowColor) {
+      colour_set(i, a);
+      colour_set(i, b);
     }
+    return i;
   } else if (ta == kBodyDynamic || tb == kBodyDynamic) {
     const int32_t dyn = ta == kBodyDynamic ? a : b;
-    for (int i = kOverflowColor - 1; i >= 1; --i) {
-      if (!colors_[i].get(dyn)) {
-        colors_[i].set(dyn);
-        return i;
-      }
-    }
+    const int32_t i = free_colour_static(dyn);
+    if (i != kOverflowColor) colour_set(i, dyn);
+    return i;
   }
   return kOverflowColor;
 }
@@ -642,6 +632,7 @@ struct GpuWorld::Snapshot {
   std::vector<int32_t> contact_free, non_touching;
   int32_t contact_pool_next = 0;
   ColorSet colors[kColorCount];
+  std::vector<uint32_t> body_colours;
   int32_t counters_host[kCounterCount] = {}, counters_base[kCounterCount] = {};
   int64_t pair_set_tombs = 0;
   bool pairs_pending = false;
@@ -767,6 +758,7 @@ GpuWorld::Snapshot* GpuWorld::save_state() {
   s->non_touching = non_touching_;
   s->contact_pool_next = contact_pool_next_;
   for (int c = 0; c < kColorCount; ++c) s->colors[c] = colors_[c];
+  s->body_colours = body_colours_;
   std::memcpy(s->counters_host, counters_host_, sizeof(counters_host_));
   std::memcpy(s->counters_base, counters_base_, sizeof(counters_base_));
   s->pair_set_tombs = pair_set_tombs_;
@@ -828,6 +820,7 @@ void GpuWorld::restore_state(const Snapshot& s) {
   non_touching_ = s.non_touching;
   contact_pool_next_ = s.contact_pool_next;
   for (int c = 0; c < kColorCount; ++c) colors_[c] = s.colors[c];
+  body_colours_ = s.body_colours;
   graph_restitution_ = 0;
   for (int c = 0; c < kColorCount; ++c) {
     for (int32_t id : colors_[c].convex) graph_restitution_ += hcontacts_[(size_t)id].restitution > 0.0f;
