@@ -118,6 +118,19 @@ def _timed_steps(w, scene, k: int) -> float:
     return time.perf_counter() - t0
 
 
+def _oracle_rate(w, scene, snap, k: int, threads: int, repeats: int = 3) -> float:
+    """Seconds for `k` oracle steps from snapshot `snap` on `threads` threads, the minimum of
+    `repeats` runs (each restored from the snapshot)."""
+    from oracle import pyoracle
+
+    pyoracle.set_threads(threads)
+    best = float("inf")
+    for _ in range(repeats):
+        w.restore_state(snap)
+        best = min(best, _timed_steps(w, scene, k))
+    return best
+
+
 def cpu_baseline(scene, settle: int, k: int, seconds: float, gpu_hashes: dict) -> dict:
     """The oracle (a C++ restatement of the reference step, OpenMP over the loops whose
     items are independent) on this host, timed on the same steps from the same state as the
@@ -172,7 +185,62 @@ def cpu_baseline(scene, settle: int, k: int, seconds: float, gpu_hashes: dict) -
     }
 
 
-def batched_measurement(worlds: int, steps: int, autotune: bool = True) -> dict:
+def batch_cpu_baseline(scene_list: list, steps: int, gpu_hashes, gpu_body_substeps: float, every: int = 64) -> dict:
+    """The oracle on a seeded sample of the batch's worlds (every `every`-th): each world
+    stepped once untimed, as the GPU's first step, then the same `steps` steps timed, on all
+    host threads (one world per thread, worlds in parallel) and on one thread, the minimum
+    of 3 repeats from per-world snapshots; the sample's hashes after the steps against the
+    GPU's. Rates in body-substeps/s (the sample's worlds differ in size from the whole
+    batch's mean) and world-steps/s."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import pyoracle
+
+    sample = list(range(0, len(scene_list), every))
+    threads = pyoracle.max_threads()
+    pyoracle.set_threads(1)  # one OpenMP thread per world; the worlds run on the pool's threads
+    ws = [pyoracle.load_scene(scene_list[i]) for i in sample]
+    for i, w in zip(sample, ws):
+        w.step(scene_list[i].dt, scene_list[i].substeps)
+    snaps = [w.save_state() for w in ws]
+
+    def run(j):
+        sc = scene_list[sample[j]]
+        pyoracle.set_threads(1)  # the OpenMP thread count is per calling thread
+        ws[j].restore_state(snaps[j])
+        for _ in range(steps):
+            ws[j].step(sc.dt, sc.substeps)
+
+    def timed(pool_threads):
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            if pool_threads == 1:
+                for j in range(len(ws)):
+                    run(j)
+            else:
+                with ThreadPoolExecutor(max_workers=pool_threads) as ex:
+                    list(ex.map(run, range(len(ws))))
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    t_n = timed(threads)
+    equal = all(int(ws[j].world_state_hash()) == int(gpu_hashes[sample[j]]) for j in range(len(ws)))
+    t_1 = timed(1)
+    pyoracle.set_threads(threads)
+    body_substeps = sum(scene_list[i].dynamic_count for i in sample) * scene_list[0].substeps * steps
+    return {
+        "value": body_substeps / t_n, "unit": "body-substeps/s", "cores": threads, "kind": "port",
+        "world_steps_per_s": len(sample) * steps / t_n,
+        "sample": f"CPU oracle on {len(sample)} of the {len(scene_list)} worlds (every {every}th), {steps} steps after one "
+                  f"untimed step, the worlds spread over {threads} host threads (one OpenMP thread each), minimum of 3 repeats",
+        "single_thread": {"value": body_substeps / t_1, "world_steps_per_s": len(sample) * steps / t_1},
+        "hash_equal_to_gpu_for_sample": equal,
+        "gpu_over_cpu": gpu_body_substeps / (body_substeps / t_n),
+    }
+
+
+def batched_measurement(worlds: int, steps: int, autotune: bool = True, cpu: bool = True) -> dict:
     """Batched worlds (the log-uniform 8..256-body piles and stacks of
     natsel_amd.scenes.batched): every world stepped together, `steps` steps in one call
     after one untimed step; device time from HIP events around the call. With `autotune`,
@@ -191,6 +259,7 @@ def batched_measurement(worlds: int, steps: int, autotune: bool = True) -> dict:
     s0 = b.stats()
     b.step(dt, sub, steps)
     s1 = b.stats()
+    gpu_hashes = b.state_hashes()
     b.close()
     seconds = s1["last_step_ms"] / 1e3
     bodies = s1["bodies"]
@@ -218,6 +287,7 @@ def batched_measurement(worlds: int, steps: int, autotune: bool = True) -> dict:
         "points_per_step": counts["points"] / steps,
         "error_worlds": s1["error_worlds"],
         "autotune": tuning,
+        "cpu_baseline": batch_cpu_baseline(scene_list, steps, gpu_hashes, bodies * sub * steps / seconds) if cpu else None,
         "dispatch_window": [2 + trial_steps, 1 + trial_steps + steps],
         "dispatch_window_note": "each batch kernel runs once per step: one untimed step, the launch-shape trials' "
                                 f"{trial_steps} steps, then the timed steps (this process's first batch)",
@@ -228,7 +298,99 @@ def batched_measurement(worlds: int, steps: int, autotune: bool = True) -> dict:
 WORLD_OPTS: dict = {}
 
 
-def large_world_measurement(steps: int) -> dict:
+def pyramid_measurement(steps: int = 600, cpu: bool = True) -> dict:
+    """The pyramid of base 20 (210 boxes) with sleep and continuous collision off: one
+    untimed step, then `steps` steps timed on the GPU and on the oracle (all host threads and
+    one thread, minimum of 3 repeats from the snapshot), hashes compared."""
+    import dataclasses
+
+    import torch
+
+    from natsel_amd import scenes
+    from natsel_amd.world import load_scene
+    from oracle import pyoracle
+
+    sc = dataclasses.replace(scenes.pyramid(20), enable_sleep=False, enable_continuous=False)
+    g = load_scene(sc, **WORLD_OPTS)
+    g.step(sc.dt, sc.substeps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step(sc.dt, sc.substeps)
+    torch.cuda.synchronize()
+    t_g = time.perf_counter() - t0
+    gpu_hash = g.world_state_hash()
+    g.close()
+    n = sc.dynamic_count * sc.substeps * steps
+    out = {"metric": "body-substeps/s (pyramid, base 20)", "value": n / t_g, "unit": "body-substeps/s",
+           "ms_per_step": 1e3 * t_g / steps, "steps": steps,
+           "config": {"workload": f"pyramid of base 20 ({sc.dynamic_count} boxes), dt 1/60, 4 substeps, sleep and "
+                                  "continuous collision off; one untimed step first", "bodies": sc.dynamic_count}}
+    if cpu:
+        threads = pyoracle.max_threads()
+        w = pyoracle.load_scene(sc)
+        pyoracle.set_threads(threads)
+        w.step(sc.dt, sc.substeps)
+        snap = w.save_state()
+        t_n = _oracle_rate(w, sc, snap, steps, threads)
+        equal = w.world_state_hash() == gpu_hash
+        t_1 = _oracle_rate(w, sc, snap, steps, 1)
+        pyoracle.set_threads(threads)
+        out["cpu_baseline"] = {"value": n / t_n, "unit": "body-substeps/s", "cores": threads, "kind": "port",
+                               "sample": f"CPU oracle, the same {steps} steps, minimum of 3 repeats",
+                               "single_thread": {"value": n / t_1}, "hash_equal_to_gpu": equal,
+                               "gpu_over_cpu": (n / t_g) / (n / t_n)}
+    return out
+
+
+def large_world_cpu_baseline(sc, seconds: float) -> dict:
+    """The oracle on the large world: one untimed step, then a bounded sample of k steps
+    (sized to about a third of `seconds` per repeat) timed on all host threads and fewer on one
+    thread, minimum of 3 repeats from the snapshot; the GPU over the same steps 2..k + 1 from
+    the scene's start, and its hash after them against the oracle's."""
+    import torch
+
+    from natsel_amd.world import load_scene
+    from oracle import pyoracle
+
+    threads = pyoracle.max_threads()
+    pyoracle.set_threads(threads)
+    w = pyoracle.load_scene(sc)
+    w.step(sc.dt, sc.substeps)
+    snap = w.save_state()
+    one = _timed_steps(w, sc, 1)
+    k = int(max(2, min(60, seconds / 3 / max(one, 1e-6))))
+    t_n = _oracle_rate(w, sc, snap, k, threads)
+    cpu_hash = w.world_state_hash()
+    w.restore_state(snap)
+    pyoracle.set_threads(1)
+    one1 = _timed_steps(w, sc, 1)
+    k1 = int(max(1, min(k, seconds / 6 / max(one1, 1e-6))))
+    t_1 = _oracle_rate(w, sc, snap, k1, 1)
+    pyoracle.set_threads(threads)
+    del w, snap
+    g = load_scene(sc, **WORLD_OPTS)
+    g.step(sc.dt, sc.substeps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        g.step(sc.dt, sc.substeps)
+    torch.cuda.synchronize()
+    t_g = time.perf_counter() - t0
+    equal = g.world_state_hash() == cpu_hash
+    g.close()
+    n = sc.dynamic_count * sc.substeps
+    return {
+        "value": n * k / t_n, "unit": "body-substeps/s", "cores": threads, "kind": "port",
+        "sample": f"CPU oracle, steps 2..{k + 1} from the scene's start on {threads} threads (minimum of 3 repeats); "
+                  f"one thread: steps 2..{k1 + 1}",
+        "single_thread": {"value": n * k1 / t_1, "steps": k1},
+        "gpu_same_steps": {"value": n * k / t_g, "gpu_over_cpu": (n * k / t_g) / (n * k / t_n)},
+        "hash_equal_to_gpu_after_sample": equal,
+    }
+
+
+def large_world_measurement(steps: int, cpu_seconds: float = 0.0) -> dict:
     """The large single world: 100,000 mixed bodies in a seeded pile plus 2,000 chains of 16
     capsule links (30,000 revolute joints), steps 1..`steps` from the scene's start. One run
     for the rate, a second with the step kernel's phase clock on for the per-stage
@@ -288,6 +450,7 @@ def large_world_measurement(steps: int) -> dict:
         "points_per_step": ks["points"] / max(ks["steps"], 1),
         "constraints_per_step": ks["constraints"] / max(ks["steps"], 1),
         "note": "per-stage figures from the run with the phase clock on; value from the run without it",
+        "cpu_baseline": large_world_cpu_baseline(sc, cpu_seconds) if cpu_seconds > 0 else None,
     }
 
 
@@ -547,6 +710,7 @@ def main() -> int:
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched measurement (0: skip it)")
     ap.add_argument("--batched-steps", type=int, default=240)
     ap.add_argument("--large-world-steps", type=int, default=500, help="steps of the large-world measurement (0: skip it)")
+    ap.add_argument("--pyramid-steps", type=int, default=600, help="steps of the base-20 pyramid measurement (0: skip it)")
     ap.add_argument("--adjoint-worlds", type=int, default=1024, help="worlds of the differentiable rollout (0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=64)
     ap.add_argument("--defaults-steps", type=int, default=1000,
@@ -721,13 +885,16 @@ def main() -> int:
         out["cpu_baseline"] = cb
         _progress(f"cpu baseline: {cb['value'] / 1e6:.2f} M body-substeps/s on {cb['cores']} threads")
     if args.batched_worlds > 0:
-        out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps, args.autotune)
+        out["batched"] = batched_measurement(args.batched_worlds, args.batched_steps, args.autotune, not args.no_cpu)
         _progress(f"batched: {out['batched']['ms_per_step']:.3f} ms per step")
     if args.adjoint_worlds > 0:
         out["adjoint"] = adjoint_measurement(args.adjoint_worlds, args.adjoint_steps)
         _progress("adjoint done")
+    if args.pyramid_steps > 0:
+        out["pyramid"] = pyramid_measurement(args.pyramid_steps, not args.no_cpu)
+        _progress(f"pyramid: {out['pyramid']['ms_per_step']:.3f} ms per step")
     if args.large_world_steps > 0:
-        out["large_world"] = large_world_measurement(args.large_world_steps)
+        out["large_world"] = large_world_measurement(args.large_world_steps, 0.0 if args.no_cpu else args.cpu_seconds)
         _progress(f"large world: {out['large_world']['ms_per_step']:.3f} ms per step")
     if args.defaults_steps > 0:
         out["reference_defaults"] = defaults_measurement(args.defaults_steps, 0.4 * args.cpu_seconds)

@@ -296,16 +296,17 @@ void PairFinder::find(const DeviceWorld& d, const Proxy* proxies, int n, std::ve
   BP_CHECK(hipMemsetAsync(counts_ + n, 0, sizeof(int32_t), stream_));
   hipLaunchKernelGGL(k_list_moved, dim3(nb), dim3(kBlock), 0, stream_, n, d.shape_moved, large_, counts_ + n);
   // The path only changes the work: with a count of moved proxies from the caller that
-  // clearly fits the direct path, the direct kernel reads the listed count itself and the
-  // count's round trip to the host is skipped.
+  // clearly fits one path, the count's round trip to the host is skipped (the direct
+  // kernel reads the listed count itself; the grid does not need it).
   const bool direct_by_hint = moved_hint >= 0 && moved_hint * (int64_t)n <= kDirectTests / 2;
+  const bool grid_by_hint = moved_hint >= 0 && moved_hint * (int64_t)n > 2 * kDirectTests;
   int64_t n_moved = moved_hint;
-  if (!direct_by_hint) {
+  if (!direct_by_hint && !grid_by_hint) {
     BP_CHECK(hipMemcpyAsync(host_entries_, counts_ + n, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
     BP_CHECK(hipStreamSynchronize(stream_));
     n_moved = *host_entries_;
   }
-  if (direct_by_hint || n_moved * (int64_t)n <= kDirectTests) {
+  if (direct_by_hint || (!grid_by_hint && n_moved * (int64_t)n <= kDirectTests)) {
     // few moved proxies: each against every proxy, no grid
     for (int attempt = 0; attempt < 2; ++attempt) {
       BP_CHECK(hipMemsetAsync(found_, 0, 2 * sizeof(int64_t), stream_));
