@@ -328,7 +328,7 @@ __device__ __forceinline__ void colour_stage(const DeviceWorld& d, const Colours
   int a = 0, base = cs.begin[0];
   bool live = base + lane < cs.end[0];
   Constraint cur;
-  if (live) cur = d.constraints[base + lane];
+  if (live) cur = blocked_cons(d).load(base + lane);
   for (;;) {
     int na = a, nbase = base + kBatchLanes;
     if (nbase >= cs.end[a]) {
@@ -338,7 +338,7 @@ __device__ __forceinline__ void colour_stage(const DeviceWorld& d, const Colours
     const bool more = na < cs.n;
     const bool next_live = more && nbase + lane < cs.end[na];
     Constraint next;
-    if (next_live) next = d.constraints[nbase + lane];
+    if (next_live) next = blocked_cons(d).load(nbase + lane);
     if (live) item(base + lane, cur);
     if (na != a) wave_sync();
     if (!more) break;
@@ -516,7 +516,7 @@ __device__ void tape_record(const BatchDevice& b, int world, const BatchWorldDes
   const int n = mini(n_constraints, t.max_con);
   bool bad = false;
   for (int slot = lane; slot < n; slot += kBatchLanes) {
-    const Constraint& c = dl.constraints[slot];
+    const Constraint c = blocked_cons(dl).load(slot);
     const Contact& ct = dl.contacts[c.contact_id];
     t.cons[ws * t.max_con + slot] = c;
     for (int i = 0; i < 4; ++i) t.features[(ws * t.max_con + slot) * 4 + i] = i < c.point_count ? ct.points[i].feature_id : 0xffffffffu;
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kBatchLanes) __attribute__((amdgpu_waves_per_eu(2))
       const int32_t id = r.colors[s_colour_list[a] * r.color_cap + (slot - colours.begin[a])];
       dl.solve_ids[slot] = id;
       float restitution = 0.0f;
-      points += prepare_contact(dl, AosCons{dl.constraints}, slot, id, p, &restitution);
+      points += prepare_contact(dl, blocked_cons(dl), slot, id, p, &restitution);
       any_rest = any_rest || restitution > 0.0f;
     }
     for (int k = lane; k < n_ovf; k += kBatchLanes) {
@@ -723,14 +723,16 @@ __global__ __launch_bounds__(kBatchLanes) __attribute__((amdgpu_waves_per_eu(2))
         wave_sync();
       }
       for (int a = 0; a < colours.n; ++a) {
-        for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes)
-          restitution_item(dl, AosCons{dl.constraints}, slot, p, dl.constraints[slot].index_a, dl.constraints[slot].index_b);
+        for (int slot = colours.begin[a] + lane; slot < colours.end[a]; slot += kBatchLanes) {
+          const int2 ix = blocked_cons(dl).indices(slot);
+          restitution_item(dl, blocked_cons(dl), slot, p, ix.x, ix.y);
+        }
         wave_sync();
       }
     }
   }
   for (int k = lane; k < n_ovf; k += kBatchLanes) overflow_store_item(dl, k);
-  for (int slot = lane; slot < n_constraints; slot += kBatchLanes) store_item(dl, AosCons{dl.constraints}, slot);
+  for (int slot = lane; slot < n_constraints; slot += kBatchLanes) store_item(dl, blocked_cons(dl), slot);
   for (int i = lane; i < wd.n_awake; i += kBatchLanes) finalize_item(dl, i, p);
   wave_sync();
   for (int i = lane; i < wd.n_awake; i += kBatchLanes) dg.states[i] = s_states[i];
@@ -916,7 +918,8 @@ void BatchStepper::upload() {
     d.color_off = (int32_t)color_total;
     d.bits_off = (int32_t)bits_total;
     d.pair_off = (int32_t)pair_total;
-    contact_total += d.contact_cap;
+    // each world's records start on a block of kRecBlock slots (the blocked record layout)
+    contact_total += (d.contact_cap + kRecBlock - 1) / kRecBlock * kRecBlock;
     overflow_total += d.overflow_cap;
     color_total += (int64_t)kOverflowColor * d.color_cap;
     bits_total += (int64_t)kOverflowColor * std::max(1, d.bits_words);

@@ -239,20 +239,9 @@ static_assert(sizeof(Constraint) % 16 == 0, "constraint records move in 16-byte 
 static_assert(offsetof(Constraint, index_a) == 4 && offsetof(Constraint, index_b) == 8,
               "body indices sit in a record's first piece");
 
-// Records in slot order (the batched stepper).
-struct AosCons {
-  Constraint* p;
-  __device__ __forceinline__ Constraint load(int s) const { return p[s]; }
-  __device__ __forceinline__ void store(int s, const Constraint& c) const { p[s] = c; }
-  __device__ __forceinline__ int2 indices(int s) const { return make_int2(p[s].index_a, p[s].index_b); }
-  template <class T>
-  __device__ __forceinline__ void put(int s, size_t off, T v) const {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(p + s) + off) = v;
-  }
-};
-
-// Records of the single world in blocks of kRecBlock slots (nsl_types.h), piece-major inside a
-// block: piece j of slot s at p[((s / kRecBlock) * kRecPieces + j) * kRecBlock + s % kRecBlock].
+// Constraint records (the single world's, and each batched world's segment, which starts on a
+// block) in blocks of kRecBlock slots (nsl_types.h), piece-major inside a block: piece j of
+// slot s at p[((s / kRecBlock) * kRecPieces + j) * kRecBlock + s % kRecBlock].
 // A wavefront touching consecutive slots reads or writes one contiguous KiB per piece (a record
 // in slot order would spread each such access over 64 cache lines). Capacity: whole blocks.
 struct BlockedCons {
@@ -926,13 +915,13 @@ __device__ __forceinline__ void warm_start_record(const DeviceWorld& d, const Co
 }
 __device__ __forceinline__ void push_record(const DeviceWorld& d, int slot, const StepParams& p, const Constraint& c) {
   BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
-  push_core(c, con_out(AosCons{d.constraints}, slot), p, ba, bb);
+  push_core(c, con_out(blocked_cons(d), slot), p, ba, bb);
   store_state(d, c.index_a, ba);
   store_state(d, c.index_b, bb);
 }
 __device__ __forceinline__ void relax_record(const DeviceWorld& d, int slot, const StepParams& p, const Constraint& c) {
   BodyState ba = load_state(d, c.index_a), bb = load_state(d, c.index_b);
-  relax_core(c, con_out(AosCons{d.constraints}, slot), p, ba, bb);
+  relax_core(c, con_out(blocked_cons(d), slot), p, ba, bb);
   store_state(d, c.index_a, ba);
   store_state(d, c.index_b, bb);
 }
